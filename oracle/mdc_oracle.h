@@ -1,0 +1,117 @@
+/*
+ * mdc_oracle.h -- CPU restatement of mdc's classical-MD force path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the parity checker for the
+ * MI355X engine: only tests/, __graft_entry__.smoke() and bench.py's CPU
+ * baseline leg may load it.  No product code links or calls it.
+ *
+ * It restates, function by function, the single-threaded ANSI-C path of
+ * mdc (KazumeNishidate/Molecular-dynamics-in-C, mdc/program):
+ *
+ *   orc_erfcc              ext.c:73-104       Mori's erfc
+ *   orc_unit_converter     init.c:25-80       derived parameters
+ *   orc_potential_nacl     control.c:85-181   Tosi-Fumi NaCl
+ *   orc_potential_kcl      (not in mdc)       Tosi-Fumi KCl, same functional form
+ *   orc_set_loc            control.c:46-82    rock-salt lattice + species
+ *   orc_mk_table           control.c:183-248  erfc+BMH pair table
+ *   orc_clear_foc          init.c:179-190
+ *   orc_real_space         real.c:15-84       table mode, all pairs i<j
+ *   orc_real_space_analytic pr-mdc/program/ewald.c:11-68 (+ BMH of control.c:219-240)
+ *   orc_reciprocal_space   rcprcl.c:11-97
+ *   orc_reciprocal_space3  rcprcl.c:108-133
+ *   orc_calc_foc           rv.c:12-18
+ *   orc_next_v / next_r    rv.c:33-66
+ *   orc_calc_kin           rv.c:68-86
+ *   orc_moment_correction  rv.c:88-116
+ *   orc_calc_press         pt.c:8-21
+ *   orc_nve                main.c:26-33 without thermostat and I/O
+ *
+ * Differences from the reference that do not change results on the same
+ * inputs: `int` ion counts (mdc uses `short`, which caps N at 32767), an
+ * explicit box struct instead of globals, and the reciprocal-space energy
+ * also kept separately so that a consistent (reciprocal/4) total can be
+ * reported next to the reference's own `pot`.
+ *
+ * Parity status per function is recorded in oracle/README (pinned by the
+ * compiled reference in oracle/_ref, closed forms, or invariants).
+ */
+#ifndef MDC_ORACLE_H
+#define MDC_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Born-Mayer-Huggins parameters for a two-species salt.
+ * Pair type index t: 0 = (0,0), 1 = (0,1), 2 = (1,1). */
+typedef struct {
+  double Z[2];      /* effective charges, sqrt(kk) units          */
+  double m[2];      /* masses [m_u]                                */
+  double b;         /* BMH prefactor [1e-18 J]                     */
+  double rho;       /* BMH softness [A]                            */
+  double c[3];      /* Pauling factors per pair type               */
+  double sigma[3];  /* sigma_i + sigma_j per pair type [A]         */
+  double C[3];      /* dipole-dipole dispersion [1e-18 J A^6]      */
+  double D[3];      /* dipole-quadrupole dispersion [1e-18 J A^8]  */
+} orc_potential;
+
+typedef struct {
+  /* ---- parameters (orc_unit_converter fills the derived ones) ---- */
+  int     N;
+  int     nx, ny, nz;       /* conventional cells                   */
+  double  Ax, Ay, Az;       /* lattice constants [A]                */
+  double  Lx, Ly, Lz;       /* box [A]                              */
+  double  a1;               /* Ewald alpha [1/A]; 0 or >10 skips    */
+  double  p2a2, a3;         /* pi^2/alpha^2, alpha/sqrt(pi)         */
+  int     hm, hm_sqrt;      /* integer |h|^2 cutoff and its root    */
+  double  radius, radius2;  /* real-space cutoff and its square     */
+  double  delta_time_fs;
+  double  dt, dt2;
+  int     table_division;
+  double  kB, kk, e2t;
+  int     real_mode;        /* 0 = table (mdc), 1 = analytic (pr-mdc) */
+  orc_potential pp;
+  int     lookup[4];
+  /* ---- arrays (caller-owned) ---- */
+  double *pe1r1, *fe1r1;    /* [3*(table_division+1)]               */
+  double *rx, *ry, *rz, *vx, *vy, *vz;
+  double *fx, *fy, *fz, *fx0, *fy0, *fz0;
+  int    *ion;              /* species 0/1                          */
+  /* ---- scalars ---- */
+  int     step;
+  double  pot;              /* as mdc accumulates it                */
+  double  pot_real, pot_recip, pot_self;  /* pieces of pot          */
+  double  virX, virY, virZ;
+  double  kin, kinX, kinY, kinZ;
+  double  presX, presY, presZ, pres;
+} orc_box;
+
+double orc_erfcc(double x);
+void   orc_potential_nacl(double kk, orc_potential *pp);
+void   orc_potential_kcl(double kk, orc_potential *pp);
+void   orc_unit_converter(orc_box *b);
+void   orc_set_loc(orc_box *b);
+void   orc_mk_table(orc_box *b);
+void   orc_clear_foc(orc_box *b);
+void   orc_real_space(orc_box *b);
+void   orc_real_space_analytic(orc_box *b);
+void   orc_reciprocal_space(orc_box *b);
+void   orc_reciprocal_space3(orc_box *b);
+void   orc_calc_foc(orc_box *b);
+void   orc_next_v(orc_box *b);
+void   orc_next_r(orc_box *b);
+void   orc_next_rv_verlet(orc_box *b);
+void   orc_calc_kin(orc_box *b);
+void   orc_moment_correction(orc_box *b);
+void   orc_calc_press(orc_box *b);
+/* NVE driver: `steps` iterations of (next_rv_verlet, calc_kin, calc_press).
+ * If etot != NULL, etot[s] = kin + pot_real + pot_recip/4 + pot_self after
+ * step s (the energy whose gradient the forces are). */
+void   orc_nve(orc_box *b, int steps, double *etot);
+/* number of half-sphere k vectors that reciprocal_space visits */
+int    orc_count_kvectors(int hm);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,263 @@
+"""ctypes access to the parity checker (oracle/liboracle.so) and, where it
+was built, the compiled mdc reference harness (oracle/_ref/libmdcref.so).
+
+Test infrastructure: imported by tests/, __graft_entry__.smoke() and the
+CPU-baseline leg of bench.py only.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_SO = os.path.join(ROOT, "oracle", "liboracle.so")
+REF_SO = os.path.join(ROOT, "oracle", "_ref", "libmdcref.so")
+
+KB = 1.380658e-5          # Boltzmann constant [1e-18 J / K]
+KK = 2.307079556          # e^2 / (4 pi eps0) [1e-18 J A]
+SECD_2_FS = 4.074972637944947
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+
+
+class OrcPotential(C.Structure):
+    _fields_ = [("Z", C.c_double * 2), ("m", C.c_double * 2), ("b", C.c_double),
+                ("rho", C.c_double), ("c", C.c_double * 3), ("sigma", C.c_double * 3),
+                ("C", C.c_double * 3), ("D", C.c_double * 3)]
+
+
+class OrcBox(C.Structure):
+    _fields_ = [
+        ("N", C.c_int), ("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int),
+        ("Ax", C.c_double), ("Ay", C.c_double), ("Az", C.c_double),
+        ("Lx", C.c_double), ("Ly", C.c_double), ("Lz", C.c_double),
+        ("a1", C.c_double), ("p2a2", C.c_double), ("a3", C.c_double),
+        ("hm", C.c_int), ("hm_sqrt", C.c_int),
+        ("radius", C.c_double), ("radius2", C.c_double),
+        ("delta_time_fs", C.c_double), ("dt", C.c_double), ("dt2", C.c_double),
+        ("table_division", C.c_int),
+        ("kB", C.c_double), ("kk", C.c_double), ("e2t", C.c_double),
+        ("real_mode", C.c_int),
+        ("pp", OrcPotential),
+        ("lookup", C.c_int * 4),
+        ("pe1r1", _dp), ("fe1r1", _dp),
+        ("rx", _dp), ("ry", _dp), ("rz", _dp), ("vx", _dp), ("vy", _dp), ("vz", _dp),
+        ("fx", _dp), ("fy", _dp), ("fz", _dp), ("fx0", _dp), ("fy0", _dp), ("fz0", _dp),
+        ("ion", _ip),
+        ("step", C.c_int),
+        ("pot", C.c_double), ("pot_real", C.c_double), ("pot_recip", C.c_double),
+        ("pot_self", C.c_double),
+        ("virX", C.c_double), ("virY", C.c_double), ("virZ", C.c_double),
+        ("kin", C.c_double), ("kinX", C.c_double), ("kinY", C.c_double), ("kinZ", C.c_double),
+        ("presX", C.c_double), ("presY", C.c_double), ("presZ", C.c_double), ("pres", C.c_double),
+    ]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(ORACLE_SO):
+            raise RuntimeError("oracle not built: run `make -C oracle` (or __graft_entry__.build())")
+        L = C.CDLL(ORACLE_SO)
+        bp = C.POINTER(OrcBox)
+        for name in ("orc_unit_converter", "orc_set_loc", "orc_mk_table", "orc_clear_foc",
+                     "orc_real_space", "orc_real_space_analytic", "orc_reciprocal_space",
+                     "orc_reciprocal_space3", "orc_calc_foc", "orc_next_v", "orc_next_r",
+                     "orc_next_rv_verlet", "orc_calc_kin", "orc_moment_correction",
+                     "orc_calc_press"):
+            getattr(L, name).argtypes = [bp]
+            getattr(L, name).restype = None
+        L.orc_nve.argtypes = [bp, C.c_int, _dp]
+        L.orc_nve.restype = None
+        L.orc_erfcc.argtypes = [C.c_double]
+        L.orc_erfcc.restype = C.c_double
+        L.orc_potential_nacl.argtypes = [C.c_double, C.POINTER(OrcPotential)]
+        L.orc_potential_kcl.argtypes = [C.c_double, C.POINTER(OrcPotential)]
+        L.orc_count_kvectors.argtypes = [C.c_int]
+        L.orc_count_kvectors.restype = C.c_int
+        _lib = L
+    return _lib
+
+
+def _ptr(a, ct=C.c_double):
+    return a.ctypes.data_as(C.POINTER(ct))
+
+
+class Oracle:
+    """One box of the CPU restatement.  Arrays live in numpy and are
+    passed by pointer, so self.r / self.v / self.f are live views."""
+
+    def __init__(self, nx, ny, nz, a, alpha, hm, radius, dt_fs=2.0, salt="NaCl",
+                 real_mode=0):
+        L = lib()
+        self.b = OrcBox()
+        b = self.b
+        b.nx, b.ny, b.nz = nx, ny, nz
+        b.Ax = b.Ay = b.Az = a
+        b.a1, b.hm, b.radius, b.delta_time_fs = alpha, hm, radius, dt_fs
+        b.real_mode = real_mode
+        L.orc_unit_converter(C.byref(b))
+        if salt == "NaCl":
+            L.orc_potential_nacl(b.kk, C.byref(b.pp))
+        elif salt == "KCl":
+            L.orc_potential_kcl(b.kk, C.byref(b.pp))
+        else:
+            raise ValueError(salt)
+        n = b.N
+        nb = 3 * (b.table_division + 1)
+        self.pe = np.zeros(nb)
+        self.fe = np.zeros(nb)
+        self.r = np.zeros((3, n))
+        self.v = np.zeros((3, n))
+        self.f = np.zeros((3, n))
+        self.f0 = np.zeros((3, n))
+        self.ion = np.zeros(n, dtype=np.int32)
+        b.pe1r1, b.fe1r1 = _ptr(self.pe), _ptr(self.fe)
+        b.rx, b.ry, b.rz = (_ptr(self.r[k]) for k in range(3))
+        b.vx, b.vy, b.vz = (_ptr(self.v[k]) for k in range(3))
+        b.fx, b.fy, b.fz = (_ptr(self.f[k]) for k in range(3))
+        b.fx0, b.fy0, b.fz0 = (_ptr(self.f0[k]) for k in range(3))
+        b.ion = _ptr(self.ion, C.c_int)
+        L.orc_set_loc(C.byref(b))
+        L.orc_mk_table(C.byref(b))
+        b.step = 1
+
+    @property
+    def N(self):
+        return self.b.N
+
+    @property
+    def L(self):
+        return np.array([self.b.Lx, self.b.Ly, self.b.Lz])
+
+    def set_state(self, r=None, v=None, f=None):
+        if r is not None:
+            self.r[...] = r
+        if v is not None:
+            self.v[...] = v
+        if f is not None:
+            self.f[...] = f
+
+    def call(self, name):
+        getattr(lib(), "orc_" + name)(C.byref(self.b))
+
+    def calc_foc(self):
+        self.call("calc_foc")
+
+    def nve(self, steps, energies=False):
+        e = np.zeros(steps) if energies else None
+        lib().orc_nve(C.byref(self.b), steps, _ptr(e) if energies else None)
+        return e
+
+    def scalars(self):
+        b = self.b
+        return dict(pot=b.pot, pot_real=b.pot_real, pot_recip=b.pot_recip, pot_self=b.pot_self,
+                    virX=b.virX, virY=b.virY, virZ=b.virZ, kin=b.kin, kinX=b.kinX,
+                    kinY=b.kinY, kinZ=b.kinZ, presX=b.presX, presY=b.presY,
+                    presZ=b.presZ, pres=b.pres)
+
+    def masses(self):
+        return np.array([self.b.pp.m[0], self.b.pp.m[1]])[self.ion]
+
+    def charges(self):
+        return np.array([self.b.pp.Z[0], self.b.pp.Z[1]])[self.ion]
+
+
+def maxwell_velocities(masses, temp_k, seed):
+    """Seeded Maxwell-Boltzmann velocities [A / time unit] with the centre
+    of mass momentum removed the way mdc's moment_correction does."""
+    rng = np.random.default_rng(seed)
+    m = np.asarray(masses, dtype=np.float64)
+    sig = np.sqrt(KB * temp_k / m)
+    v = rng.standard_normal((3, m.size)) * sig
+    p = (m * v).sum(axis=1) / m.size
+    v -= p[:, None] / m
+    return v
+
+
+def jitter_positions(r, L, amp, seed):
+    rng = np.random.default_rng(seed)
+    out = r + rng.uniform(-amp, amp, size=r.shape)
+    L = np.asarray(L)[:, None]
+    return np.mod(out, L)
+
+
+# ---------------------------------------------------------------- reference
+_ref = None
+
+
+def ref_available():
+    return os.path.exists(REF_SO)
+
+
+def ref():
+    global _ref
+    if _ref is None:
+        R = C.CDLL(REF_SO)
+        R.ref_setup.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_int,
+                                C.c_double, C.c_double]
+        R.ref_setup.restype = C.c_int
+        for n in ("ref_set_positions", "ref_set_velocities", "ref_set_forces"):
+            getattr(R, n).argtypes = [_dp, _dp, _dp]
+        R.ref_get_state.argtypes = [_dp]
+        R.ref_get_species.argtypes = [_ip]
+        R.ref_get_scalars.argtypes = [_dp]
+        R.ref_get_table.argtypes = [_dp, _dp]
+        R.ref_erfcc.argtypes = [C.c_double]
+        R.ref_erfcc.restype = C.c_double
+        R.ref_run.argtypes = [C.c_int, C.c_int]
+        _ref = R
+    return _ref
+
+
+class Reference:
+    """The compiled mdc sources (one global system per process)."""
+
+    def __init__(self, nx, ny, nz, a, alpha, hm, radius, dt_fs=2.0):
+        self.N = ref().ref_setup(nx, ny, nz, a, alpha, hm, radius, dt_fs)
+
+    def set_state(self, r=None, v=None, f=None):
+        R = ref()
+        if r is not None:
+            r = np.ascontiguousarray(r, dtype=np.float64)
+            R.ref_set_positions(_ptr(r[0]), _ptr(r[1]), _ptr(r[2]))
+        if v is not None:
+            v = np.ascontiguousarray(v, dtype=np.float64)
+            R.ref_set_velocities(_ptr(v[0]), _ptr(v[1]), _ptr(v[2]))
+        if f is not None:
+            f = np.ascontiguousarray(f, dtype=np.float64)
+            R.ref_set_forces(_ptr(f[0]), _ptr(f[1]), _ptr(f[2]))
+
+    def state(self):
+        out = np.zeros(12 * self.N)
+        ref().ref_get_state(_ptr(out))
+        out = out.reshape(4, 3, self.N)
+        return dict(r=out[0], v=out[1], f=out[2], f0=out[3])
+
+    def species(self):
+        out = np.zeros(self.N, dtype=np.int32)
+        ref().ref_get_species(_ptr(out, C.c_int))
+        return out
+
+    def scalars(self):
+        out = np.zeros(12)
+        ref().ref_get_scalars(_ptr(out))
+        keys = ("pot", "virX", "virY", "virZ", "kin", "kinX", "kinY", "kinZ",
+                "presX", "presY", "presZ", "pres")
+        return dict(zip(keys, out))
+
+    def table(self):
+        n = 3 * (ref().ref_table_division() + 1)
+        pe, fe = np.zeros(n), np.zeros(n)
+        ref().ref_get_table(_ptr(pe), _ptr(fe))
+        return pe, fe
+
+    def call(self, name):
+        getattr(ref(), "ref_" + name)()
+
+    def run(self, first_step, nsteps):
+        ref().ref_run(first_step, nsteps)
