@@ -1,0 +1,182 @@
+/*
+ * mdgpu.h -- handle-based C ABI of libmdgpu: device-resident boxes,
+ * batched stepping of many boxes per launch, and a serving loop.
+ *
+ * mdc itself has one global system per process (mdc/program/md.h:197-202)
+ * and no API beyond `void f(void)` routines; mdc_abi.h keeps that surface.
+ * This header is the surface mdc's singleton cannot express: any number of
+ * independent boxes whose state stays in HBM between steps.  Each step
+ * performs, per box, what mdc's newton() loop does per step
+ * (mdc/program/main.c:26-33): calc_foc (rv.c:12-18) -> next_v -> next_r
+ * (rv.c:33-66) -> calc_kin (rv.c:68-86) -> calc_press (pt.c:8-21).
+ *
+ * Units are mdc's (mdc/manual/units.IEMD): length A, energy 1e-18 J,
+ * mass m_u, time 4.074972637944947 fs.  Arrays are SoA: r = x[N], y[N], z[N].
+ * Every function returns MDG_OK or an MDG_E* code; none exits the process.
+ */
+#ifndef MDGPU_H
+#define MDGPU_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MDG_OK          0
+#define MDG_E_NODEVICE  1   /* no HIP device / library built without one  */
+#define MDG_E_HIP       2   /* a HIP runtime call failed                   */
+#define MDG_E_ARG       3   /* invalid argument                            */
+#define MDG_E_NOMEM     4   /* device allocation failed                    */
+#define MDG_E_BUDGET    5   /* request exceeds the HBM budget              */
+#define MDG_E_STATE     6   /* non-finite or out-of-box state detected     */
+#define MDG_E_RANGE     7   /* pair inside the 0.5 A table start, or rc > L/2 */
+#define MDG_E_TIMEOUT   8   /* a bounded device wait expired               */
+
+#define MDG_FP64     0      /* plain path: all arithmetic FP64             */
+#define MDG_MIXED    1      /* FP32 pair/k-space arithmetic, FP64 state and sums */
+
+#define MDG_TABLE    0      /* mdc's tabulated erfc+BMH (0.001 A bins)     */
+#define MDG_ANALYTIC 1      /* per-pair evaluation (pr-mdc ewald1 style)   */
+
+#define MDG_CELLS_AUTO  -1
+#define MDG_ALL_PAIRS    0
+#define MDG_CELL_LIST    1
+
+/* Born-Mayer-Huggins salt: pair type 0 = (0,0), 1 = (0,1), 2 = (1,1). */
+typedef struct {
+  double Z[2];       /* effective charges (sqrt(kk) = 1 e)          */
+  double m[2];       /* masses [m_u]                                 */
+  double b, rho;     /* BMH prefactor [1e-18 J], softness [A]        */
+  double c[3];       /* Pauling factors                              */
+  double sigma[3];   /* sigma_i + sigma_j [A]                        */
+  double C[3], D[3]; /* dispersion [1e-18 J A^6], [1e-18 J A^8]      */
+} mdg_potential;
+
+typedef struct {
+  int    nx, ny, nz;    /* conventional rock-salt cells, N = 8 nx ny nz   */
+  double ax, ay, az;    /* lattice constants [A]; L = a * n              */
+  double alpha;         /* Ewald alpha [1/A]; 0 or > 10 turns Ewald off  */
+  int    hm;            /* reciprocal cutoff: 0 < hx^2+hy^2+hz^2 <= hm   */
+  double rc;            /* real-space cutoff [A]                         */
+  double dt_fs;         /* time step [fs]                                */
+  int    real_mode;     /* MDG_TABLE or MDG_ANALYTIC                     */
+  int    cells;         /* MDG_CELLS_AUTO, MDG_ALL_PAIRS or MDG_CELL_LIST */
+  mdg_potential pot;
+} mdg_box_params;
+
+typedef struct {
+  double pot;           /* as mdc accumulates sys.pot: real + recip + self */
+  double pot_real;      /* real-space erfc + BMH                           */
+  double pot_recip;     /* reciprocal term as mdc writes it (4x, see below) */
+  double pot_self;      /* -alpha/sqrt(pi) sum Z^2                         */
+  double etot;          /* kin + pot_real + pot_recip/4 + pot_self: the
+                           energy the forces conserve (mdc doubles both
+                           structure-factor sums before squaring them,
+                           rcprcl.c:77-82)                                  */
+  double vir[3];        /* diagonal real-space virial (pt.c uses it)       */
+  double kin, kin_xyz[3];
+  double pres, pres_xyz[3];
+  long long step;       /* completed steps                                 */
+} mdg_scalars;
+
+typedef struct mdg_batch mdg_batch;
+
+/* ---------------------------------------------------------------- setup */
+int  mdg_device_count(int *n);
+/* Tosi-Fumi NaCl (mdc/program/control.c:85-181) and KCl (same form). */
+int  mdg_potential_nacl(mdg_potential *p);
+int  mdg_potential_kcl(mdg_potential *p);
+/* Rock-salt positions and species in the order of mdc's set_loc /
+ * identify_ion (control.c:46-82).  r: 3N, species: N. */
+int  mdg_rocksalt(int nx, int ny, int nz, double ax, double ay, double az,
+                  double *r, int *species);
+/* Seeded Maxwell-Boltzmann velocities at temp_k with the centre-of-mass
+ * momentum removed as mdc's moment_correction does (rv.c:88-116). */
+int  mdg_maxwell(unsigned long long seed, double temp_k, int n, const int *species,
+                 const mdg_potential *p, double *v);
+/* Ewald split for a target relative error eps: alpha = s/rc and
+ * hm = floor((s alpha L_max / pi)^2) with s = sqrt(-ln eps). */
+int  mdg_ewald_params(double rc, double l_max, double eps, double *alpha, int *hm);
+
+/* ---------------------------------------------------------------- batch */
+/* nbox independent boxes in one precision mode.  Each box's arithmetic
+ * depends on that box alone (tiling, reduction order), so a box steps to
+ * identical bits alone or in any mix. */
+int  mdg_batch_create(int nbox, const mdg_box_params *params, int precision,
+                      mdg_batch **out);
+int  mdg_batch_destroy(mdg_batch *b);
+int  mdg_batch_n_ions(const mdg_batch *b, int box);
+/* device bytes held by box `box` (all its arrays), for HBM budgeting */
+size_t mdg_batch_box_bytes(const mdg_batch *b, int box);
+/* r: 3N SoA [A], v: 3N or NULL (keeps), species: N or NULL (keeps) */
+int  mdg_batch_set_state(mdg_batch *b, int box, const double *r, const double *v,
+                         const int *species);
+/* f0 is the previous step's force (mdc's fx0..); any pointer may be NULL */
+int  mdg_batch_get_state(mdg_batch *b, int box, double *r, double *v, double *f,
+                         double *f0);
+/* mdc's calc_foc for every box: f0 <- f, f <- real + reciprocal forces,
+ * pot/vir/self as mdc writes them (no integration). */
+int  mdg_batch_forces(mdg_batch *b);
+/* nsteps velocity-Verlet steps (next_rv_verlet + calc_kin + calc_press) on
+ * every box; the step sequence is captured once in a hipGraph and replayed */
+int  mdg_batch_step(mdg_batch *b, int nsteps);
+int  mdg_batch_scalars(mdg_batch *b, int box, mdg_scalars *out);
+int  mdg_batch_sync(mdg_batch *b);
+/* Kernel timing over the next `nsteps` steps, measured with hipEvents on
+ * the batch's stream (the graph is bypassed while timing).  names/ms_avg:
+ * up to `cap` entries, returns the count in *n. */
+int  mdg_batch_profile(mdg_batch *b, int nsteps, int cap, char names[][32],
+                       double *ms_avg, int *n);
+/* Algorithmic work of one step of box `box`: accepted real-space pairs
+ * (unordered), k vectors, and ions; pairs are counted on the device at the
+ * last force evaluation. */
+int  mdg_batch_work(mdg_batch *b, int box, long long *pairs, int *nk, int *n_ions);
+
+const char *mdg_error_string(int code);
+
+/* -------------------------------------------------------------- serving */
+typedef struct {
+  double arrival_s;     /* seconds after the start of the stream          */
+  int    nx, ny, nz;    /* NaCl box in conventional cells                  */
+  double temp_k;
+  int    steps;
+  unsigned long long seed;
+} mdg_job;
+
+typedef struct {
+  size_t hbm_budget;    /* bytes the resident jobs may hold               */
+  int    max_slots;     /* jobs stepped together at most                   */
+  int    precision;     /* MDG_FP64 or MDG_MIXED                           */
+  int    real_mode;
+  double eps;           /* Ewald target relative error                     */
+  double rc_max;        /* real-space cutoff cap [A]                       */
+  double dt_fs;
+  int    one_at_a_time; /* 1: run jobs alone in arrival order (baseline)   */
+  int    realtime;      /* 1: jobs become visible at their arrival times    */
+  int    steps_per_tick;/* steps per batched launch between admissions     */
+} mdg_serve_config;
+
+typedef struct {
+  double admit_s, finish_s, latency_s;   /* latency = finish - arrival     */
+  double etot_first, etot_last;          /* energy after first / last step */
+  double final_checksum;                 /* sum of final positions          */
+  int    status;                         /* MDG_OK or the job's error       */
+} mdg_job_result;
+
+typedef struct {
+  double wall_s;
+  double ion_steps;
+  double ion_steps_per_s;
+  double p50_latency_s, p99_latency_s, mean_latency_s;
+  int    max_resident, ticks, rejected;
+  size_t peak_bytes;
+} mdg_serve_stats;
+
+int  mdg_serve(const mdg_job *jobs, int njobs, const mdg_serve_config *cfg,
+               mdg_job_result *results, mdg_serve_stats *stats);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

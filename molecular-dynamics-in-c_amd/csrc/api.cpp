@@ -1,0 +1,268 @@
+// api.cpp -- the handle-based C ABI of libmdgpu (include/mdgpu.h).
+#include <math.h>
+#include <string.h>
+
+#include <string>
+#include <vector>
+
+#include "engine.h"
+#include "host_setup.h"
+#include "mdgpu.h"
+
+using namespace mdg;
+
+// mdc's constants (mdc/program/md.h:8-16)
+static const double kSqrtPi = 1.77245385090551602731;
+static const double kPiPi = 9.869604401089358619;
+static const double kSecd2Fs = 4.074972637944947;
+
+struct mdg_batch {
+  int precision = MDG_FP64;
+  int real_mode = MDG_TABLE;
+  std::vector<Box*> boxes;
+  BoxSet* set = nullptr;
+  hipStream_t stream = nullptr;
+};
+
+namespace mdg {
+
+// BoxSpec of a rock-salt box described by public parameters, with mdc's
+// unit conversion (mdc/program/init.c:25-80).  The table, when needed, is
+// built into `pe`/`fe`.
+int spec_from_params(const mdg_box_params& p, BoxSpec* s, std::vector<double>* pe,
+                     std::vector<double>* fe) {
+  if (p.nx <= 0 || p.ny <= 0 || p.nz <= 0 || !(p.ax > 0) || !(p.ay > 0) || !(p.az > 0) ||
+      !(p.rc > 0) || !(p.dt_fs > 0) || p.hm < 0)
+    return MDG_E_ARG;
+  if (p.real_mode != MDG_TABLE && p.real_mode != MDG_ANALYTIC) return MDG_E_ARG;
+  *s = BoxSpec();
+  s->N = 8 * p.nx * p.ny * p.nz;
+  s->L[0] = p.ax * (double)p.nx;
+  s->L[1] = p.ay * (double)p.ny;
+  s->L[2] = p.az * (double)p.nz;
+  s->alpha = p.alpha;
+  if (p.alpha > 0) {
+    s->p2a2 = kPiPi / (p.alpha * p.alpha);
+    s->a3 = p.alpha / kSqrtPi;
+  }
+  s->hm = p.hm;
+  s->rc = p.rc;
+  s->rc2 = p.rc * p.rc;
+  s->dt = p.dt_fs / kSecd2Fs;
+  s->dt2 = s->dt * s->dt;
+  s->real_mode = p.real_mode;
+  s->cells = p.cells;
+  s->kinds = 2;
+  for (int k = 0; k < 2; ++k) {
+    s->Z[k] = p.pot.Z[k];
+    s->m[k] = p.pot.m[k];
+  }
+  s->pot = p.pot;
+  if (p.real_mode == MDG_TABLE) {
+    if (p.rc <= 0.5) return MDG_E_ARG;
+    const int td = (int)((p.rc - 0.5) * 1000.0) + 1;  // init.c:65
+    const int nb = td + 1;
+    pe->assign(3 * (size_t)nb, 0.0);
+    fe->assign(3 * (size_t)nb, 0.0);
+    host_mk_table(p.pot, p.alpha, s->a3, nb, pe->data(), fe->data());
+    s->pe = pe->data();
+    s->fe = fe->data();
+    s->table_nb = nb;
+    s->table_ntypes = 3;
+  }
+  return MDG_OK;
+}
+
+}  // namespace mdg
+
+extern "C" {
+
+const char* mdg_error_string(int code) {
+  switch (code) {
+    case MDG_OK: return "ok";
+    case MDG_E_NODEVICE: return "no HIP device";
+    case MDG_E_HIP: return "HIP runtime error";
+    case MDG_E_ARG: return "invalid argument";
+    case MDG_E_NOMEM: return "device allocation failed";
+    case MDG_E_BUDGET: return "exceeds HBM budget";
+    case MDG_E_STATE: return "non-finite or out-of-box state";
+    case MDG_E_RANGE: return "pair inside table start or cutoff/box mismatch";
+    case MDG_E_TIMEOUT: return "bounded device wait expired";
+    default: return "unknown error";
+  }
+}
+
+int mdg_device_count(int* n) {
+  int c = 0;
+  if (hipGetDeviceCount(&c) != hipSuccess) {
+    (void)hipGetLastError();
+    if (n) *n = 0;
+    return MDG_E_NODEVICE;
+  }
+  if (n) *n = c;
+  return c > 0 ? MDG_OK : MDG_E_NODEVICE;
+}
+
+int mdg_batch_destroy(mdg_batch* b) {
+  if (!b) return MDG_OK;
+  delete b->set;
+  for (Box* x : b->boxes) box_destroy(x);
+  if (b->stream) (void)hipStreamDestroy(b->stream);
+  delete b;
+  return MDG_OK;
+}
+
+int mdg_batch_create(int nbox, const mdg_box_params* params, int precision, mdg_batch** out) {
+  if (!out || nbox <= 0 || !params) return MDG_E_ARG;
+  if (precision != MDG_FP64 && precision != MDG_MIXED) return MDG_E_ARG;
+  *out = nullptr;
+  int ndev = 0;
+  if (mdg_device_count(&ndev) != MDG_OK) {
+    fprintf(stderr, "libmdgpu: no HIP device available\n");
+    return MDG_E_NODEVICE;
+  }
+  const int rmode = params[0].real_mode;
+  for (int k = 1; k < nbox; ++k)
+    if (params[k].real_mode != rmode) return MDG_E_ARG;  // one pair kernel per launch
+  mdg_batch* b = new mdg_batch();
+  b->precision = precision;
+  b->real_mode = rmode;
+  if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete b;
+    return MDG_E_HIP;
+  }
+  for (int k = 0; k < nbox; ++k) {
+    BoxSpec spec;
+    std::vector<double> pe, fe;
+    int rc = spec_from_params(params[k], &spec, &pe, &fe);
+    Box* box = nullptr;
+    std::string why;
+    if (rc == MDG_OK) rc = box_create(spec, precision, &box, &why);
+    if (rc != MDG_OK) {
+      if (!why.empty()) fprintf(stderr, "libmdgpu: box %d: %s\n", k, why.c_str());
+      mdg_batch_destroy(b);
+      return rc;
+    }
+    b->boxes.push_back(box);
+    // start from mdc's lattice at rest
+    const int N = spec.N;
+    std::vector<double> r(3 * (size_t)N);
+    std::vector<int> sp(N);
+    mdg_rocksalt(params[k].nx, params[k].ny, params[k].nz, params[k].ax, params[k].ay,
+                 params[k].az, r.data(), sp.data());
+    rc = box_upload_state(box, r.data(), nullptr, sp.data(), b->stream);
+    if (rc != MDG_OK) {
+      mdg_batch_destroy(b);
+      return rc;
+    }
+  }
+  b->set = new BoxSet(precision, rmode);
+  int rc = b->set->set_boxes(b->boxes);
+  if (rc != MDG_OK) {
+    mdg_batch_destroy(b);
+    return rc;
+  }
+  *out = b;
+  return MDG_OK;
+}
+
+int mdg_batch_n_ions(const mdg_batch* b, int box) {
+  if (!b || box < 0 || box >= (int)b->boxes.size()) return -1;
+  return b->boxes[box]->desc.N;
+}
+
+size_t mdg_batch_box_bytes(const mdg_batch* b, int box) {
+  if (!b || box < 0 || box >= (int)b->boxes.size()) return 0;
+  return b->boxes[box]->bytes;
+}
+
+int mdg_batch_set_state(mdg_batch* b, int box, const double* r, const double* v,
+                        const int* species) {
+  if (!b || box < 0 || box >= (int)b->boxes.size()) return MDG_E_ARG;
+  return box_upload_state(b->boxes[box], r, v, species, b->stream);
+}
+
+int mdg_batch_get_state(mdg_batch* b, int box, double* r, double* v, double* f, double* f0) {
+  if (!b || box < 0 || box >= (int)b->boxes.size()) return MDG_E_ARG;
+  return box_download_state(b->boxes[box], r, v, f, f0, b->stream);
+}
+
+int mdg_batch_forces(mdg_batch* b) {
+  if (!b) return MDG_E_ARG;
+  int rc = b->set->sync_desc(b->stream);
+  if (rc != MDG_OK) return rc;
+  b->set->enqueue_step(b->stream, false);
+  MDG_HIP(hipGetLastError());
+  MDG_HIP(hipStreamSynchronize(b->stream));
+  return MDG_OK;
+}
+
+int mdg_batch_step(mdg_batch* b, int nsteps) {
+  if (!b || nsteps < 0) return MDG_E_ARG;
+  return b->set->step(nsteps, b->stream);
+}
+
+int mdg_batch_sync(mdg_batch* b) {
+  if (!b) return MDG_E_ARG;
+  MDG_HIP(hipStreamSynchronize(b->stream));
+  return MDG_OK;
+}
+
+static int err_code(int e) {
+  if (e & kErrState) return MDG_E_STATE;
+  if (e & kErrRange) return MDG_E_RANGE;
+  return MDG_OK;
+}
+
+int mdg_batch_scalars(mdg_batch* b, int box, mdg_scalars* out) {
+  if (!b || !out || box < 0 || box >= (int)b->boxes.size()) return MDG_E_ARG;
+  double S[kScalCount];
+  int rc = box_read_scalars(b->boxes[box], S, b->stream);
+  if (rc != MDG_OK) return rc;
+  int e = 0;
+  rc = box_read_error(b->boxes[box], &e, b->stream);
+  if (rc != MDG_OK) return rc;
+  out->pot = S[kScalPot];
+  out->pot_real = S[kScalPotReal];
+  out->pot_recip = S[kScalPotRecip];
+  out->pot_self = S[kScalPotSelf];
+  out->etot = S[kScalEtot];
+  out->vir[0] = S[kScalVirX]; out->vir[1] = S[kScalVirY]; out->vir[2] = S[kScalVirZ];
+  out->kin = S[kScalKin];
+  out->kin_xyz[0] = S[kScalKinX]; out->kin_xyz[1] = S[kScalKinY]; out->kin_xyz[2] = S[kScalKinZ];
+  out->pres = S[kScalPres];
+  out->pres_xyz[0] = S[kScalPresX]; out->pres_xyz[1] = S[kScalPresY];
+  out->pres_xyz[2] = S[kScalPresZ];
+  out->step = (long long)S[kScalStep];
+  return err_code(e);
+}
+
+int mdg_batch_profile(mdg_batch* b, int nsteps, int cap, char names[][32], double* ms_avg,
+                      int* n) {
+  if (!b || nsteps <= 0 || !names || !ms_avg || !n) return MDG_E_ARG;
+  std::vector<std::string> nm;
+  std::vector<double> t;
+  int rc = b->set->profile(nsteps, b->stream, &nm, &t);
+  if (rc != MDG_OK) return rc;
+  int k = 0;
+  for (; k < (int)nm.size() && k < cap; ++k) {
+    strncpy(names[k], nm[k].c_str(), 31);
+    names[k][31] = 0;
+    ms_avg[k] = t[k];
+  }
+  *n = k;
+  return MDG_OK;
+}
+
+int mdg_batch_work(mdg_batch* b, int box, long long* pairs, int* nk, int* n_ions) {
+  if (!b || box < 0 || box >= (int)b->boxes.size()) return MDG_E_ARG;
+  double S[kScalCount];
+  int rc = box_read_scalars(b->boxes[box], S, b->stream);
+  if (rc != MDG_OK) return rc;
+  if (pairs) *pairs = (long long)S[kScalPairs];
+  if (nk) *nk = b->boxes[box]->plan.ewald ? b->boxes[box]->plan.nk : 0;
+  if (n_ions) *n_ions = b->boxes[box]->desc.N;
+  return MDG_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,352 @@
+// dropin.cpp -- mdc's force-path routines with mdc's signatures
+// (include/mdc_abi.h), computing on the GPU over mdc's global objects.
+//
+// Per call the host arrays are authoritative: the routine uploads the
+// arrays it reads, runs the kernels, and writes back the arrays and
+// scalars mdc's routine writes.  The device box is rebuilt when the system
+// parameters or the pair table change.  Real space runs as an all-pairs
+// loop visiting partners in ascending index up to N = 8192, which
+// reproduces mdc's force accumulation order bit for bit.
+#include <math.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <string>
+#include <vector>
+
+#include "engine.h"
+#include "host_setup.h"
+#include "mdc_abi.h"
+#include "mdgpu.h"
+
+using namespace mdg;
+
+// The globals of mdc/program/md.h:197-202 and potential.h:28.  An mdc
+// executable defines its own (md.h defines them in the header); ELF
+// interposition then binds these references to the executable's objects.
+system_property sys;
+ion_property ion;
+calc_control ctl;
+huggins_mayer_potential_set hm;
+
+namespace {
+
+struct DropIn {
+  Box* box = nullptr;
+  BoxSet* set = nullptr;
+  hipStream_t stream = nullptr;
+  int status = MDG_OK;
+  bool reported = false;
+  // what the device box was built from
+  int N = 0, hmv = 0, kinds = 0, td = 0;
+  double L[3] = {0, 0, 0}, a1 = 0, a3 = 0, p2a2 = 0, radius2 = 0, dt = 0, dt2 = 0;
+  double Z[4] = {0, 0, 0, 0}, m[4] = {0, 0, 0, 0};
+  int lookup[16] = {0};
+  std::vector<double> pe, fe;
+  std::vector<int> species;
+  std::vector<double> buf;
+};
+
+DropIn g;
+
+void fail(int code, const char* what) {
+  if (g.status == MDG_OK) g.status = code;
+  if (!g.reported) {
+    fprintf(stderr, "libmdgpu drop-in: %s failed: %s\n", what, mdg_error_string(code));
+    g.reported = true;
+  }
+}
+
+void release() {
+  delete g.set;
+  g.set = nullptr;
+  box_destroy(g.box);
+  g.box = nullptr;
+}
+
+bool same_params() {
+  if (!g.box) return false;
+  const int kinds = ctl.kinds_of_ions;
+  const int td = sys.table_division;
+  if (g.N != sys.N || g.hmv != sys.hm || g.kinds != kinds || g.td != td) return false;
+  if (g.L[0] != sys.Lx || g.L[1] != sys.Ly || g.L[2] != sys.Lz || g.a1 != sys.a1 ||
+      g.a3 != sys.a3 || g.p2a2 != sys.p2a2 || g.radius2 != sys.radius2 || g.dt != sys.dt ||
+      g.dt2 != sys.dt2)
+    return false;
+  for (int k = 0; k < kinds; ++k)
+    if (g.Z[k] != ion.Z[k] || g.m[k] != ion.m[k]) return false;
+  for (int k = 0; k < kinds * kinds; ++k)
+    if (g.lookup[k] != sys.lookup[k]) return false;
+  const size_t nt = (size_t)(kinds * (kinds + 1) / 2) * (size_t)(td + 1);
+  if (g.pe.size() != nt || memcmp(g.pe.data(), sys.pe1r1, nt * sizeof(double)) != 0 ||
+      memcmp(g.fe.data(), sys.fe1r1, nt * sizeof(double)) != 0)
+    return false;
+  return true;
+}
+
+// Make the device box match the globals; upload species every call.
+bool ensure(const char* who) {
+  if (g.status != MDG_OK) return false;
+  if (!g.stream) {
+    int n = 0;
+    if (mdg_device_count(&n) != MDG_OK) {
+      fail(MDG_E_NODEVICE, who);
+      return false;
+    }
+    if (hipStreamCreateWithFlags(&g.stream, hipStreamNonBlocking) != hipSuccess) {
+      fail(MDG_E_HIP, who);
+      return false;
+    }
+  }
+  const int kinds = ctl.kinds_of_ions;
+  if (sys.N <= 0 || kinds < 1 || kinds > 4 || !sys.rx || !sys.pe1r1 || !sys.fe1r1 ||
+      !sys.lookup || !ion.Z || !ion.m || !sys.ion) {
+    fail(MDG_E_ARG, who);
+    return false;
+  }
+  if (!same_params()) {
+    release();
+    g.N = sys.N; g.hmv = sys.hm; g.kinds = kinds; g.td = sys.table_division;
+    g.L[0] = sys.Lx; g.L[1] = sys.Ly; g.L[2] = sys.Lz;
+    g.a1 = sys.a1; g.a3 = sys.a3; g.p2a2 = sys.p2a2; g.radius2 = sys.radius2;
+    g.dt = sys.dt; g.dt2 = sys.dt2;
+    for (int k = 0; k < kinds; ++k) { g.Z[k] = ion.Z[k]; g.m[k] = ion.m[k]; }
+    for (int k = 0; k < kinds * kinds; ++k) g.lookup[k] = sys.lookup[k];
+    const size_t nt = (size_t)(kinds * (kinds + 1) / 2) * (size_t)(g.td + 1);
+    g.pe.assign(sys.pe1r1, sys.pe1r1 + nt);
+    g.fe.assign(sys.fe1r1, sys.fe1r1 + nt);
+    BoxSpec s;
+    s.N = sys.N;
+    s.L[0] = sys.Lx; s.L[1] = sys.Ly; s.L[2] = sys.Lz;
+    s.alpha = sys.a1; s.a3 = sys.a3; s.p2a2 = sys.p2a2;
+    s.hm = sys.hm;
+    s.rc = sqrt(sys.radius2);
+    s.rc2 = sys.radius2;
+    s.dt = sys.dt; s.dt2 = sys.dt2;
+    s.real_mode = MDG_TABLE;
+    s.cells = sys.N <= 8192 ? MDG_ALL_PAIRS : MDG_CELLS_AUTO;
+    s.kinds = kinds;
+    for (int k = 0; k < kinds; ++k) { s.Z[k] = ion.Z[k]; s.m[k] = ion.m[k]; }
+    for (int k = 0; k < 16; ++k) s.lookup[k] = k < kinds * kinds ? sys.lookup[k] : 0;
+    s.pe = g.pe.data();
+    s.fe = g.fe.data();
+    s.table_nb = g.td + 1;
+    s.table_ntypes = kinds * (kinds + 1) / 2;
+    std::string why;
+    int rc = box_create(s, MDG_FP64, &g.box, &why);
+    if (rc != MDG_OK) {
+      if (!why.empty()) fprintf(stderr, "libmdgpu drop-in: %s\n", why.c_str());
+      fail(rc, who);
+      return false;
+    }
+    g.set = new BoxSet(MDG_FP64, MDG_TABLE);
+    rc = g.set->set_boxes({g.box});
+    if (rc != MDG_OK) {
+      fail(rc, who);
+      release();
+      return false;
+    }
+  }
+  g.species.resize(sys.N);
+  for (int i = 0; i < sys.N; ++i) g.species[i] = sys.ion[i];
+  int rc = box_upload_state(g.box, nullptr, nullptr, g.species.data(), g.stream);
+  if (rc == MDG_OK) rc = g.set->sync_desc(g.stream);
+  if (rc != MDG_OK) {
+    fail(rc, who);
+    return false;
+  }
+  return true;
+}
+
+// host <-> device copies of mdc's SoA arrays
+enum { kR = 1, kV = 2, kF = 4, kF0 = 8 };
+
+bool put(int what, const char* who) {
+  const int N = sys.N;
+  const BoxDesc& D = g.box->desc;
+  const double* h[4][3] = {{sys.rx, sys.ry, sys.rz}, {sys.vx, sys.vy, sys.vz},
+                           {sys.fx, sys.fy, sys.fz}, {sys.fx0, sys.fy0, sys.fz0}};
+  double* d[4][3] = {{D.rx, D.ry, D.rz}, {D.vx, D.vy, D.vz}, {D.fx, D.fy, D.fz},
+                     {D.fx0, D.fy0, D.fz0}};
+  for (int a = 0; a < 4; ++a) {
+    if (!(what & (1 << a))) continue;
+    for (int c = 0; c < 3; ++c)
+      if (hipMemcpyAsync(d[a][c], h[a][c], sizeof(double) * N, hipMemcpyHostToDevice,
+                         g.stream) != hipSuccess) {
+        fail(MDG_E_HIP, who);
+        return false;
+      }
+  }
+  return true;
+}
+
+bool get(int what, const char* who) {
+  const int N = sys.N;
+  const BoxDesc& D = g.box->desc;
+  double* h[4][3] = {{sys.rx, sys.ry, sys.rz}, {sys.vx, sys.vy, sys.vz},
+                     {sys.fx, sys.fy, sys.fz}, {sys.fx0, sys.fy0, sys.fz0}};
+  const double* d[4][3] = {{D.rx, D.ry, D.rz}, {D.vx, D.vy, D.vz}, {D.fx, D.fy, D.fz},
+                           {D.fx0, D.fy0, D.fz0}};
+  for (int a = 0; a < 4; ++a) {
+    if (!(what & (1 << a))) continue;
+    for (int c = 0; c < 3; ++c)
+      if (hipMemcpyAsync(h[a][c], d[a][c], sizeof(double) * N, hipMemcpyDeviceToHost,
+                         g.stream) != hipSuccess) {
+        fail(MDG_E_HIP, who);
+        return false;
+      }
+  }
+  if (hipStreamSynchronize(g.stream) != hipSuccess) {
+    fail(MDG_E_HIP, who);
+    return false;
+  }
+  return true;
+}
+
+bool scalars(double* S, const char* who) {
+  int rc = box_read_scalars(g.box, S, g.stream);
+  int e = 0;
+  if (rc == MDG_OK) rc = box_read_error(g.box, &e, g.stream);
+  if (rc == MDG_OK && e) rc = (e & kErrState) ? MDG_E_STATE : MDG_E_RANGE;
+  if (rc != MDG_OK) {
+    fail(rc, who);
+    return false;
+  }
+  return true;
+}
+
+bool ewald_on() { return !(sys.a1 == 0.0 || sys.a1 > 10.0); }  // rcprcl.c:23
+
+}  // namespace
+
+extern "C" {
+
+int mdg_dropin_status(void) { return g.status; }
+void mdg_dropin_clear_status(void) {
+  g.status = MDG_OK;
+  g.reported = false;
+}
+void mdg_dropin_release(void) { release(); }
+
+// rv.c:12-18: clear_foc, real_space, reciprocal_space, reciprocal_space3
+void calc_foc(void) {
+  if (!ensure("calc_foc") || !put(kR | kF, "calc_foc")) return;
+  g.set->enqueue_step(g.stream, false);
+  double S[kScalCount];
+  if (!get(kF | kF0, "calc_foc") || !scalars(S, "calc_foc")) return;
+  sys.pot = S[kScalPot];
+  sys.virX = S[kScalVirX]; sys.virY = S[kScalVirY]; sys.virZ = S[kScalVirZ];
+}
+
+// init.c:179-190
+void clear_foc(void) {
+  if (!ensure("clear_foc") || !put(kF, "clear_foc")) return;
+  g.set->enqueue_clear_foc(g.stream);
+  get(kF | kF0, "clear_foc");
+}
+
+// real.c:15-84: adds to f, resets pot and the virial
+void real_space(void) {
+  if (!ensure("real_space") || !put(kR | kF, "real_space")) return;
+  g.set->enqueue_forces_parts(g.stream, true, 1, false);
+  g.set->enqueue_scalars(g.stream, false);
+  double S[kScalCount];
+  if (!get(kF, "real_space") || !scalars(S, "real_space")) return;
+  sys.pot = S[kScalPotReal];
+  sys.virX = S[kScalVirX]; sys.virY = S[kScalVirY]; sys.virZ = S[kScalVirZ];
+}
+
+// rcprcl.c:11-97: adds reciprocal forces and energy
+void reciprocal_space(void) {
+  if (!ewald_on()) return;
+  if (!ensure("reciprocal_space") || !put(kR | kF, "reciprocal_space")) return;
+  g.set->enqueue_forces_parts(g.stream, false, 1, true);
+  g.set->enqueue_scalars(g.stream, false);
+  double S[kScalCount];
+  if (!get(kF, "reciprocal_space") || !scalars(S, "reciprocal_space")) return;
+  sys.pot += S[kScalPotRecip];
+}
+
+// rcprcl.c:108-133: pot += -a3 sum Z^2
+void reciprocal_space3(void) {
+  if (!ewald_on()) return;
+  if (!ensure("reciprocal_space3")) return;
+  sys.pot += g.box->desc.self_pot;
+}
+
+// rv.c:26-31
+void next_rv_verlet(void) {
+  if (!ensure("next_rv_verlet") || !put(kR | kV | kF, "next_rv_verlet")) return;
+  g.set->enqueue_step(g.stream, true);
+  double S[kScalCount];
+  if (!get(kR | kV | kF | kF0, "next_rv_verlet") || !scalars(S, "next_rv_verlet")) return;
+  sys.pot = S[kScalPot];
+  sys.virX = S[kScalVirX]; sys.virY = S[kScalVirY]; sys.virZ = S[kScalVirZ];
+}
+
+// rv.c:33-44
+void next_v(void) {
+  if (!ensure("next_v") || !put(kV | kF | kF0, "next_v")) return;
+  g.set->enqueue_next_v(g.stream);
+  get(kV, "next_v");
+}
+
+// rv.c:46-66
+void next_r(void) {
+  if (!ensure("next_r") || !put(kR | kV | kF, "next_r")) return;
+  g.set->enqueue_next_r(g.stream);
+  get(kR, "next_r");
+}
+
+// rv.c:68-86
+void calc_kin(void) {
+  if (!ensure("calc_kin") || !put(kV, "calc_kin")) return;
+  g.set->enqueue_scalars(g.stream, false);
+  double S[kScalCount];
+  if (!scalars(S, "calc_kin")) return;
+  sys.kinX = S[kScalKinX]; sys.kinY = S[kScalKinY]; sys.kinZ = S[kScalKinZ];
+  sys.kin = sys.kinX + sys.kinY + sys.kinZ;
+}
+
+// rv.c:88-116
+void moment_correction(void) {
+  if (!ensure("moment_correction") || !put(kV, "moment_correction")) return;
+  g.set->enqueue_moment_correction(g.stream);
+  get(kV, "moment_correction");
+}
+
+// pt.c:8-21 (six scalar operations: evaluated on the host)
+void calc_press(void) {
+  const double vol = sys.Lx * sys.Ly * sys.Lz;
+  sys.presX = (2.0 * sys.kinX + sys.virX) / vol;
+  sys.presY = (2.0 * sys.kinY + sys.virY) / vol;
+  sys.presZ = (2.0 * sys.kinZ + sys.virZ) / vol;
+  sys.pres = (sys.presX + sys.presY + sys.presZ) / 3.0;
+}
+
+// control.c:183-248 for the two-species BMH plugin (NaCl form of
+// potential.h): fills sys.pe1r1 / sys.fe1r1 / sys.lookup on the host.
+void mk_table(void) {
+  if (!sys.pe1r1 || !sys.fe1r1 || !sys.lookup || !ion.Z || ctl.kinds_of_ions != 2) {
+    fail(MDG_E_ARG, "mk_table");
+    return;
+  }
+  mdg_potential p;
+  memset(&p, 0, sizeof(p));
+  p.Z[0] = ion.Z[0]; p.Z[1] = ion.Z[1];
+  p.b = hm.b;
+  p.rho = hm.rho;
+  p.c[0] = hm.c_na_na; p.c[1] = hm.c_na_cl; p.c[2] = hm.c_cl_cl;
+  p.sigma[0] = hm.sigma_na + hm.sigma_na;
+  p.sigma[1] = hm.sigma_na + hm.sigma_cl;
+  p.sigma[2] = hm.sigma_cl + hm.sigma_cl;
+  p.C[0] = hm.C_na_na; p.C[1] = hm.C_na_cl; p.C[2] = hm.C_cl_cl;
+  p.D[0] = hm.D_na_na; p.D[1] = hm.D_na_cl; p.D[2] = hm.D_cl_cl;
+  host_mk_table(p, sys.a1, sys.a3, sys.table_division + 1, sys.pe1r1, sys.fe1r1);
+  sys.lookup[0] = 0; sys.lookup[1] = 1; sys.lookup[2] = 1; sys.lookup[3] = 2;
+}
+
+// ext.c:73-104
+double erfcc(double x) { return host_erfcc(x); }
+
+}  // extern "C"

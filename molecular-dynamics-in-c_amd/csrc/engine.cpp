@@ -1,0 +1,641 @@
+// engine.cpp -- box plans, device-resident boxes and box sets.
+#include "engine.h"
+
+#include <math.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <mutex>
+
+#include "host_setup.h"
+#include "launch.h"
+
+namespace mdg {
+
+static std::mutex g_err_mu;
+static char g_last_hip[256] = "";
+
+void set_last_hip_error(hipError_t e, const char* what) {
+  std::lock_guard<std::mutex> lk(g_err_mu);
+  snprintf(g_last_hip, sizeof(g_last_hip), "%s: %s", what, hipGetErrorString(e));
+  fprintf(stderr, "libmdgpu: %s\n", g_last_hip);
+}
+
+// mdc's constants (mdc/program/md.h:8-16), as spelled there.
+static const double kPi = 3.1415926535897932385;
+
+// ------------------------------------------------------------------ tiling
+Tiling tiling_for(int N, int nk) {
+  Tiling t;
+  (void)nk;
+  // a lone small box cannot fill the GPU; keep its items from becoming tiny
+  if (N < 1024) {
+    t.sk_items_target = 256;
+    t.kf_items_target = 512;
+  }
+  return t;
+}
+
+static std::vector<int> partition_columns(const std::vector<KCol>& cols, int nk, int groups) {
+  std::vector<int> start;
+  const int ncol = (int)cols.size();
+  groups = std::max(1, std::min(groups, ncol));
+  start.push_back(0);
+  long acc = 0;
+  int g = 1;
+  for (int c = 0; c < ncol && g < groups; ++c) {
+    acc += cols[c].len;
+    if (acc * groups >= (long)g * nk) {
+      if (c + 1 < ncol) start.push_back(c + 1);
+      ++g;
+    }
+  }
+  start.push_back(ncol);
+  return start;
+}
+
+int make_plan(const BoxSpec& sp, int precision, BoxPlan* P, std::string* why) {
+  (void)precision;
+  *P = BoxPlan();
+  P->spec = sp;
+  if (sp.N <= 0 || !(sp.L[0] > 0) || !(sp.L[1] > 0) || !(sp.L[2] > 0) || !(sp.rc > 0)) {
+    if (why) *why = "box needs N > 0, L > 0 and rc > 0";
+    return MDG_E_ARG;
+  }
+  if (sp.kinds < 1 || sp.kinds > 4) {
+    if (why) *why = "1..4 species supported";
+    return MDG_E_ARG;
+  }
+  if (sp.real_mode == MDG_TABLE && (!sp.pe || !sp.fe || sp.table_nb <= 0)) {
+    if (why) *why = "table mode needs a pair table";
+    return MDG_E_ARG;
+  }
+  if (sp.real_mode == MDG_ANALYTIC && sp.kinds != 2) {
+    if (why) *why = "analytic mode supports two species";
+    return MDG_E_ARG;
+  }
+  P->ewald = !(sp.alpha == 0.0 || sp.alpha > 10.0);  // rcprcl.c:23
+  if (P->ewald) {
+    const int s = (int)sqrtf((float)sp.hm);           // init.c:44
+    P->hm_sqrt = s;
+    P->hmax = s;
+    const double vv = sp.L[0] * sp.L[1] * sp.L[2];
+    const double v2 = 2.0 / vv, vp = 1.0 / (kPi * vv);
+    for (int hx = -s; hx <= s; ++hx)
+      for (int hy = -s; hy <= s; ++hy) {
+        // rcprcl.c:32-38: hz >= 0; at hz = 0 only hy > 0, or hy = 0 and hx > 0
+        const int lo = (hy > 0 || (hy == 0 && hx > 0)) ? 0 : 1;
+        int hi = -1;
+        for (int hz = lo; hz <= s; ++hz)
+          if (hx * hx + hy * hy + hz * hz <= sp.hm) hi = hz;
+        if (hi < lo) continue;
+        KCol c{};
+        c.hx = hx; c.hy = hy; c.hz0 = lo; c.len = hi - lo + 1; c.k0 = P->nk;
+        P->cols.push_back(c);
+        for (int hz = lo; hz <= hi; ++hz) {
+          const double mx = ((double)hx) / sp.L[0], my = ((double)hy) / sp.L[1],
+                       mz = ((double)hz) / sp.L[2];
+          const double m2 = mx * mx + my * my + mz * mz;
+          const double ex = exp(-m2 * sp.p2a2);
+          P->kce.push_back(vp * (ex / m2));
+          P->kcf.push_back(v2 * (ex / m2));
+        }
+        P->nk += c.len;
+      }
+    P->ncol = (int)P->cols.size();
+    if (P->nk == 0) P->ewald = false;
+  }
+  // cell grid: cells wider than rc (with a relative margin, so that two ions
+  // within rc never bin two cells apart after rounding)
+  bool can_cells = true;
+  for (int d = 0; d < 3; ++d) {
+    P->nc[d] = (int)floor(sp.L[d] / (sp.rc * (1.0 + 1e-9)));
+    if (P->nc[d] < 3) can_cells = false;
+  }
+  const Tiling T = tiling_for(sp.N, P->nk);
+  bool use_cells = false;
+  if (sp.cells == MDG_CELL_LIST) {
+    if (!can_cells) {
+      if (why) *why = "cell list needs at least 3 cells of width >= rc per axis";
+      return MDG_E_RANGE;
+    }
+    use_cells = true;
+  } else if (sp.cells == MDG_CELLS_AUTO) {
+    use_cells = can_cells && sp.N >= T.cells_min_n;
+  }
+  P->all_pairs = !use_cells;
+  if (use_cells) P->ncell = P->nc[0] * P->nc[1] * P->nc[2];
+  else P->nc[0] = P->nc[1] = P->nc[2] = 0;
+  P->nreal = P->all_pairs ? (sp.N + kRealThreads - 1) / kRealThreads : P->ncell;
+  P->nchunk = (sp.N + kSkChunk - 1) / kSkChunk;
+  if (P->ewald) {
+    const int ion_blocks = (sp.N + kKfThreads - 1) / kKfThreads;
+    int gsk = (T.sk_items_target + P->nchunk - 1) / P->nchunk;
+    gsk = std::max(1, std::min(gsk, std::max(1, P->nk / T.sk_min_k)));
+    P->sk_group_start = partition_columns(P->cols, P->nk, gsk);
+    int gkf = (T.kf_items_target + ion_blocks - 1) / ion_blocks;
+    gkf = std::max(1, std::min(gkf, std::max(1, P->nk / T.kf_min_k)));
+    P->kf_group_start = partition_columns(P->cols, P->nk, gkf);
+  }
+  return MDG_OK;
+}
+
+// ------------------------------------------------------------------ boxes
+static size_t take(size_t& off, size_t bytes) {
+  const size_t o = (off + 255) & ~(size_t)255;
+  off = o + bytes;
+  return o;
+}
+
+int box_create(const BoxSpec& spec, int precision, Box** out, std::string* why) {
+  Box* b = new Box();
+  b->precision = precision;
+  int rc = make_plan(spec, precision, &b->plan, why);
+  if (rc != MDG_OK) {
+    delete b;
+    return rc;
+  }
+  const BoxPlan& P = b->plan;
+  const int N = spec.N;
+  const bool mixed = precision == MDG_MIXED;
+  const size_t cplx = mixed ? sizeof(float2) : sizeof(double2);
+  const int ngroup = P.ewald ? (int)P.kf_group_start.size() - 1 : 0;
+  size_t off = 0;
+  size_t o_ion[12];
+  for (int k = 0; k < 12; ++k) o_ion[k] = take(off, sizeof(double) * N);
+  const size_t o_typ = take(off, sizeof(int) * N);
+  size_t o_cell_of = 0, o_cnt = 0, o_start = 0, o_sorted = 0, o_s[3] = {0, 0, 0}, o_styp = 0;
+  if (!P.all_pairs) {
+    o_cell_of = take(off, sizeof(int) * N);
+    o_cnt = take(off, sizeof(int) * P.ncell);
+    o_start = take(off, sizeof(int) * (P.ncell + 1));
+    o_sorted = take(off, sizeof(int) * N);
+    for (int d = 0; d < 3; ++d) o_s[d] = take(off, sizeof(double) * N);
+    o_styp = take(off, sizeof(int) * N);
+  }
+  size_t o_cols = 0, o_kce = 0, o_kcf = 0, o_kab = 0, o_kab32 = 0, o_kpot = 0, o_tx = 0,
+         o_ty = 0, o_tw = 0, o_skp = 0, o_fp = 0;
+  if (P.ewald) {
+    o_cols = take(off, sizeof(KCol) * P.ncol);
+    o_kce = take(off, sizeof(double) * P.nk);
+    o_kcf = take(off, sizeof(double) * P.nk);
+    o_kab = take(off, sizeof(double) * 2 * P.nk);
+    if (mixed) o_kab32 = take(off, sizeof(float) * 2 * P.nk);
+    o_kpot = take(off, sizeof(double) * P.nk);
+    o_tx = take(off, cplx * (size_t)(P.hmax + 1) * N);
+    o_ty = take(off, cplx * (size_t)(P.hmax + 1) * N);
+    o_tw = take(off, cplx * (size_t)N);
+    o_skp = take(off, sizeof(double2) * (size_t)P.nchunk * P.nk);
+    o_fp = take(off, sizeof(double) * 3 * (size_t)ngroup * N);
+  }
+  size_t o_tab = 0, o_tab32 = 0;
+  const int ntab = spec.table_ntypes * spec.table_nb;
+  if (spec.real_mode == MDG_TABLE) {
+    if (mixed) o_tab32 = take(off, sizeof(float2) * ntab);
+    else o_tab = take(off, sizeof(double2) * ntab);
+  }
+  const size_t o_rpart = take(off, sizeof(double) * 4 * P.nreal);
+  const size_t o_rpairs = take(off, sizeof(unsigned long long) * P.nreal);
+  const size_t o_scal = take(off, sizeof(double) * kScalCount);
+  const size_t o_err = take(off, sizeof(int));
+  b->bytes = (off + 255) & ~(size_t)255;
+  if (hipMalloc(&b->dmem, b->bytes) != hipSuccess) {
+    (void)hipGetLastError();
+    if (why) *why = "device allocation failed";
+    delete b;
+    return MDG_E_NOMEM;
+  }
+  char* base = (char*)b->dmem;
+  if (hipMemset(base, 0, b->bytes) != hipSuccess) {
+    (void)hipFree(b->dmem);
+    delete b;
+    return MDG_E_HIP;
+  }
+  BoxDesc& D = b->desc;
+  memset(&D, 0, sizeof(D));
+  double** ion_ptrs[12] = {&D.rx, &D.ry, &D.rz, &D.vx, &D.vy, &D.vz,
+                           &D.fx, &D.fy, &D.fz, &D.fx0, &D.fy0, &D.fz0};
+  for (int k = 0; k < 12; ++k) *ion_ptrs[k] = (double*)(base + o_ion[k]);
+  D.typ = (int*)(base + o_typ);
+  D.N = N;
+  D.ewald = P.ewald ? 1 : 0;
+  D.real_mode = spec.real_mode;
+  D.all_pairs = P.all_pairs ? 1 : 0;
+  D.kinds = spec.kinds;
+  for (int k = 0; k < 16; ++k) D.lookup[k] = spec.lookup[k];
+  for (int d = 0; d < 3; ++d) {
+    D.L[d] = spec.L[d];
+    D.halfL[d] = spec.L[d] * 0.5;  // real.c:24-26
+    D.invL[d] = 1.0 / spec.L[d];
+  }
+  D.rc2 = spec.rc2 > 0 ? spec.rc2 : spec.rc * spec.rc;  // init.c:43
+  D.alpha = spec.alpha;
+  D.a3 = spec.a3;
+  D.dt = spec.dt;
+  D.dt2 = spec.dt2;
+  for (int k = 0; k < 4; ++k) {
+    D.Z[k] = spec.Z[k];
+    D.m[k] = spec.m[k];
+  }
+  D.vol = spec.L[0] * spec.L[1] * spec.L[2];
+  for (int t = 0; t < 3; ++t) {
+    D.cb[t] = spec.pot.c[t] * spec.pot.b;
+    D.sig[t] = spec.pot.sigma[t];
+    D.Cd[t] = spec.pot.C[t];
+    D.Dd[t] = spec.pot.D[t];
+  }
+  D.rho = spec.pot.rho;
+  D.table_nb = spec.table_nb;
+  if (!P.all_pairs) {
+    for (int d = 0; d < 3; ++d) {
+      D.nc[d] = P.nc[d];
+      D.cellscale[d] = (double)P.nc[d] / spec.L[d];
+    }
+    D.ncell = P.ncell;
+    D.cell_of = (int*)(base + o_cell_of);
+    D.cell_count = (int*)(base + o_cnt);
+    D.cell_start = (int*)(base + o_start);
+    D.sorted = (int*)(base + o_sorted);
+    D.sx = (double*)(base + o_s[0]);
+    D.sy = (double*)(base + o_s[1]);
+    D.sz = (double*)(base + o_s[2]);
+    D.styp = (int*)(base + o_styp);
+  }
+  int err = MDG_OK;
+  if (P.ewald) {
+    D.nk = P.nk;
+    D.ncol = P.ncol;
+    D.hmax = P.hmax;
+    D.cols = (const KCol*)(base + o_cols);
+    D.kce = (const double*)(base + o_kce);
+    D.kcf = (const double*)(base + o_kcf);
+    D.kab = (double*)(base + o_kab);
+    D.kab32 = mixed ? (float*)(base + o_kab32) : nullptr;
+    D.kpot = (double*)(base + o_kpot);
+    D.tabx = base + o_tx;
+    D.taby = base + o_ty;
+    D.tabw = base + o_tw;
+    D.skpart = (double2*)(base + o_skp);
+    D.nchunk = P.nchunk;
+    D.fpart = (double*)(base + o_fp);
+    D.ngroup = ngroup;
+    if (hipMemcpy(base + o_cols, P.cols.data(), sizeof(KCol) * P.ncol, hipMemcpyHostToDevice) !=
+            hipSuccess ||
+        hipMemcpy(base + o_kce, P.kce.data(), sizeof(double) * P.nk, hipMemcpyHostToDevice) !=
+            hipSuccess ||
+        hipMemcpy(base + o_kcf, P.kcf.data(), sizeof(double) * P.nk, hipMemcpyHostToDevice) !=
+            hipSuccess)
+      err = MDG_E_HIP;
+  }
+  if (spec.real_mode == MDG_TABLE && err == MDG_OK) {
+    if (mixed) {
+      std::vector<float2> t(ntab);
+      for (int k = 0; k < ntab; ++k) t[k] = make_float2((float)spec.pe[k], (float)spec.fe[k]);
+      if (hipMemcpy(base + o_tab32, t.data(), sizeof(float2) * ntab, hipMemcpyHostToDevice) !=
+          hipSuccess)
+        err = MDG_E_HIP;
+      D.table32 = (const float2*)(base + o_tab32);
+    } else {
+      std::vector<double2> t(ntab);
+      for (int k = 0; k < ntab; ++k) t[k] = make_double2(spec.pe[k], spec.fe[k]);
+      if (hipMemcpy(base + o_tab, t.data(), sizeof(double2) * ntab, hipMemcpyHostToDevice) !=
+          hipSuccess)
+        err = MDG_E_HIP;
+      D.table = (const double2*)(base + o_tab);
+    }
+  }
+  D.rpart = (double*)(base + o_rpart);
+  D.rpairs = (unsigned long long*)(base + o_rpairs);
+  D.nreal = P.nreal;
+  D.scal = (double*)(base + o_scal);
+  D.err = (int*)(base + o_err);
+  if (err != MDG_OK) {
+    if (why) *why = "uploading box constants failed";
+    box_destroy(b);
+    return err;
+  }
+  b->species.assign(N, 0);
+  b->desc_dirty = true;
+  *out = b;
+  return MDG_OK;
+}
+
+void box_destroy(Box* b) {
+  if (!b) return;
+  if (b->dmem) (void)hipFree(b->dmem);
+  delete b;
+}
+
+int box_upload_state(Box* b, const double* r, const double* v, const int* species,
+                     hipStream_t s) {
+  const int N = b->desc.N;
+  BoxDesc& D = b->desc;
+  if (species) {
+    for (int i = 0; i < N; ++i)
+      if (species[i] < 0 || species[i] >= b->plan.spec.kinds) return MDG_E_ARG;
+    b->species.assign(species, species + N);
+    MDG_HIP(hipMemcpyAsync(D.typ, species, sizeof(int) * N, hipMemcpyHostToDevice, s));
+    // mdc's self term, rcprcl.c:119-125: Z2 summed in index order, then -Z2*a3
+    double z2 = 0.0;
+    for (int i = 0; i < N; ++i) {
+      const double z = b->plan.spec.Z[species[i]];
+      z2 += z * z;
+    }
+    D.self_pot = -z2 * b->plan.spec.a3;
+    b->desc_dirty = true;
+  }
+  if (r) {
+    for (long k = 0; k < 3L * N; ++k)
+      if (!isfinite(r[k])) return MDG_E_STATE;
+    MDG_HIP(hipMemcpyAsync(D.rx, r, sizeof(double) * N, hipMemcpyHostToDevice, s));
+    MDG_HIP(hipMemcpyAsync(D.ry, r + N, sizeof(double) * N, hipMemcpyHostToDevice, s));
+    MDG_HIP(hipMemcpyAsync(D.rz, r + 2 * (long)N, sizeof(double) * N, hipMemcpyHostToDevice, s));
+  }
+  if (v) {
+    MDG_HIP(hipMemcpyAsync(D.vx, v, sizeof(double) * N, hipMemcpyHostToDevice, s));
+    MDG_HIP(hipMemcpyAsync(D.vy, v + N, sizeof(double) * N, hipMemcpyHostToDevice, s));
+    MDG_HIP(hipMemcpyAsync(D.vz, v + 2 * (long)N, sizeof(double) * N, hipMemcpyHostToDevice, s));
+  }
+  MDG_HIP(hipStreamSynchronize(s));
+  return MDG_OK;
+}
+
+int box_download_state(Box* b, double* r, double* v, double* f, double* f0, hipStream_t s) {
+  const int N = b->desc.N;
+  const BoxDesc& D = b->desc;
+  double* dst[4] = {r, v, f, f0};
+  const double* src[4][3] = {{D.rx, D.ry, D.rz}, {D.vx, D.vy, D.vz}, {D.fx, D.fy, D.fz},
+                             {D.fx0, D.fy0, D.fz0}};
+  for (int a = 0; a < 4; ++a) {
+    if (!dst[a]) continue;
+    for (int d = 0; d < 3; ++d)
+      MDG_HIP(hipMemcpyAsync(dst[a] + (long)d * N, src[a][d], sizeof(double) * N,
+                             hipMemcpyDeviceToHost, s));
+  }
+  MDG_HIP(hipStreamSynchronize(s));
+  return MDG_OK;
+}
+
+int box_read_scalars(Box* b, double* scal, hipStream_t s) {
+  MDG_HIP(hipMemcpyAsync(scal, b->desc.scal, sizeof(double) * kScalCount, hipMemcpyDeviceToHost,
+                         s));
+  MDG_HIP(hipStreamSynchronize(s));
+  return MDG_OK;
+}
+
+int box_read_error(Box* b, int* err, hipStream_t s) {
+  MDG_HIP(hipMemcpyAsync(err, b->desc.err, sizeof(int), hipMemcpyDeviceToHost, s));
+  MDG_HIP(hipStreamSynchronize(s));
+  return MDG_OK;
+}
+
+// ------------------------------------------------------------------ sets
+BoxSet::BoxSet(int precision, int real_mode) : precision_(precision), real_mode_(real_mode) {}
+
+BoxSet::~BoxSet() {
+  invalidate_graph();
+  if (d_desc_) (void)hipFree(d_desc_);
+  if (d_items_) (void)hipFree(d_items_);
+  if (d_ids_) (void)hipFree(d_ids_);
+}
+
+void BoxSet::invalidate_graph() {
+  if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
+  if (graph_) (void)hipGraphDestroy(graph_);
+  graph_exec_ = nullptr;
+  graph_ = nullptr;
+  graph_steps_ = 0;
+}
+
+int BoxSet::set_boxes(const std::vector<Box*>& boxes) {
+  invalidate_graph();
+  boxes_ = boxes;
+  const int nb = (int)boxes_.size();
+  std::vector<int> order(nb);
+  for (int k = 0; k < nb; ++k) order[k] = k;
+  // big boxes' items first, so the longest tiles start earliest
+  std::stable_sort(order.begin(), order.end(),
+                   [&](int a, int b) { return boxes_[a]->desc.N > boxes_[b]->desc.N; });
+  std::vector<Item> real, tab, sk, fin, kf, ion;
+  std::vector<int> cellid, allid;
+  for (int bi : order) {
+    const Box* B = boxes_[bi];
+    const BoxPlan& P = B->plan;
+    const int N = B->desc.N;
+    allid.push_back(bi);
+    if (P.all_pairs) {
+      int slot = 0;
+      for (int i0 = 0; i0 < N; i0 += kRealThreads) real.push_back({bi, i0, slot++, 0, 0});
+    } else {
+      cellid.push_back(bi);
+      for (int c = 0; c < P.ncell; ++c) real.push_back({bi, c, c, 0, 0});
+    }
+    for (int i0 = 0; i0 < N; i0 += kIonThreads) ion.push_back({bi, i0, 0, 0, 0});
+    if (P.ewald) {
+      for (int i0 = 0; i0 < N; i0 += kIonThreads) tab.push_back({bi, i0, 0, 0, 0});
+      const int gs = (int)P.sk_group_start.size() - 1;
+      for (int ch = 0; ch < P.nchunk; ++ch)
+        for (int g = 0; g < gs; ++g)
+          sk.push_back({bi, ch, P.sk_group_start[g],
+                        P.sk_group_start[g + 1] - P.sk_group_start[g], g});
+      for (int k0 = 0; k0 < P.nk; k0 += kIonThreads) fin.push_back({bi, k0, 0, 0, 0});
+      const int gk = (int)P.kf_group_start.size() - 1;
+      for (int i0 = 0; i0 < N; i0 += kKfThreads)
+        for (int g = 0; g < gk; ++g)
+          kf.push_back({bi, i0, P.kf_group_start[g],
+                        P.kf_group_start[g + 1] - P.kf_group_start[g], g});
+    }
+  }
+  std::vector<Item> items;
+  auto app = [&](std::vector<Item>& v, int& off, int& n) {
+    off = (int)items.size();
+    n = (int)v.size();
+    items.insert(items.end(), v.begin(), v.end());
+  };
+  app(real, off_real_, n_real_);
+  app(tab, off_tab_, n_tab_);
+  app(sk, off_sk_, n_sk_);
+  app(fin, off_fin_, n_fin_);
+  app(kf, off_kf_, n_kf_);
+  app(ion, off_ion_, n_ion_);
+  std::vector<int> ids;
+  off_cellid_ = 0;
+  n_cellid_ = (int)cellid.size();
+  ids.insert(ids.end(), cellid.begin(), cellid.end());
+  off_allid_ = (int)ids.size();
+  n_allid_ = (int)allid.size();
+  ids.insert(ids.end(), allid.begin(), allid.end());
+  if (items.size() > items_cap_) {
+    if (d_items_) (void)hipFree(d_items_);
+    d_items_ = nullptr;
+    items_cap_ = std::max<size_t>(items.size(), 2 * items_cap_);
+    MDG_HIP(hipMalloc(&d_items_, sizeof(Item) * items_cap_));
+  }
+  if (ids.size() > ids_cap_) {
+    if (d_ids_) (void)hipFree(d_ids_);
+    d_ids_ = nullptr;
+    ids_cap_ = std::max<size_t>(ids.size(), 2 * ids_cap_);
+    MDG_HIP(hipMalloc(&d_ids_, sizeof(int) * ids_cap_));
+  }
+  if ((size_t)nb > desc_cap_) {
+    if (d_desc_) (void)hipFree(d_desc_);
+    d_desc_ = nullptr;
+    desc_cap_ = std::max<size_t>(nb, 2 * desc_cap_);
+    MDG_HIP(hipMalloc(&d_desc_, sizeof(BoxDesc) * desc_cap_));
+  }
+  if (!items.empty())
+    MDG_HIP(hipMemcpy(d_items_, items.data(), sizeof(Item) * items.size(), hipMemcpyHostToDevice));
+  if (!ids.empty())
+    MDG_HIP(hipMemcpy(d_ids_, ids.data(), sizeof(int) * ids.size(), hipMemcpyHostToDevice));
+  for (Box* b : boxes_) b->desc_dirty = true;
+  return MDG_OK;
+}
+
+int BoxSet::sync_desc(hipStream_t s) {
+  bool any = false;
+  for (Box* b : boxes_) any |= b->desc_dirty;
+  if (!any) return MDG_OK;
+  std::vector<BoxDesc> d(boxes_.size());
+  for (size_t k = 0; k < boxes_.size(); ++k) {
+    d[k] = boxes_[k]->desc;
+    boxes_[k]->desc_dirty = false;
+  }
+  MDG_HIP(hipMemcpyAsync(d_desc_, d.data(), sizeof(BoxDesc) * d.size(), hipMemcpyHostToDevice, s));
+  MDG_HIP(hipStreamSynchronize(s));
+  return MDG_OK;
+}
+
+void BoxSet::launch_real(hipStream_t s, int mode) {
+  const bool analytic = real_mode_ == MDG_ANALYTIC;
+  if (precision_ == MDG_MIXED)
+    launch_real32(analytic, d_desc_, d_items_ + off_real_, n_real_, mode, s);
+  else
+    launch_real64(analytic, d_desc_, d_items_ + off_real_, n_real_, mode, s);
+}
+
+void BoxSet::enqueue_forces_parts(hipStream_t s, bool real, int real_write_mode, bool recip) {
+  const bool mixed = precision_ == MDG_MIXED;
+  if (real) {
+    launch_cell_build(d_desc_, d_ids_ + off_cellid_, n_cellid_, s);
+    launch_real(s, real_write_mode);
+  }
+  if (recip && n_sk_ > 0) {
+    launch_phase_tables(mixed, d_desc_, d_items_ + off_tab_, n_tab_, s);
+    if (mixed) launch_sk_partial32(d_desc_, d_items_ + off_sk_, n_sk_, s);
+    else launch_sk_partial64(d_desc_, d_items_ + off_sk_, n_sk_, s);
+    launch_sk_finalize(d_desc_, d_items_ + off_fin_, n_fin_, s);
+    if (mixed) launch_kforce32(d_desc_, d_items_ + off_kf_, n_kf_, s);
+    else launch_kforce64(d_desc_, d_items_ + off_kf_, n_kf_, s);
+    launch_force_vv(false, d_desc_, d_items_ + off_ion_, n_ion_, s);
+  }
+}
+
+void BoxSet::enqueue_step(hipStream_t s, bool vv) {
+  const bool mixed = precision_ == MDG_MIXED;
+  launch_cell_build(d_desc_, d_ids_ + off_cellid_, n_cellid_, s);
+  if (n_sk_ > 0) launch_phase_tables(mixed, d_desc_, d_items_ + off_tab_, n_tab_, s);
+  launch_real(s, 0);
+  if (n_sk_ > 0) {
+    if (mixed) launch_sk_partial32(d_desc_, d_items_ + off_sk_, n_sk_, s);
+    else launch_sk_partial64(d_desc_, d_items_ + off_sk_, n_sk_, s);
+    launch_sk_finalize(d_desc_, d_items_ + off_fin_, n_fin_, s);
+    if (mixed) launch_kforce32(d_desc_, d_items_ + off_kf_, n_kf_, s);
+    else launch_kforce64(d_desc_, d_items_ + off_kf_, n_kf_, s);
+  }
+  launch_force_vv(vv, d_desc_, d_items_ + off_ion_, n_ion_, s);
+  launch_box_scalars(d_desc_, d_ids_ + off_allid_, n_allid_, vv ? 1 : 0, s);
+}
+
+void BoxSet::enqueue_scalars(hipStream_t s, bool count_step) {
+  launch_box_scalars(d_desc_, d_ids_ + off_allid_, n_allid_, count_step ? 1 : 0, s);
+}
+void BoxSet::enqueue_clear_foc(hipStream_t s) {
+  launch_clear_foc(d_desc_, d_items_ + off_ion_, n_ion_, s);
+}
+void BoxSet::enqueue_next_v(hipStream_t s) { launch_next_v(d_desc_, d_items_ + off_ion_, n_ion_, s); }
+void BoxSet::enqueue_next_r(hipStream_t s) { launch_next_r(d_desc_, d_items_ + off_ion_, n_ion_, s); }
+void BoxSet::enqueue_moment_correction(hipStream_t s) {
+  launch_moment_correction(d_desc_, d_ids_ + off_allid_, n_allid_, s);
+}
+
+int BoxSet::step(int nsteps, hipStream_t s) {
+  if (nsteps <= 0 || boxes_.empty()) return MDG_OK;
+  int rc = sync_desc(s);
+  if (rc != MDG_OK) return rc;
+  const int G = std::min(nsteps, 8);
+  if (!graph_exec_ || graph_steps_ != G) {
+    invalidate_graph();
+    MDG_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    for (int k = 0; k < G; ++k) enqueue_step(s, true);
+    MDG_HIP(hipStreamEndCapture(s, &graph_));
+    MDG_HIP(hipGraphInstantiate(&graph_exec_, graph_, nullptr, nullptr, 0));
+    graph_steps_ = G;
+  }
+  int left = nsteps;
+  while (left >= G) {
+    MDG_HIP(hipGraphLaunch(graph_exec_, s));
+    left -= G;
+  }
+  for (; left > 0; --left) enqueue_step(s, true);
+  MDG_HIP(hipGetLastError());
+  return MDG_OK;
+}
+
+int BoxSet::profile(int nsteps, hipStream_t s, std::vector<std::string>* names,
+                    std::vector<double>* ms) {
+  int rc = sync_desc(s);
+  if (rc != MDG_OK) return rc;
+  const bool mixed = precision_ == MDG_MIXED;
+  const char* kn[8] = {"cell_build", "phase_tables", "real_space", "sk_partial",
+                       "sk_finalize", "kforce",      "force_vv",   "box_scalars"};
+  std::vector<hipEvent_t> ev(2 * 8 * std::max(1, nsteps));
+  for (auto& e : ev) MDG_HIP(hipEventCreate(&e));
+  std::vector<int> used(8, 0);
+  int e = 0;
+  auto rec = [&](int k, auto&& fn) {
+    MDG_HIP(hipEventRecord(ev[e], s));
+    fn();
+    MDG_HIP(hipEventRecord(ev[e + 1], s));
+    used[k] = 1;
+    e += 2;
+    return MDG_OK;
+  };
+  std::vector<int> which;
+  for (int st = 0; st < nsteps; ++st) {
+    if (n_cellid_) { rec(0, [&] { launch_cell_build(d_desc_, d_ids_ + off_cellid_, n_cellid_, s); }); which.push_back(0); }
+    if (n_sk_) { rec(1, [&] { launch_phase_tables(mixed, d_desc_, d_items_ + off_tab_, n_tab_, s); }); which.push_back(1); }
+    rec(2, [&] { launch_real(s, 0); }); which.push_back(2);
+    if (n_sk_) {
+      rec(3, [&] {
+        if (mixed) launch_sk_partial32(d_desc_, d_items_ + off_sk_, n_sk_, s);
+        else launch_sk_partial64(d_desc_, d_items_ + off_sk_, n_sk_, s);
+      }); which.push_back(3);
+      rec(4, [&] { launch_sk_finalize(d_desc_, d_items_ + off_fin_, n_fin_, s); }); which.push_back(4);
+      rec(5, [&] {
+        if (mixed) launch_kforce32(d_desc_, d_items_ + off_kf_, n_kf_, s);
+        else launch_kforce64(d_desc_, d_items_ + off_kf_, n_kf_, s);
+      }); which.push_back(5);
+    }
+    rec(6, [&] { launch_force_vv(true, d_desc_, d_items_ + off_ion_, n_ion_, s); }); which.push_back(6);
+    rec(7, [&] { launch_box_scalars(d_desc_, d_ids_ + off_allid_, n_allid_, 1, s); }); which.push_back(7);
+  }
+  MDG_HIP(hipStreamSynchronize(s));
+  std::vector<double> tot(8, 0.0);
+  for (size_t q = 0; q < which.size(); ++q) {
+    float t = 0.f;
+    MDG_HIP(hipEventElapsedTime(&t, ev[2 * q], ev[2 * q + 1]));
+    tot[which[q]] += t;
+  }
+  for (auto& x : ev) (void)hipEventDestroy(x);
+  names->clear();
+  ms->clear();
+  for (int k = 0; k < 8; ++k)
+    if (used[k]) {
+      names->push_back(kn[k]);
+      ms->push_back(tot[k] / nsteps);
+    }
+  return MDG_OK;
+}
+
+}  // namespace mdg

@@ -1,0 +1,132 @@
+// engine.h -- host orchestration of libmdgpu: box plans, device-resident
+// boxes, and box sets (the work lists of one launch over many boxes).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "mdg_internal.h"
+#include "mdgpu.h"
+
+namespace mdg {
+
+// Everything that defines one box's computation.
+struct BoxSpec {
+  int N = 0;
+  double L[3] = {0, 0, 0};
+  double alpha = 0;       // Ewald alpha; 0 or > 10 skips Ewald
+  double a3 = 0;          // alpha / sqrt(pi) as mdc computes it
+  double p2a2 = 0;        // pi^2 / alpha^2
+  int hm = 0;
+  double rc = 0;
+  double rc2 = 0;         // cutoff squared; rc * rc when 0
+  double dt = 0, dt2 = 0; // mdc time units
+  int real_mode = MDG_TABLE;
+  int cells = MDG_CELLS_AUTO;
+  int kinds = 2;
+  double Z[4] = {0, 0, 0, 0}, m[4] = {0, 0, 0, 0};
+  int lookup[16] = {0, 1, 1, 2};
+  mdg_potential pot{};    // BMH parameters (analytic mode)
+  // pair table (table mode): ntypes * nb values each, mdc's layout
+  const double* pe = nullptr;
+  const double* fe = nullptr;
+  int table_nb = 0, table_ntypes = 3;
+};
+
+// Tile sizes of one box class; chosen per box from its own size only.
+struct Tiling {
+  int sk_items_target = 1024;  // structure-factor items a lone box aims for
+  int kf_items_target = 2048;  // back-projection items a lone box aims for
+  int sk_min_k = 16;           // k vectors per structure-factor item, at least
+  int kf_min_k = 64;           // k vectors per back-projection item, at least
+  int cells_min_n = 1500;      // MDG_CELLS_AUTO uses cell lists from this N
+};
+
+Tiling tiling_for(int N, int nk);
+
+struct BoxPlan {
+  BoxSpec spec;
+  bool ewald = false;
+  int hm_sqrt = 0, hmax = 0, nk = 0, ncol = 0;
+  std::vector<KCol> cols;
+  std::vector<double> kce, kcf;
+  bool all_pairs = true;
+  int nc[3] = {0, 0, 0}, ncell = 0;
+  int nreal = 0, nchunk = 0;
+  std::vector<int> sk_group_start;  // column index where each sk group starts (+ end)
+  std::vector<int> kf_group_start;
+};
+
+int make_plan(const BoxSpec& spec, int precision, BoxPlan* out, std::string* why);
+
+struct Box {
+  BoxPlan plan;
+  int precision = MDG_FP64;
+  BoxDesc desc{};
+  void* dmem = nullptr;
+  size_t bytes = 0;
+  std::vector<int> species;   // host copy (self term, masses)
+  bool desc_dirty = true;
+};
+
+int box_create(const BoxSpec& spec, int precision, Box** out, std::string* why);
+void box_destroy(Box* b);
+int box_upload_state(Box* b, const double* r, const double* v, const int* species,
+                     hipStream_t s);
+int box_download_state(Box* b, double* r, double* v, double* f, double* f0, hipStream_t s);
+int box_read_scalars(Box* b, double* scal /*kScalCount*/, hipStream_t s);
+int box_read_error(Box* b, int* err, hipStream_t s);
+
+// A set of boxes stepped together (one launch per kernel per step).
+class BoxSet {
+ public:
+  explicit BoxSet(int precision, int real_mode);
+  ~BoxSet();
+  int set_boxes(const std::vector<Box*>& boxes);  // rebuild work lists
+  const std::vector<Box*>& boxes() const { return boxes_; }
+  // Enqueue mdc's calc_foc (real_mode_write 0: f0 <- f, f <- forces).
+  // vv: also next_v/next_r; count_step: bump the step counter.
+  void enqueue_step(hipStream_t s, bool vv);
+  void enqueue_forces_parts(hipStream_t s, bool real, int real_write_mode, bool recip);
+  void enqueue_scalars(hipStream_t s, bool count_step);
+  void enqueue_clear_foc(hipStream_t s);
+  void enqueue_next_v(hipStream_t s);
+  void enqueue_next_r(hipStream_t s);
+  void enqueue_moment_correction(hipStream_t s);
+  int step(int nsteps, hipStream_t s);          // graph replay
+  int profile(int nsteps, hipStream_t s, std::vector<std::string>* names,
+              std::vector<double>* ms);
+  int sync_desc(hipStream_t s);                 // upload dirty descriptors
+  void invalidate_graph();
+
+ private:
+  int precision_, real_mode_;
+  std::vector<Box*> boxes_;
+  BoxDesc* d_desc_ = nullptr;
+  Item* d_items_ = nullptr;
+  int* d_ids_ = nullptr;
+  size_t items_cap_ = 0, ids_cap_ = 0, desc_cap_ = 0;
+  // offsets into d_items_ / d_ids_ and counts
+  int off_real_ = 0, n_real_ = 0, off_tab_ = 0, n_tab_ = 0, off_sk_ = 0, n_sk_ = 0;
+  int off_fin_ = 0, n_fin_ = 0, off_kf_ = 0, n_kf_ = 0, off_ion_ = 0, n_ion_ = 0;
+  int off_cellid_ = 0, n_cellid_ = 0, off_allid_ = 0, n_allid_ = 0;
+  hipGraphExec_t graph_exec_ = nullptr;
+  hipGraph_t graph_ = nullptr;
+  int graph_steps_ = 0;
+  void launch_real(hipStream_t s, int mode);
+};
+
+void set_last_hip_error(hipError_t e, const char* what);
+
+}  // namespace mdg
+
+#define MDG_HIP(x)                                       \
+  do {                                                   \
+    hipError_t e_ = (x);                                 \
+    if (e_ != hipSuccess) {                              \
+      ::mdg::set_last_hip_error(e_, #x);                 \
+      return MDG_E_HIP;                                  \
+    }                                                    \
+  } while (0)

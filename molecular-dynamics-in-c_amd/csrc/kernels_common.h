@@ -1,0 +1,339 @@
+// kernels_common.h -- FP64 kernels shared by both precision modes: cell
+// lists, phase tables, structure-factor finalisation, the velocity-Verlet
+// update and per-box reductions.  Included only by k_fp64.hip, which is
+// compiled with -ffp-contract=off: the integrator then rounds exactly as
+// mdc's next_v / next_r do on the host.
+#pragma once
+
+#include "kernels.h"
+
+namespace mdg {
+
+// Fixed-order workgroup sum of NV doubles (wave butterflies, then waves in
+// index order).  Result valid in thread 0.
+template <int NT, int NV>
+__device__ __forceinline__ void block_sum(double (&v)[NV]) {
+  __shared__ double sh[NT / 64][NV];
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+#pragma unroll
+  for (int k = 0; k < NV; ++k) v[k] = wave_sum(v[k]);
+  if (lane == 0) {
+#pragma unroll
+    for (int k = 0; k < NV; ++k) sh[w][k] = v[k];
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+      double s = 0.0;
+      for (int q = 0; q < NT / 64; ++q) s += sh[q][k];
+      v[k] = s;
+    }
+  }
+  __syncthreads();
+}
+
+// ----------------------------------------------------------------------
+// Cell list of one box per workgroup.  Binning is integer and exact:
+// c_a = (int)(r_a * (n_a / L_a)) clamped to n_a - 1 (r_a == L_a is a legal
+// coordinate after mdc's fold, rv.c:59-64).  Counting sort by cell, then
+// each cell's list sorted by ion index, so the order is deterministic.
+// Also writes cell-ordered copies of positions and species.
+// ----------------------------------------------------------------------
+__global__ __launch_bounds__(kCellThreads) void k_cell_build(const BoxDesc* __restrict__ boxes,
+                                                             const int* __restrict__ box_ids) {
+  __shared__ int sh[kCellThreads];
+  const BoxDesc& B = boxes[box_ids[blockIdx.x]];
+  const int N = B.N, ncell = B.ncell, tid = threadIdx.x;
+  const int ncx = B.nc[0], ncy = B.nc[1], ncz = B.nc[2];
+  int err = 0;
+  for (int c = tid; c < ncell; c += kCellThreads) B.cell_count[c] = 0;
+  __syncthreads();
+  for (int i = tid; i < N; i += kCellThreads) {
+    const double x = B.rx[i], y = B.ry[i], z = B.rz[i];
+    if (!(x >= 0.0 && x <= B.L[0] && y >= 0.0 && y <= B.L[1] && z >= 0.0 && z <= B.L[2]))
+      err |= kErrState;
+    int cx = (x >= 0.0) ? (int)(x * B.cellscale[0]) : 0;
+    int cy = (y >= 0.0) ? (int)(y * B.cellscale[1]) : 0;
+    int cz = (z >= 0.0) ? (int)(z * B.cellscale[2]) : 0;
+    cx = min(max(cx, 0), ncx - 1);
+    cy = min(max(cy, 0), ncy - 1);
+    cz = min(max(cz, 0), ncz - 1);
+    const int c = (cz * ncy + cy) * ncx + cx;
+    B.cell_of[i] = c;
+    atomicAdd(&B.cell_count[c], 1);
+  }
+  __syncthreads();
+  // exclusive scan of counts -> cell_start
+  const int per = (ncell + kCellThreads - 1) / kCellThreads;
+  const int c0 = min(tid * per, ncell), c1 = min(c0 + per, ncell);
+  int local = 0;
+  for (int c = c0; c < c1; ++c) local += B.cell_count[c];
+  sh[tid] = local;
+  __syncthreads();
+  for (int off = 1; off < kCellThreads; off <<= 1) {
+    const int add = tid >= off ? sh[tid - off] : 0;
+    __syncthreads();
+    sh[tid] += add;
+    __syncthreads();
+  }
+  int run = sh[tid] - local;
+  for (int c = c0; c < c1; ++c) {
+    B.cell_start[c] = run;
+    run += B.cell_count[c];
+    B.cell_count[c] = 0;  // becomes the fill cursor
+  }
+  if (tid == kCellThreads - 1) B.cell_start[ncell] = sh[tid];
+  __syncthreads();
+  for (int i = tid; i < N; i += kCellThreads) {
+    const int c = B.cell_of[i];
+    const int p = B.cell_start[c] + atomicAdd(&B.cell_count[c], 1);
+    B.sorted[p] = i;
+  }
+  __syncthreads();
+  for (int c = tid; c < ncell; c += kCellThreads) {  // insertion sort per cell
+    const int s = B.cell_start[c], e = B.cell_start[c + 1];
+    for (int p = s + 1; p < e; ++p) {
+      const int key = B.sorted[p];
+      int q = p - 1;
+      while (q >= s && B.sorted[q] > key) {
+        B.sorted[q + 1] = B.sorted[q];
+        --q;
+      }
+      B.sorted[q + 1] = key;
+    }
+  }
+  __syncthreads();
+  for (int p = tid; p < N; p += kCellThreads) {
+    const int i = B.sorted[p];
+    B.sx[p] = B.rx[i]; B.sy[p] = B.ry[i]; B.sz[p] = B.rz[i];
+    B.styp[p] = B.typ[i];
+  }
+  if (err) atomicOr(B.err, err);
+}
+
+// ----------------------------------------------------------------------
+// Per-ion phase tables X[h] = e^{2 pi i h x/Lx}, Y[h] = e^{2 pi i h y/Ly}
+// for h = 0..hmax and W = e^{2 pi i z/Lz}, from FP64 sincospi (argument
+// reduction by exact multiples of 2); stored FP64 or rounded to FP32.
+// ----------------------------------------------------------------------
+template <typename CT>
+__global__ __launch_bounds__(kIonThreads) void k_phase_tables(const BoxDesc* __restrict__ boxes,
+                                                              const Item* __restrict__ items) {
+  const Item it = items[blockIdx.x];
+  const BoxDesc& B = boxes[it.box];
+  const int N = B.N;
+  const int i = it.a + threadIdx.x;
+  if (i >= N) return;
+  const double s0 = B.rx[i] * B.invL[0], s1 = B.ry[i] * B.invL[1], s2 = B.rz[i] * B.invL[2];
+  CT* tx = reinterpret_cast<CT*>(B.tabx);
+  CT* ty = reinterpret_cast<CT*>(B.taby);
+  CT* tw = reinterpret_cast<CT*>(B.tabw);
+  for (int h = 0; h <= B.hmax; ++h) {
+    double sn, cs;
+    sincospi(2.0 * (double)h * s0, &sn, &cs);
+    tx[(size_t)h * N + i] = CT{(decltype(CT{}.x))cs, (decltype(CT{}.x))sn};
+    sincospi(2.0 * (double)h * s1, &sn, &cs);
+    ty[(size_t)h * N + i] = CT{(decltype(CT{}.x))cs, (decltype(CT{}.x))sn};
+  }
+  double sn, cs;
+  sincospi(2.0 * s2, &sn, &cs);
+  tw[i] = CT{(decltype(CT{}.x))cs, (decltype(CT{}.x))sn};
+}
+
+// ----------------------------------------------------------------------
+// Structure factor per k from the chunk partials (fixed chunk order), with
+// mdc's doubling of both sums (rcprcl.c:77-78), the energy term
+// vp exp(..)/m^2 (2Sc^2 + 2Ss^2) and the back-projection weights.
+// ----------------------------------------------------------------------
+__global__ __launch_bounds__(kIonThreads) void k_sk_finalize(const BoxDesc* __restrict__ boxes,
+                                                             const Item* __restrict__ items) {
+  const Item it = items[blockIdx.x];
+  const BoxDesc& B = boxes[it.box];
+  const int k = it.a + threadIdx.x;
+  if (k >= B.nk) return;
+  double sc = 0.0, ss = 0.0;
+  for (int ch = 0; ch < B.nchunk; ++ch) {
+    const double2 p = B.skpart[(size_t)ch * B.nk + k];
+    sc += p.x;
+    ss += p.y;
+  }
+  sc *= 2.0;
+  ss *= 2.0;
+  B.kpot[k] = B.kce[k] * (sc * sc + ss * ss);
+  const double a = B.kcf[k] * ss, b = B.kcf[k] * sc;
+  B.kab[2 * k] = a;
+  B.kab[2 * k + 1] = b;
+  if (B.kab32) {
+    B.kab32[2 * k] = (float)a;
+    B.kab32[2 * k + 1] = (float)b;
+  }
+}
+
+// ----------------------------------------------------------------------
+// Reciprocal forces added to f (groups in index order); with VV also
+// mdc's next_v then next_r (rv.c:33-66) in mdc's operation order, with the
+// one-fold strict-comparison wrap.
+// ----------------------------------------------------------------------
+template <int VV>
+__global__ __launch_bounds__(kIonThreads) void k_force_vv(const BoxDesc* __restrict__ boxes,
+                                                          const Item* __restrict__ items) {
+  const Item it = items[blockIdx.x];
+  const BoxDesc& B = boxes[it.box];
+  const int N = B.N;
+  const int i = it.a + threadIdx.x;
+  if (i >= N) return;
+  double fx = B.fx[i], fy = B.fy[i], fz = B.fz[i];
+  if (B.ewald) {
+    for (int g = 0; g < B.ngroup; ++g) {
+      const double* fp = B.fpart + (size_t)g * 3 * N;
+      fx += fp[i];
+      fy += fp[N + i];
+      fz += fp[2 * N + i];
+    }
+    B.fx[i] = fx; B.fy[i] = fy; B.fz[i] = fz;
+  }
+  if (VV) {
+    const double m2 = 2.0 * B.m[B.typ[i]];
+    const double vx = B.vx[i] + B.dt * (fx + B.fx0[i]) / m2;
+    const double vy = B.vy[i] + B.dt * (fy + B.fy0[i]) / m2;
+    const double vz = B.vz[i] + B.dt * (fz + B.fz0[i]) / m2;
+    double rx = B.rx[i] + (B.dt * vx + B.dt2 * fx / m2);
+    double ry = B.ry[i] + (B.dt * vy + B.dt2 * fy / m2);
+    double rz = B.rz[i] + (B.dt * vz + B.dt2 * fz / m2);
+    if (rx > B.L[0]) rx = rx - B.L[0];
+    if (ry > B.L[1]) ry = ry - B.L[1];
+    if (rz > B.L[2]) rz = rz - B.L[2];
+    if (rx < 0.0) rx = B.L[0] + rx;
+    if (ry < 0.0) ry = B.L[1] + ry;
+    if (rz < 0.0) rz = B.L[2] + rz;
+    if (!(rx >= 0.0 && rx <= B.L[0] && ry >= 0.0 && ry <= B.L[1] && rz >= 0.0 && rz <= B.L[2]))
+      atomicOr(B.err, kErrState);
+    B.vx[i] = vx; B.vy[i] = vy; B.vz[i] = vz;
+    B.rx[i] = rx; B.ry[i] = ry; B.rz[i] = rz;
+  }
+}
+
+// mdc's clear_foc (init.c:179-190), next_v (rv.c:33-44), next_r (rv.c:46-66)
+// as separate kernels for the one-routine-per-call drop-in.
+__global__ __launch_bounds__(kIonThreads) void k_clear_foc(const BoxDesc* __restrict__ boxes,
+                                                           const Item* __restrict__ items) {
+  const Item it = items[blockIdx.x];
+  const BoxDesc& B = boxes[it.box];
+  const int i = it.a + threadIdx.x;
+  if (i >= B.N) return;
+  B.fx0[i] = B.fx[i]; B.fy0[i] = B.fy[i]; B.fz0[i] = B.fz[i];
+  B.fx[i] = 0.0; B.fy[i] = 0.0; B.fz[i] = 0.0;
+}
+
+__global__ __launch_bounds__(kIonThreads) void k_next_v(const BoxDesc* __restrict__ boxes,
+                                                        const Item* __restrict__ items) {
+  const Item it = items[blockIdx.x];
+  const BoxDesc& B = boxes[it.box];
+  const int i = it.a + threadIdx.x;
+  if (i >= B.N) return;
+  const double m2 = 2.0 * B.m[B.typ[i]];
+  B.vx[i] += B.dt * (B.fx[i] + B.fx0[i]) / m2;
+  B.vy[i] += B.dt * (B.fy[i] + B.fy0[i]) / m2;
+  B.vz[i] += B.dt * (B.fz[i] + B.fz0[i]) / m2;
+}
+
+__global__ __launch_bounds__(kIonThreads) void k_next_r(const BoxDesc* __restrict__ boxes,
+                                                        const Item* __restrict__ items) {
+  const Item it = items[blockIdx.x];
+  const BoxDesc& B = boxes[it.box];
+  const int i = it.a + threadIdx.x;
+  if (i >= B.N) return;
+  const double m2 = 2.0 * B.m[B.typ[i]];
+  double rx = B.rx[i] + (B.dt * B.vx[i] + B.dt2 * B.fx[i] / m2);
+  double ry = B.ry[i] + (B.dt * B.vy[i] + B.dt2 * B.fy[i] / m2);
+  double rz = B.rz[i] + (B.dt * B.vz[i] + B.dt2 * B.fz[i] / m2);
+  if (rx > B.L[0]) rx = rx - B.L[0];
+  if (ry > B.L[1]) ry = ry - B.L[1];
+  if (rz > B.L[2]) rz = rz - B.L[2];
+  if (rx < 0.0) rx = B.L[0] + rx;
+  if (ry < 0.0) ry = B.L[1] + ry;
+  if (rz < 0.0) rz = B.L[2] + rz;
+  B.rx[i] = rx; B.ry[i] = ry; B.rz[i] = rz;
+}
+
+// ----------------------------------------------------------------------
+// Per-box scalars (one workgroup per box, fixed summation order):
+// real-space pot and diagonal virial from the item partials, reciprocal
+// energy from the per-k terms, kinetic energy as mdc's calc_kin
+// (rv.c:68-86), pressure as calc_press (pt.c:8-21).
+// flags: bit0 count a completed step, bit1 include reciprocal/self terms.
+// ----------------------------------------------------------------------
+__global__ __launch_bounds__(kScalThreads) void k_box_scalars(const BoxDesc* __restrict__ boxes,
+                                                              const int* __restrict__ box_ids,
+                                                              int flags) {
+  const BoxDesc& B = boxes[box_ids[blockIdx.x]];
+  const int tid = threadIdx.x;
+  double v[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  for (int s = tid; s < B.nreal; s += kScalThreads) {
+    v[0] += B.rpart[4 * s];
+    v[1] += B.rpart[4 * s + 1];
+    v[2] += B.rpart[4 * s + 2];
+    v[3] += B.rpart[4 * s + 3];
+    v[8] += (double)B.rpairs[s];
+  }
+  if (B.ewald)
+    for (int k = tid; k < B.nk; k += kScalThreads) v[4] += B.kpot[k];
+  for (int i = tid; i < B.N; i += kScalThreads) {
+    const double m = B.m[B.typ[i]];
+    v[5] += m * B.vx[i] * B.vx[i];
+    v[6] += m * B.vy[i] * B.vy[i];
+    v[7] += m * B.vz[i] * B.vz[i];
+  }
+  block_sum<kScalThreads, 9>(v);
+  if (tid == 0) {
+    double* S = B.scal;
+    const double pself = B.ewald ? B.self_pot : 0.0;
+    const double precip = B.ewald ? v[4] : 0.0;
+    S[kScalPotReal] = v[0];
+    S[kScalPotRecip] = precip;
+    S[kScalPotSelf] = pself;
+    S[kScalPot] = v[0] + precip + pself;
+    S[kScalVirX] = v[1]; S[kScalVirY] = v[2]; S[kScalVirZ] = v[3];
+    S[kScalKinX] = v[5] / 2.0; S[kScalKinY] = v[6] / 2.0; S[kScalKinZ] = v[7] / 2.0;
+    const double kin = S[kScalKinX] + S[kScalKinY] + S[kScalKinZ];
+    S[kScalKin] = kin;
+    S[kScalEtot] = kin + v[0] + 0.25 * precip + pself;
+    S[kScalPresX] = (2.0 * S[kScalKinX] + v[1]) / B.vol;
+    S[kScalPresY] = (2.0 * S[kScalKinY] + v[2]) / B.vol;
+    S[kScalPresZ] = (2.0 * S[kScalKinZ] + v[3]) / B.vol;
+    S[kScalPres] = (S[kScalPresX] + S[kScalPresY] + S[kScalPresZ]) / 3.0;
+    if (flags & 1) S[kScalStep] += 1.0;
+    S[kScalPairs] = v[8];
+  }
+}
+
+// mdc's moment_correction (rv.c:88-116) for one box per workgroup.
+__global__ __launch_bounds__(kScalThreads) void k_moment_correction(
+    const BoxDesc* __restrict__ boxes, const int* __restrict__ box_ids) {
+  __shared__ double mean[3];
+  const BoxDesc& B = boxes[box_ids[blockIdx.x]];
+  const int tid = threadIdx.x;
+  double v[3] = {0, 0, 0};
+  for (int i = tid; i < B.N; i += kScalThreads) {
+    const double m = B.m[B.typ[i]];
+    v[0] += m * B.vx[i];
+    v[1] += m * B.vy[i];
+    v[2] += m * B.vz[i];
+  }
+  block_sum<kScalThreads, 3>(v);
+  if (tid == 0) {
+    mean[0] = v[0] / (double)B.N;
+    mean[1] = v[1] / (double)B.N;
+    mean[2] = v[2] / (double)B.N;
+  }
+  __syncthreads();
+  for (int i = tid; i < B.N; i += kScalThreads) {
+    const double m = B.m[B.typ[i]];
+    B.vx[i] -= mean[0] / m;
+    B.vy[i] -= mean[1] / m;
+    B.vz[i] -= mean[2] / m;
+  }
+}
+
+}  // namespace mdg

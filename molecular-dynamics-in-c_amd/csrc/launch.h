@@ -1,0 +1,30 @@
+// launch.h -- host-side launchers of libmdgpu's kernels.  Each launches one
+// workgroup per work item on `s`; the caller owns the item arrays.
+#pragma once
+
+#include "mdg_internal.h"
+
+namespace mdg {
+
+// k_fp64.hip (compiled with -ffp-contract=off)
+void launch_cell_build(const BoxDesc* boxes, const int* ids, int n, hipStream_t s);
+void launch_phase_tables(bool fp32, const BoxDesc* boxes, const Item* items, int n, hipStream_t s);
+void launch_real64(bool analytic, const BoxDesc* boxes, const Item* items, int n, int mode,
+                   hipStream_t s);
+void launch_sk_partial64(const BoxDesc* boxes, const Item* items, int n, hipStream_t s);
+void launch_kforce64(const BoxDesc* boxes, const Item* items, int n, hipStream_t s);
+void launch_sk_finalize(const BoxDesc* boxes, const Item* items, int n, hipStream_t s);
+void launch_force_vv(bool vv, const BoxDesc* boxes, const Item* items, int n, hipStream_t s);
+void launch_clear_foc(const BoxDesc* boxes, const Item* items, int n, hipStream_t s);
+void launch_next_v(const BoxDesc* boxes, const Item* items, int n, hipStream_t s);
+void launch_next_r(const BoxDesc* boxes, const Item* items, int n, hipStream_t s);
+void launch_box_scalars(const BoxDesc* boxes, const int* ids, int n, int flags, hipStream_t s);
+void launch_moment_correction(const BoxDesc* boxes, const int* ids, int n, hipStream_t s);
+
+// k_mixed.hip (FP32 pair and k-space arithmetic)
+void launch_real32(bool analytic, const BoxDesc* boxes, const Item* items, int n, int mode,
+                   hipStream_t s);
+void launch_sk_partial32(const BoxDesc* boxes, const Item* items, int n, hipStream_t s);
+void launch_kforce32(const BoxDesc* boxes, const Item* items, int n, hipStream_t s);
+
+}  // namespace mdg

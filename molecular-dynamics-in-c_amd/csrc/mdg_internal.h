@@ -1,0 +1,113 @@
+// mdg_internal.h -- types shared by the host orchestration and the
+// CDNA4 kernels of libmdgpu.
+//
+// Data layout in HBM.  Every box owns one device allocation carved into
+// SoA arrays (positions, velocities, forces and previous forces as FP64
+// x[N], y[N], z[N]; species as int32), its k-space data (k columns,
+// per-k coefficients, per-ion phase tables, partial sums) and its cell
+// list.  A BoxDesc holds the device pointers and constants of one box; a
+// launch over many boxes reads one BoxDesc per work item, so boxes of any
+// size share a launch without repacking their arrays.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace mdg {
+
+constexpr int kRealThreads = 64;    // one wave per real-space work item
+constexpr int kSkThreads = 256;     // structure-factor workgroup
+constexpr int kSkIpt = 4;           // ions per thread in the S(k) pass
+constexpr int kSkChunk = kSkThreads * kSkIpt;
+constexpr int kSkSub = 8;           // k vectors per register sub-block
+constexpr int kKfThreads = 256;     // back-projection workgroup (one ion per thread)
+constexpr int kIonThreads = 256;    // per-ion streaming kernels
+constexpr int kCellThreads = 1024;  // cell-list build, one workgroup per box
+constexpr int kScalThreads = 256;   // per-box scalar reduction
+
+// error bits written by kernels into BoxDesc::err
+constexpr int kErrState = 1;        // non-finite / out-of-box coordinate
+constexpr int kErrRange = 2;        // pair closer than the table start
+
+// One column of the half-sphere k set: fixed (hx, hy), hz = hz0 .. hz0+len-1.
+// Columns are ordered by hx, then hy, as mdc's loops visit them
+// (mdc/program/rcprcl.c:32-38), so the k index order equals mdc's.
+struct KCol {
+  int hx, hy, hz0, len;
+  int k0;            // index of the column's first k in the per-k arrays
+  int pad[3];
+};
+
+// Work item of a launch over many boxes.
+struct Item {
+  int box;   // index into the BoxDesc array of the launch
+  int a;     // first ion / cell / chunk / k, by kernel
+  int b;     // partial-sum slot or column range start
+  int c;     // column count
+  int d;     // group index (back-projection partial slot)
+};
+
+struct BoxDesc {
+  // ---- ion state (N each) ----
+  double *rx, *ry, *rz, *vx, *vy, *vz, *fx, *fy, *fz, *fx0, *fy0, *fz0;
+  int *typ;
+  int N;
+  int ewald;          // 0: Ewald skipped (alpha == 0 or alpha > 10)
+  int real_mode;      // 0 table, 1 analytic
+  int all_pairs;      // 1: O(N^2) pair loop in mdc's order; 0: cell list
+  int kinds;          // species count (table lookup stride)
+  int lookup[16];     // pair type of (ti, tj): lookup[ti*kinds + tj]
+  // ---- geometry and Ewald constants ----
+  double L[3], halfL[3], invL[3];
+  double rc2;
+  double alpha, a3;   // alpha, alpha/sqrt(pi)
+  double dt, dt2;
+  double Z[4], m[4];
+  double self_pot;    // -a3 * sum Z^2
+  double vol;
+  // ---- BMH (analytic mode) per pair type ----
+  double cb[3], sig[3], rho, Cd[3], Dd[3];
+  // ---- pair table (table mode): {pe, fe} per 0.001 A bin ----
+  const double2 *table;
+  const float2 *table32;
+  int table_nb;       // bins per pair type (table_division + 1)
+  // ---- cell list ----
+  int nc[3], ncell;
+  double cellscale[3];  // nc / L
+  int *cell_of;       // [N]
+  int *cell_count;    // [ncell]
+  int *cell_start;    // [ncell + 1]
+  int *sorted;        // [N] ion index in cell order (ascending within a cell)
+  double *sx, *sy, *sz;
+  int *styp;
+  // ---- k space ----
+  int nk, ncol, hmax;
+  const KCol *cols;
+  const double *kce;  // per k: vp * exp(-pi^2 m^2/alpha^2) / m^2  (energy)
+  const double *kcf;  // per k: v2 * exp(...) / m^2                  (force)
+  void *tabx, *taby;  // phase tables e^{2 pi i h x/L}, h = 0..hmax: [(hmax+1)][N] complex
+  void *tabw;         // e^{2 pi i z/L}: [N] complex
+  double2 *skpart;    // [nchunk][nk] partial structure factors
+  int nchunk;
+  double *kab;        // [nk][2]: cf*2Ss, cf*2Sc  (FP64)
+  float *kab32;       // same, FP32 (mixed)
+  double *kpot;       // [nk] energy terms
+  double *fpart;      // [ngroup][3][N] back-projection partial forces
+  int ngroup;
+  // ---- reductions ----
+  double *rpart;      // [nreal][4]: pot, vir x, y, z per real-space item
+  unsigned long long *rpairs;  // [nreal] accepted unordered pairs per item
+  int nreal;
+  double *scal;       // kScal* layout below
+  int *err;
+};
+
+// per-box scalar layout (device double array)
+enum {
+  kScalPot = 0, kScalPotReal, kScalPotRecip, kScalPotSelf, kScalEtot,
+  kScalVirX, kScalVirY, kScalVirZ, kScalKin, kScalKinX, kScalKinY, kScalKinZ,
+  kScalPres, kScalPresX, kScalPresY, kScalPresZ, kScalStep, kScalPairs,
+  kScalCount
+};
+
+}  // namespace mdg

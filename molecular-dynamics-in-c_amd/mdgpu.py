@@ -1,0 +1,350 @@
+"""ctypes binding of libmdgpu's C ABI (include/mdgpu.h, include/mdc_abi.h).
+
+This is the Python side a test, benchmark or notebook uses; the product is
+the C library.  Nothing here computes: every call goes to lib/libmdgpu.so,
+and a missing library raises instead of falling back to anything else.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+PKG_DIR = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG_DIR, "lib", "libmdgpu.so")
+
+OK, E_NODEVICE, E_HIP, E_ARG, E_NOMEM, E_BUDGET, E_STATE, E_RANGE, E_TIMEOUT = range(9)
+FP64, MIXED = 0, 1
+TABLE, ANALYTIC = 0, 1
+CELLS_AUTO, ALL_PAIRS, CELL_LIST = -1, 0, 1
+
+SECD_2_FS = 4.074972637944947
+KB = 1.380658e-5
+
+
+class MdgError(RuntimeError):
+    def __init__(self, code, what):
+        super().__init__(f"{what}: {errstr(code)} (code {code})")
+        self.code = code
+
+
+class Potential(C.Structure):
+    _fields_ = [("Z", C.c_double * 2), ("m", C.c_double * 2), ("b", C.c_double),
+                ("rho", C.c_double), ("c", C.c_double * 3), ("sigma", C.c_double * 3),
+                ("C", C.c_double * 3), ("D", C.c_double * 3)]
+
+
+class BoxParams(C.Structure):
+    _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int),
+                ("ax", C.c_double), ("ay", C.c_double), ("az", C.c_double),
+                ("alpha", C.c_double), ("hm", C.c_int), ("rc", C.c_double),
+                ("dt_fs", C.c_double), ("real_mode", C.c_int), ("cells", C.c_int),
+                ("pot", Potential)]
+
+
+class Scalars(C.Structure):
+    _fields_ = [("pot", C.c_double), ("pot_real", C.c_double), ("pot_recip", C.c_double),
+                ("pot_self", C.c_double), ("etot", C.c_double), ("vir", C.c_double * 3),
+                ("kin", C.c_double), ("kin_xyz", C.c_double * 3), ("pres", C.c_double),
+                ("pres_xyz", C.c_double * 3), ("step", C.c_longlong)]
+
+    def as_dict(self):
+        return dict(pot=self.pot, pot_real=self.pot_real, pot_recip=self.pot_recip,
+                    pot_self=self.pot_self, etot=self.etot, vir=list(self.vir), kin=self.kin,
+                    kin_xyz=list(self.kin_xyz), pres=self.pres, pres_xyz=list(self.pres_xyz),
+                    step=self.step)
+
+
+class Job(C.Structure):
+    _fields_ = [("arrival_s", C.c_double), ("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int),
+                ("temp_k", C.c_double), ("steps", C.c_int), ("seed", C.c_ulonglong)]
+
+
+class ServeConfig(C.Structure):
+    _fields_ = [("hbm_budget", C.c_size_t), ("max_slots", C.c_int), ("precision", C.c_int),
+                ("real_mode", C.c_int), ("eps", C.c_double), ("rc_max", C.c_double),
+                ("dt_fs", C.c_double), ("one_at_a_time", C.c_int), ("realtime", C.c_int),
+                ("steps_per_tick", C.c_int)]
+
+
+class JobResult(C.Structure):
+    _fields_ = [("admit_s", C.c_double), ("finish_s", C.c_double), ("latency_s", C.c_double),
+                ("etot_first", C.c_double), ("etot_last", C.c_double),
+                ("final_checksum", C.c_double), ("status", C.c_int)]
+
+
+class ServeStats(C.Structure):
+    _fields_ = [("wall_s", C.c_double), ("ion_steps", C.c_double),
+                ("ion_steps_per_s", C.c_double), ("p50_latency_s", C.c_double),
+                ("p99_latency_s", C.c_double), ("mean_latency_s", C.c_double),
+                ("max_resident", C.c_int), ("ticks", C.c_int), ("rejected", C.c_int),
+                ("peak_bytes", C.c_size_t)]
+
+
+# ---- mdc's globals (mdc/program/md.h:23-144, potential.h:2-28) ----------
+_dp = C.POINTER(C.c_double)
+
+
+class CalcControl(C.Structure):
+    _fields_ = [("delta_time_fs", C.c_double), ("calc_max", C.c_int), ("temp", C.c_double),
+                ("t_control_step", C.c_int), ("p_control_step", C.c_int),
+                ("set_press_GPa_X", C.c_double), ("set_press_GPa_Y", C.c_double),
+                ("set_press_GPa_Z", C.c_double), ("press_X", C.c_double),
+                ("press_Y", C.c_double), ("press_Z", C.c_double), ("kinds_of_ions", C.c_int),
+                ("natoms_in_mol_unit", C.c_int), ("natoms_in_unit_cell", C.c_int),
+                ("kinds_of_interaction", C.c_int)]
+
+
+class SystemProperty(C.Structure):
+    _fields_ = [("rx", _dp), ("ry", _dp), ("rz", _dp), ("vx", _dp), ("vy", _dp), ("vz", _dp),
+                ("fx0", _dp), ("fy0", _dp), ("fz0", _dp), ("fx", _dp), ("fy", _dp), ("fz", _dp),
+                ("ion", C.POINTER(C.c_short)), ("N", C.c_short), ("nx", C.c_short),
+                ("ny", C.c_short), ("nz", C.c_short),
+                ("Ax", C.c_double), ("Ay", C.c_double), ("Az", C.c_double),
+                ("Lx", C.c_double), ("Ly", C.c_double), ("Lz", C.c_double),
+                ("kB", C.c_double), ("kk", C.c_double), ("perMol", C.c_double),
+                ("e2t", C.c_double), ("e2tX", C.c_double), ("e2tY", C.c_double),
+                ("e2tZ", C.c_double), ("pp2gpa", C.c_double), ("step", C.c_int),
+                ("dt", C.c_double), ("dt2", C.c_double), ("a1", C.c_double),
+                ("p2a2", C.c_double), ("a3", C.c_double), ("hm", C.c_short),
+                ("hm_sqrt", C.c_short), ("radius", C.c_double), ("radius2", C.c_double),
+                ("rcsi", _dp), ("rsni", _dp), ("table_division", C.c_int),
+                ("pe1r1", _dp), ("fe1r1", _dp), ("lookup", C.POINTER(C.c_int)),
+                ("virX", C.c_double), ("virY", C.c_double), ("virZ", C.c_double),
+                ("presX", C.c_double), ("presY", C.c_double), ("presZ", C.c_double),
+                ("pres", C.c_double), ("pres_scalling_factor", C.c_double),
+                ("pot", C.c_double), ("kin", C.c_double), ("kinX", C.c_double),
+                ("kinY", C.c_double), ("kinZ", C.c_double)]
+
+
+class IonProperty(C.Structure):
+    _fields_ = [("m", _dp), ("Z", _dp)]
+
+
+class HugginsMayer(C.Structure):
+    _fields_ = [(n, C.c_double) for n in (
+        "b", "c_na_na", "c_na_cl", "c_cl_cl", "sigma_na", "sigma_cl", "rho", "C_na_na",
+        "C_na_cl", "C_cl_cl", "D_na_na", "D_na_cl", "D_cl_cl")]
+
+
+DROPIN_ROUTINES = ("calc_foc", "clear_foc", "real_space", "reciprocal_space",
+                   "reciprocal_space3", "next_rv_verlet", "next_r", "next_v", "calc_kin",
+                   "moment_correction", "calc_press", "mk_table")
+
+_lib = None
+
+
+def lib():
+    """Load lib/libmdgpu.so (raises if it was not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} missing: build it with __graft_entry__.build() "
+                               "or `make -C molecular-dynamics-in-c_amd`")
+        L = C.CDLL(LIB_PATH)
+        vp = C.c_void_p
+        sig = {
+            "mdg_device_count": ([C.POINTER(C.c_int)], C.c_int),
+            "mdg_potential_nacl": ([C.POINTER(Potential)], C.c_int),
+            "mdg_potential_kcl": ([C.POINTER(Potential)], C.c_int),
+            "mdg_rocksalt": ([C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,
+                              _dp, C.POINTER(C.c_int)], C.c_int),
+            "mdg_maxwell": ([C.c_ulonglong, C.c_double, C.c_int, C.POINTER(C.c_int),
+                             C.POINTER(Potential), _dp], C.c_int),
+            "mdg_ewald_params": ([C.c_double, C.c_double, C.c_double, _dp,
+                                  C.POINTER(C.c_int)], C.c_int),
+            "mdg_batch_create": ([C.c_int, C.POINTER(BoxParams), C.c_int, C.POINTER(vp)],
+                                 C.c_int),
+            "mdg_batch_destroy": ([vp], C.c_int),
+            "mdg_batch_n_ions": ([vp, C.c_int], C.c_int),
+            "mdg_batch_box_bytes": ([vp, C.c_int], C.c_size_t),
+            "mdg_batch_set_state": ([vp, C.c_int, _dp, _dp, C.POINTER(C.c_int)], C.c_int),
+            "mdg_batch_get_state": ([vp, C.c_int, _dp, _dp, _dp, _dp], C.c_int),
+            "mdg_batch_forces": ([vp], C.c_int),
+            "mdg_batch_step": ([vp, C.c_int], C.c_int),
+            "mdg_batch_scalars": ([vp, C.c_int, C.POINTER(Scalars)], C.c_int),
+            "mdg_batch_sync": ([vp], C.c_int),
+            "mdg_batch_profile": ([vp, C.c_int, C.c_int, C.c_void_p, _dp,
+                                   C.POINTER(C.c_int)], C.c_int),
+            "mdg_batch_work": ([vp, C.c_int, C.POINTER(C.c_longlong), C.POINTER(C.c_int),
+                                C.POINTER(C.c_int)], C.c_int),
+            "mdg_error_string": ([C.c_int], C.c_char_p),
+            "mdg_serve": ([C.POINTER(Job), C.c_int, C.POINTER(ServeConfig),
+                           C.POINTER(JobResult), C.POINTER(ServeStats)], C.c_int),
+            "mdg_dropin_status": ([], C.c_int),
+            "mdg_dropin_clear_status": ([], None),
+            "mdg_dropin_release": ([], None),
+            "erfcc": ([C.c_double], C.c_double),
+        }
+        for name, (args, res) in sig.items():
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = res
+        for name in DROPIN_ROUTINES:
+            f = getattr(L, name)
+            f.argtypes = []
+            f.restype = None
+        _lib = L
+    return _lib
+
+
+def errstr(code):
+    try:
+        return lib().mdg_error_string(code).decode()
+    except Exception:  # library missing
+        return f"error {code}"
+
+
+def check(code, what):
+    if code != OK:
+        raise MdgError(code, what)
+
+
+def device_count():
+    n = C.c_int(0)
+    rc = lib().mdg_device_count(C.byref(n))
+    return n.value if rc == OK else 0
+
+
+def potential(salt="NaCl"):
+    p = Potential()
+    fn = {"NaCl": lib().mdg_potential_nacl, "KCl": lib().mdg_potential_kcl}[salt]
+    check(fn(C.byref(p)), "potential")
+    return p
+
+
+def ewald_params(rc, l_max, eps):
+    a, h = C.c_double(), C.c_int()
+    check(lib().mdg_ewald_params(rc, l_max, eps, C.byref(a), C.byref(h)), "ewald_params")
+    return a.value, h.value
+
+
+def rocksalt(nx, ny, nz, a):
+    ax, ay, az = (a, a, a) if np.isscalar(a) else a
+    n = 8 * nx * ny * nz
+    r = np.zeros(3 * n)
+    sp = np.zeros(n, dtype=np.int32)
+    check(lib().mdg_rocksalt(nx, ny, nz, ax, ay, az, _p(r), _ip(sp)), "rocksalt")
+    return r.reshape(3, n), sp
+
+
+def maxwell(seed, temp_k, species, pot):
+    species = np.ascontiguousarray(species, dtype=np.int32)
+    v = np.zeros(3 * species.size)
+    check(lib().mdg_maxwell(seed, temp_k, species.size, _ip(species), C.byref(pot), _p(v)),
+          "maxwell")
+    return v.reshape(3, species.size)
+
+
+def box_params(nx, ny, nz, a, alpha, hm, rc, dt_fs=2.0, real_mode=TABLE, cells=CELLS_AUTO,
+               salt="NaCl"):
+    ax, ay, az = (a, a, a) if np.isscalar(a) else a
+    p = BoxParams()
+    p.nx, p.ny, p.nz = nx, ny, nz
+    p.ax, p.ay, p.az = ax, ay, az
+    p.alpha, p.hm, p.rc, p.dt_fs = alpha, hm, rc, dt_fs
+    p.real_mode, p.cells = real_mode, cells
+    p.pot = potential(salt) if isinstance(salt, str) else salt
+    return p
+
+
+def _p(a):
+    return a.ctypes.data_as(_dp)
+
+
+def _ip(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int))
+
+
+class Batch:
+    """A set of device-resident boxes stepped together."""
+
+    def __init__(self, params, precision=FP64):
+        arr = (BoxParams * len(params))(*params)
+        h = C.c_void_p()
+        check(lib().mdg_batch_create(len(params), arr, precision, C.byref(h)), "batch_create")
+        self.h = h
+        self.nbox = len(params)
+        self.precision = precision
+
+    def close(self):
+        if self.h:
+            lib().mdg_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def n_ions(self, b=0):
+        return lib().mdg_batch_n_ions(self.h, b)
+
+    def box_bytes(self, b=0):
+        return lib().mdg_batch_box_bytes(self.h, b)
+
+    def set_state(self, b=0, r=None, v=None, species=None):
+        n = self.n_ions(b)
+        ra = None if r is None else np.ascontiguousarray(np.asarray(r, dtype=np.float64).reshape(3 * n))
+        va = None if v is None else np.ascontiguousarray(np.asarray(v, dtype=np.float64).reshape(3 * n))
+        sa = None if species is None else np.ascontiguousarray(species, dtype=np.int32)
+        check(lib().mdg_batch_set_state(self.h, b, None if ra is None else _p(ra),
+                                        None if va is None else _p(va),
+                                        None if sa is None else _ip(sa)), "set_state")
+
+    def state(self, b=0):
+        n = self.n_ions(b)
+        out = [np.zeros(3 * n) for _ in range(4)]
+        check(lib().mdg_batch_get_state(self.h, b, *(_p(x) for x in out)), "get_state")
+        r, v, f, f0 = (x.reshape(3, n) for x in out)
+        return dict(r=r, v=v, f=f, f0=f0)
+
+    def forces(self):
+        check(lib().mdg_batch_forces(self.h), "forces")
+
+    def step(self, n):
+        check(lib().mdg_batch_step(self.h, n), "step")
+
+    def sync(self):
+        check(lib().mdg_batch_sync(self.h), "sync")
+
+    def scalars(self, b=0):
+        s = Scalars()
+        check(lib().mdg_batch_scalars(self.h, b, C.byref(s)), "scalars")
+        return s.as_dict()
+
+    def work(self, b=0):
+        p, k, n = C.c_longlong(), C.c_int(), C.c_int()
+        check(lib().mdg_batch_work(self.h, b, C.byref(p), C.byref(k), C.byref(n)), "work")
+        return dict(pairs=p.value, nk=k.value, n_ions=n.value)
+
+    def profile(self, nsteps):
+        cap = 16
+        names = (C.c_char * 32 * cap)()
+        ms = np.zeros(cap)
+        n = C.c_int()
+        check(lib().mdg_batch_profile(self.h, nsteps, cap, names, _p(ms), C.byref(n)), "profile")
+        return {bytes(names[k]).split(b"\0")[0].decode(): float(ms[k]) for k in range(n.value)}
+
+
+def serve(jobs, cfg):
+    arr = (Job * len(jobs))(*jobs)
+    res = (JobResult * len(jobs))()
+    st = ServeStats()
+    check(lib().mdg_serve(arr, len(jobs), C.byref(cfg), res, C.byref(st)), "serve")
+    return list(res), st
+
+
+# ---- the drop-in surface -------------------------------------------------
+def dropin_globals():
+    L = lib()
+    return (SystemProperty.in_dll(L, "sys"), IonProperty.in_dll(L, "ion"),
+            CalcControl.in_dll(L, "ctl"), HugginsMayer.in_dll(L, "hm"))
+
+
+def dropin_call(name):
+    getattr(lib(), name)()
+    st = lib().mdg_dropin_status()
+    if st != OK:
+        lib().mdg_dropin_clear_status()
+        raise MdgError(st, name)

@@ -1,0 +1,134 @@
+"""The C-ABI library: loads, exports what include/*.h declares, lays out
+mdc's globals as mdc does, and its host-side setup matches the oracle.
+No device computation here (these run without a GPU)."""
+import ctypes as C
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+import mdgpu
+import mdref
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+INC = os.path.join(ROOT, "include")
+MDC_PROGRAM = "/root/reference/mdc/program"
+
+
+def declared_symbols():
+    names = set()
+    for h in ("mdgpu.h", "mdc_abi.h"):
+        src = open(os.path.join(INC, h)).read()
+        src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+        for m in re.finditer(r"^\s*(?:const\s+)?(?:int|void|double|size_t|char)\s*\*?\s*(\w+)\s*\(",
+                             src, flags=re.M):
+            names.add(m.group(1))
+        for m in re.finditer(r"^extern\s+\w+\s+(\w+);", src, flags=re.M):
+            names.add(m.group(1))
+    return names
+
+
+def exported_symbols():
+    out = subprocess.run(["nm", "-D", "--defined-only", mdgpu.LIB_PATH], check=True,
+                         capture_output=True, text=True).stdout
+    return {ln.split()[-1] for ln in out.splitlines() if ln.strip()}
+
+
+def test_library_exports_every_declared_symbol():
+    decl = declared_symbols()
+    assert {"calc_foc", "real_space", "reciprocal_space", "next_rv_verlet", "sys",
+            "mdg_batch_step", "mdg_serve"} <= decl
+    exp = exported_symbols()
+    assert decl <= exp, sorted(decl - exp)
+    # nothing but the C ABI leaks out of the library
+    assert exp <= decl, sorted(exp - decl)
+    mdgpu.lib()   # loads with all its dependencies
+
+
+def _layout(tmp_path, mdc_headers):
+    exe = tmp_path / ("probe_mdc" if mdc_headers else "probe_abi")
+    cmd = ["gcc", "-std=c99", "-fcommon", "-o", str(exe), os.path.join(ROOT, "tests", "abi_layout.c")]
+    cmd += ["-DUSE_MDC_HEADERS", "-I" + MDC_PROGRAM] if mdc_headers else ["-I" + INC]
+    subprocess.run(cmd, check=True, capture_output=True)
+    return subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout
+
+
+def test_ctypes_mirror_matches_header_layout(tmp_path):
+    lines = dict(ln.rsplit(" ", 1) for ln in _layout(tmp_path, False).splitlines())
+    for cname, cls in (("system_property", mdgpu.SystemProperty),
+                       ("calc_control", mdgpu.CalcControl),
+                       ("ion_property", mdgpu.IonProperty),
+                       ("huggins_mayer_potential_set", mdgpu.HugginsMayer)):
+        assert int(lines[cname]) == C.sizeof(cls), cname
+        for f, _ in cls._fields_:
+            assert int(lines[f"{cname}.{f}"]) == getattr(cls, f).offset, (cname, f)
+
+
+@pytest.mark.skipif(not os.path.isdir(MDC_PROGRAM), reason="reference checkout absent")
+def test_header_layout_is_mdc_layout(tmp_path):
+    assert _layout(tmp_path, False) == _layout(tmp_path, True)
+
+
+def test_host_setup_matches_oracle():
+    L = mdgpu.lib()
+    for x in (0.05, 0.3, 1.1, 2.7, 4.9):
+        assert L.erfcc(x) == mdref.lib().orc_erfcc(x)
+    o = mdref.Oracle(3, 2, 4, 5.64, 0.3, 27, 5.0)
+    r, sp = mdgpu.rocksalt(3, 2, 4, 5.64)
+    assert np.array_equal(r, o.r) and np.array_equal(sp, o.ion)
+    p = mdgpu.potential("NaCl")
+    for f in ("Z", "m", "c", "sigma", "C", "D"):
+        assert list(getattr(p, f)) == list(getattr(o.b.pp, f)), f
+    assert (p.b, p.rho) == (o.b.pp.b, o.b.pp.rho)
+    k = mdgpu.potential("KCl")
+    ok = mdref.Oracle(2, 2, 2, 6.29, 0.3, 27, 5.0, salt="KCl")
+    for f in ("Z", "m", "c", "sigma", "C", "D"):
+        assert list(getattr(k, f)) == list(getattr(ok.b.pp, f)), f
+
+
+def test_dropin_mk_table_matches_oracle():
+    """mk_table through mdc's globals (host routine of the drop-in)."""
+    sysg, iong, ctlg, hmg = mdgpu.dropin_globals()
+    o = mdref.Oracle(3, 3, 3, 5.64, 5.6 / 16.92, 27, 8.46)
+    nb = o.b.table_division + 1
+    pe, fe = np.zeros(3 * nb), np.zeros(3 * nb)
+    lookup = np.zeros(4, dtype=np.int32)
+    Z = np.array(list(o.b.pp.Z))
+    m = np.array(list(o.b.pp.m))
+    sysg.pe1r1 = pe.ctypes.data_as(C.POINTER(C.c_double))
+    sysg.fe1r1 = fe.ctypes.data_as(C.POINTER(C.c_double))
+    sysg.lookup = lookup.ctypes.data_as(C.POINTER(C.c_int))
+    sysg.table_division = o.b.table_division
+    sysg.a1, sysg.a3 = o.b.a1, o.b.a3
+    iong.Z = Z.ctypes.data_as(C.POINTER(C.c_double))
+    iong.m = m.ctypes.data_as(C.POINTER(C.c_double))
+    ctlg.kinds_of_ions = 2
+    hmg.b, hmg.rho = 3.38 / 100.0, 0.317
+    hmg.c_na_na, hmg.c_na_cl, hmg.c_cl_cl = 1.25, 1.0, 0.75
+    hmg.sigma_na, hmg.sigma_cl = 1.170, 1.585
+    hmg.C_na_na, hmg.C_na_cl, hmg.C_cl_cl = 1.68 / 10.0, 11.2 / 10.0, 116.0 / 10.0
+    hmg.D_na_na, hmg.D_na_cl, hmg.D_cl_cl = 0.8 / 10.0, 13.9 / 10.0, 233.0 / 10.0
+    mdgpu.dropin_call("mk_table")
+    assert np.array_equal(pe, o.pe) and np.array_equal(fe, o.fe)
+    assert list(lookup) == [0, 1, 1, 2]
+    sysg.pe1r1 = sysg.fe1r1 = None
+    sysg.lookup = None
+    iong.Z = iong.m = None
+
+
+def test_ewald_params_rule():
+    a, hm = mdgpu.ewald_params(10.0, 45.12, 1e-6)
+    s = np.sqrt(np.log(1e6))
+    assert a == pytest.approx(s / 10.0)
+    assert hm == int(np.floor((s * a * 45.12 / np.pi) ** 2))
+    assert 380 < hm < 400   # cfg2: alpha ~ 0.372, hm ~ 393
+
+
+def test_no_device_fails_loudly():
+    if mdgpu.device_count() > 0:
+        pytest.skip("a device is present")
+    with pytest.raises(mdgpu.MdgError) as e:
+        mdgpu.Batch([mdgpu.box_params(2, 2, 2, 5.64, 0.5, 27, 5.0)])
+    assert e.value.code == mdgpu.E_NODEVICE

@@ -1,0 +1,122 @@
+"""Mixed precision against FP64, batching invariance and graph replay."""
+import numpy as np
+import pytest
+
+import gpuhelp as H
+import mdgpu
+import mdref
+
+pytestmark = pytest.mark.gpu
+
+# north-star bound for the mixed path: relative force error <= 1e-5 vs FP64
+MIXED_TOL = 1e-5
+
+
+def _params(n, a, eps=1e-6, rc_max=10.0, salt="NaCl", real_mode=mdgpu.ANALYTIC,
+            cells=mdgpu.CELLS_AUTO):
+    nx, ny, nz = n if isinstance(n, tuple) else (n, n, n)
+    L = np.array([nx, ny, nz]) * a
+    rc = min(rc_max, 0.5 * L.min())
+    alpha, hm = mdgpu.ewald_params(rc, L.max(), eps)
+    return mdgpu.box_params(nx, ny, nz, a, alpha, hm, rc, 2.0, real_mode=real_mode,
+                            cells=cells, salt=salt)
+
+
+def _state(p, seed, temp=1500.0, amp=0.3):
+    r, sp = mdgpu.rocksalt(p.nx, p.ny, p.nz, (p.ax, p.ay, p.az))
+    L = np.array([p.nx * p.ax, p.ny * p.ay, p.nz * p.az])
+    r = mdref.jitter_positions(r, L, amp, seed)
+    v = mdgpu.maxwell(seed, temp, sp, p.pot)
+    return r, v, sp
+
+
+@pytest.mark.parametrize("n,salt,a", [(3, "NaCl", H.A_NACL), (8, "NaCl", H.A_NACL),
+                                      (6, "KCl", H.A_KCL)])
+def test_mixed_forces_within_1e5_of_fp64(gpu, n, salt, a):
+    p = _params(n, a, salt=salt)
+    r, v, sp = _state(p, 5)
+    res = {}
+    for prec in (mdgpu.FP64, mdgpu.MIXED):
+        b = mdgpu.Batch([p], prec)
+        b.set_state(0, r=r, v=v, species=sp)
+        b.forces()
+        res[prec] = (b.state(0)["f"], b.scalars(0))
+        b.close()
+    f64, s64 = res[mdgpu.FP64]
+    f32, s32 = res[mdgpu.MIXED]
+    assert H.rel_err(f32, f64) <= MIXED_TOL
+    assert H.rms_rel_err(f32, f64) <= MIXED_TOL
+    assert s32["etot"] == pytest.approx(s64["etot"], rel=1e-6)
+    for i in range(3):
+        assert s32["vir"][i] == pytest.approx(s64["vir"][i], rel=1e-5)
+
+
+def test_batch_members_match_lone_boxes(gpu):
+    """Each box steps to identical bits alone and inside a mixed batch."""
+    specs = [(2, "NaCl", H.A_NACL), (3, "KCl", H.A_KCL), (5, "NaCl", H.A_NACL),
+             ((2, 3, 4), "NaCl", H.A_NACL)]
+    for prec in (mdgpu.MIXED, mdgpu.FP64):
+        ps = [_params(n, a, salt=s) for n, s, a in specs]
+        sts = [_state(p, 100 + k) for k, p in enumerate(ps)]
+        mix = mdgpu.Batch(ps, prec)
+        for k, (r, v, sp) in enumerate(sts):
+            mix.set_state(k, r=r, v=v, species=sp)
+        mix.step(20)
+        for k, p in enumerate(ps):
+            lone = mdgpu.Batch([p], prec)
+            r, v, sp = sts[k]
+            lone.set_state(0, r=r, v=v, species=sp)
+            lone.step(20)
+            a_, b_ = lone.state(0), mix.state(k)
+            for key in ("r", "v", "f"):
+                assert np.array_equal(a_[key], b_[key]), (prec, k, key)
+            assert lone.scalars(0)["etot"] == mix.scalars(k)["etot"]
+            lone.close()
+        mix.close()
+
+
+def test_graph_replay_equals_eager_steps(gpu):
+    p = _params(4, H.A_NACL)
+    r, v, sp = _state(p, 7)
+    out = []
+    for chunks in ([8, 8], [1] * 16):   # 8-step graphs vs single eager steps
+        b = mdgpu.Batch([p], mdgpu.MIXED)
+        b.set_state(0, r=r, v=v, species=sp)
+        for c in chunks:
+            b.step(c)
+        out.append(b.state(0))
+        assert b.scalars(0)["step"] == 16
+        b.close()
+    for key in ("r", "v", "f", "f0"):
+        assert np.array_equal(out[0][key], out[1][key]), key
+
+
+def nve_drift(p, prec, r, v, sp, steps=1000, every=10):
+    """Least-squares slope of the consistent total energy per ion
+    [1e-18 J], scaled to 1e4 steps (the first sample, which still carries
+    mdc's f0 = 0 start, is left out), and the peak-to-peak fluctuation."""
+    b = mdgpu.Batch([p], prec)
+    b.set_state(0, r=r, v=v, species=sp)
+    es = []
+    for _ in range(steps // every):
+        b.step(every)
+        es.append(b.scalars(0)["etot"])
+    n = b.n_ions(0)
+    b.close()
+    es = np.array(es[1:]) / n
+    t = np.arange(es.size) * every
+    return np.polyfit(t, es, 1)[0] * 1e4, np.ptp(es)
+
+
+def test_mixed_nve_energy_drift(gpu):
+    """NVE (analytic real space, dt = 1 fs, 1000 K): the mixed path's energy
+    drift is <= 1e-5 per ion per 1e4 steps and equals the FP64 path's to
+    1e-6 (the drift comes from the unshifted cutoff, not from FP32)."""
+    p = _params(4, H.A_NACL)
+    p.dt_fs = 1.0
+    r, v, sp = _state(p, 9, temp=1000.0, amp=0.05)
+    d32, fl32 = nve_drift(p, mdgpu.MIXED, r, v, sp)
+    d64, fl64 = nve_drift(p, mdgpu.FP64, r, v, sp)
+    assert abs(d32) <= 1e-5
+    assert abs(d32 - d64) <= 1e-6
+    assert fl32 == pytest.approx(fl64, rel=0.1)
