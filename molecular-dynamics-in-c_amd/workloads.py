@@ -1,0 +1,105 @@
+"""Synthetic, seeded inputs for the five workloads of mdc's force path.
+
+  nacl216      NaCl, 216 ions (3x3x3 cells, a = 5.64 A), alpha = 5.6/L,
+               cutoff L/2, hm = 27, 1000 K, dt = 2 fs, FP64
+  nacl4096     NaCl, 4096 ions (8x8x8), melted at 1500 K, cutoff 10 A,
+               alpha/hm for a 1e-6 Ewald error, FP64, cell lists
+  kcl32768     KCl, 32768 ions (16x16x16, a = 6.29 A), 1300 K, mixed
+  batch256     256 alkali-halide boxes (NaCl or KCl by seed), 64..4096 ions
+               log-uniform in whole conventional cells, 300..2000 K, mixed
+  stream       Poisson stream (mean 50 jobs/s) of NaCl boxes, 216..32768
+               ions log-uniform, 100..5000 steps, 300..2000 K
+
+Ewald split for the accuracy-targeted boxes (everything but nacl216): real
+cutoff rc = min(rc_max, L_min/2), alpha = s/rc, hm = floor((s alpha L_max/pi)^2),
+s = sqrt(-ln eps).  Velocities are Maxwell-Boltzmann from the library's
+seeded generator with mdc's centre-of-mass correction.
+"""
+import math
+
+import numpy as np
+
+import mdgpu
+
+A_NACL = 5.64
+A_KCL = 6.29
+
+
+def ewald_box(nx, ny, nz, a, salt, eps=1e-6, rc_max=10.0, dt_fs=2.0, real_mode=None,
+              cells=mdgpu.CELLS_AUTO):
+    L = np.array([nx, ny, nz], dtype=float) * a
+    rc = min(rc_max, 0.5 * L.min())
+    alpha, hm = mdgpu.ewald_params(rc, L.max(), eps)
+    if real_mode is None:
+        real_mode = mdgpu.ANALYTIC
+    return mdgpu.box_params(nx, ny, nz, a, alpha, hm, rc, dt_fs, real_mode=real_mode,
+                            cells=cells, salt=salt)
+
+
+def nacl216(real_mode=mdgpu.TABLE):
+    n, a = 3, A_NACL
+    L = n * a
+    return mdgpu.box_params(n, n, n, a, 5.6 / L, 27, L / 2, 2.0, real_mode=real_mode)
+
+
+def nacl4096(real_mode=mdgpu.TABLE):
+    return ewald_box(8, 8, 8, A_NACL, "NaCl", real_mode=real_mode, cells=mdgpu.CELL_LIST)
+
+
+def kcl32768(rc_max=10.0):
+    return ewald_box(16, 16, 16, A_KCL, "KCl", rc_max=rc_max, cells=mdgpu.CELL_LIST)
+
+
+def cells_for(n_ions):
+    """Near-cubic nx x ny x nz conventional cells holding about n_ions."""
+    target = max(1.0, n_ions / 8.0)
+    c = max(1, int(round(target ** (1.0 / 3.0))))
+    nz = max(1, int(round(target / (c * c))))
+    return c, c, nz
+
+
+def batch256(seed=2024, nbox=256, eps=1e-6, rc_max=10.0):
+    """Box parameters and temperatures of the batched workload."""
+    rng = np.random.default_rng(seed)
+    boxes, temps, salts = [], [], []
+    for _ in range(nbox):
+        n = math.exp(rng.uniform(math.log(64), math.log(4096)))
+        salt = "NaCl" if rng.random() < 0.5 else "KCl"
+        a = A_NACL if salt == "NaCl" else A_KCL
+        nx, ny, nz = cells_for(n)
+        boxes.append(ewald_box(nx, ny, nz, a, salt, eps=eps, rc_max=rc_max))
+        temps.append(float(rng.uniform(300.0, 2000.0)))
+        salts.append(salt)
+    return boxes, temps, salts
+
+
+def initial_state(p, temp_k, seed, jitter=0.0):
+    """mdc's rock-salt lattice (optionally jittered) and seeded velocities."""
+    r, sp = mdgpu.rocksalt(p.nx, p.ny, p.nz, (p.ax, p.ay, p.az))
+    if jitter > 0:
+        L = np.array([p.nx * p.ax, p.ny * p.ay, p.nz * p.az])[:, None]
+        rng = np.random.default_rng(seed + 7919)
+        r = np.mod(r + rng.uniform(-jitter, jitter, r.shape), L)
+    v = mdgpu.maxwell(seed, temp_k, sp, p.pot)
+    return r, v, sp
+
+
+def stream(seed=7, njobs=64, rate_hz=50.0, n_min=216, n_max=32768, steps_min=100,
+           steps_max=5000, steps_scale=1.0):
+    """Poisson job stream of NaCl boxes.  steps_scale shortens every job by
+    the same factor (for bounded benchmark runs)."""
+    rng = np.random.default_rng(seed)
+    t = 0.0
+    jobs = []
+    for k in range(njobs):
+        t += rng.exponential(1.0 / rate_hz)
+        n = math.exp(rng.uniform(math.log(n_min), math.log(n_max)))
+        nx, ny, nz = cells_for(n)
+        j = mdgpu.Job()
+        j.arrival_s = t
+        j.nx, j.ny, j.nz = nx, ny, nz
+        j.temp_k = float(rng.uniform(300.0, 2000.0))
+        j.steps = max(1, int(round(rng.integers(steps_min, steps_max + 1) * steps_scale)))
+        j.seed = seed * 100003 + k
+        jobs.append(j)
+    return jobs
