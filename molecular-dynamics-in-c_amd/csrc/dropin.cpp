@@ -125,6 +125,7 @@ bool ensure(const char* who) {
     s.dt = sys.dt; s.dt2 = sys.dt2;
     s.real_mode = MDG_TABLE;
     s.cells = sys.N <= 8192 ? MDG_ALL_PAIRS : MDG_CELLS_AUTO;
+    s.ordered = true;
     s.kinds = kinds;
     for (int k = 0; k < kinds; ++k) { s.Z[k] = ion.Z[k]; s.m[k] = ion.m[k]; }
     for (int k = 0; k < 16; ++k) s.lookup[k] = k < kinds * kinds ? sys.lookup[k] : 0;

@@ -22,36 +22,69 @@ void set_last_hip_error(hipError_t e, const char* what) {
   fprintf(stderr, "libmdgpu: %s\n", g_last_hip);
 }
 
-// mdc's constants (mdc/program/md.h:8-16), as spelled there.
+// mdc's constant (mdc/program/md.h:8), as spelled there.
 static const double kPi = 3.1415926535897932385;
 
+static int ceil_div(long a, long b) { return (int)((a + b - 1) / b); }
+
 // ------------------------------------------------------------------ tiling
-Tiling tiling_for(int N, int nk) {
+// Work granularity per box, from the box's own size only.  A lone box
+// should still spread over the chip (a few thousand waves for the large
+// ones); partial-sum buffers stay bounded.
+static Tiling choose_tiling(const BoxPlan& P, bool can_cells) {
+  const BoxSpec& sp = P.spec;
   Tiling t;
-  (void)nk;
-  // a lone small box cannot fill the GPU; keep its items from becoming tiny
-  if (N < 1024) {
-    t.sk_items_target = 256;
-    t.kf_items_target = 512;
+  const int N = sp.N;
+  if (sp.cells == MDG_CELL_LIST) t.cells = true;
+  else if (sp.cells == MDG_CELLS_AUTO) t.cells = can_cells && N >= 1500;
+  // real space
+  if (!sp.ordered) {
+    if (t.cells) {
+      const int ncell = P.nc[0] * P.nc[1] * P.nc[2];
+      const int opts[4] = {1, 3, 9, 27};
+      t.rsplit = 27;
+      for (int s : opts)
+        if ((long)ncell * s >= 512) { t.rsplit = s; break; }
+    } else {
+      const int nhome = ceil_div(N, kRealThreads);
+      t.rsplit = std::max(1, std::min(ceil_div(64, nhome), N / 128));
+    }
+  }
+  if (P.ewald) {
+    // structure factor: ion chunks of >= 128 ions, at most 64 chunks and
+    // 64 MB of partial sums
+    int nch = std::max(1, std::min(ceil_div(N, 128), 64));
+    nch = std::max(1, std::min(nch, (int)((64l << 20) / (16l * std::max(1, P.nk)))));
+    t.chunk = ceil_div(ceil_div(N, nch), kSkTile) * kSkTile;
+    t.nchunk = ceil_div(N, t.chunk);
+    // back-projection: ~2048 waves for a lone box, >= 64 k per group
+    const int nib = ceil_div(N, kKfThreads);
+    int g = ceil_div(2048, nib);
+    g = std::min(g, std::max(1, P.nk / 64));
+    g = std::min(g, 64);
+    t.ngroup = std::max(1, std::min(g, P.ncol));
   }
   return t;
 }
 
-static std::vector<int> partition_columns(const std::vector<KCol>& cols, int nk, int groups) {
+// Contiguous runs of quads with about equal numbers of weight rows.
+static std::vector<int> partition_quads(const std::vector<KQuad>& quads, int groups) {
   std::vector<int> start;
-  const int ncol = (int)cols.size();
-  groups = std::max(1, std::min(groups, ncol));
+  const int nq = (int)quads.size();
+  long rows = 0;
+  for (const KQuad& q : quads) rows += q.nrow;
+  groups = std::max(1, std::min(groups, nq));
   start.push_back(0);
   long acc = 0;
   int g = 1;
-  for (int c = 0; c < ncol && g < groups; ++c) {
-    acc += cols[c].len;
-    if (acc * groups >= (long)g * nk) {
-      if (c + 1 < ncol) start.push_back(c + 1);
+  for (int c = 0; c < nq && g < groups; ++c) {
+    acc += quads[c].nrow;
+    if (acc * groups >= (long)g * rows) {
+      if (c + 1 < nq) start.push_back(c + 1);
       ++g;
     }
   }
-  start.push_back(ncol);
+  start.push_back(nq);
   return start;
 }
 
@@ -78,6 +111,10 @@ int make_plan(const BoxSpec& sp, int precision, BoxPlan* P, std::string* why) {
   P->ewald = !(sp.alpha == 0.0 || sp.alpha > 10.0);  // rcprcl.c:23
   if (P->ewald) {
     const int s = (int)sqrtf((float)sp.hm);           // init.c:44
+    if (kforce_hz_for(s) < 0) {
+      if (why) *why = "reciprocal cutoff too large (hm_sqrt > 63)";
+      return MDG_E_ARG;
+    }
     P->hm_sqrt = s;
     P->hmax = s;
     const double vv = sp.L[0] * sp.L[1] * sp.L[2];
@@ -100,11 +137,47 @@ int make_plan(const BoxSpec& sp, int precision, BoxPlan* P, std::string* why) {
           const double ex = exp(-m2 * sp.p2a2);
           P->kce.push_back(vp * (ex / m2));
           P->kcf.push_back(v2 * (ex / m2));
+          P->khz.push_back(hz);
+        }
+        for (int q = 0; q < c.len; q += kSegLen) {
+          KSeg g{};
+          g.hx = hx; g.hy = hy; g.hz0 = lo + q; g.n = std::min(kSegLen, c.len - q);
+          g.k0 = c.k0 + q;
+          P->segs.push_back(g);
         }
         P->nk += c.len;
       }
     P->ncol = (int)P->cols.size();
+    P->nseg = (int)P->segs.size();
     if (P->nk == 0) P->ewald = false;
+    // quads of four consecutive columns over zero-padded weight rows
+    P->kslot.assign(P->nk, 0);
+    int row = 0;
+    for (int c0 = 0; c0 < P->ncol; c0 += 4) {
+      KQuad qd{};
+      const int nc = std::min(4, P->ncol - c0);
+      int lo = 1 << 30, hi = -1;
+      for (int c = 0; c < nc; ++c) {
+        const KCol& col = P->cols[c0 + c];
+        lo = std::min(lo, col.hz0);
+        hi = std::max(hi, col.hz0 + col.len);
+        qd.hx[c] = col.hx;
+        qd.hy[c] = col.hy;
+      }
+      for (int c = 0; c < nc; ++c) {
+        const KCol& col = P->cols[c0 + c];
+        for (int j = 0; j < col.len; ++j)
+          P->kslot[col.k0 + j] = (row + col.hz0 + j - lo) * 4 + c;
+      }
+      qd.lo = lo;
+      qd.nrow = hi - lo;
+      qd.row0 = row;
+      qd.ncol = nc;
+      row += qd.nrow;
+      P->quads.push_back(qd);
+    }
+    P->nquad = (int)P->quads.size();
+    P->nslot = 4 * row;
   }
   // cell grid: cells wider than rc (with a relative margin, so that two ions
   // within rc never bin two cells apart after rounding)
@@ -113,41 +186,91 @@ int make_plan(const BoxSpec& sp, int precision, BoxPlan* P, std::string* why) {
     P->nc[d] = (int)floor(sp.L[d] / (sp.rc * (1.0 + 1e-9)));
     if (P->nc[d] < 3) can_cells = false;
   }
-  const Tiling T = tiling_for(sp.N, P->nk);
-  bool use_cells = false;
-  if (sp.cells == MDG_CELL_LIST) {
-    if (!can_cells) {
-      if (why) *why = "cell list needs at least 3 cells of width >= rc per axis";
-      return MDG_E_RANGE;
-    }
-    use_cells = true;
-  } else if (sp.cells == MDG_CELLS_AUTO) {
-    use_cells = can_cells && sp.N >= T.cells_min_n;
+  if (sp.cells == MDG_CELL_LIST && !can_cells) {
+    if (why) *why = "cell list needs at least 3 cells of width >= rc per axis";
+    return MDG_E_RANGE;
   }
-  P->all_pairs = !use_cells;
-  if (use_cells) P->ncell = P->nc[0] * P->nc[1] * P->nc[2];
+  P->tile = choose_tiling(*P, can_cells);
+  P->all_pairs = !P->tile.cells;
+  if (!P->all_pairs) P->ncell = P->nc[0] * P->nc[1] * P->nc[2];
   else P->nc[0] = P->nc[1] = P->nc[2] = 0;
-  P->nreal = P->all_pairs ? (sp.N + kRealThreads - 1) / kRealThreads : P->ncell;
-  P->nchunk = (sp.N + kSkChunk - 1) / kSkChunk;
-  if (P->ewald) {
-    const int ion_blocks = (sp.N + kKfThreads - 1) / kKfThreads;
-    int gsk = (T.sk_items_target + P->nchunk - 1) / P->nchunk;
-    gsk = std::max(1, std::min(gsk, std::max(1, P->nk / T.sk_min_k)));
-    P->sk_group_start = partition_columns(P->cols, P->nk, gsk);
-    int gkf = (T.kf_items_target + ion_blocks - 1) / ion_blocks;
-    gkf = std::max(1, std::min(gkf, std::max(1, P->nk / T.kf_min_k)));
-    P->kf_group_start = partition_columns(P->cols, P->nk, gkf);
-  }
+  const int homes = P->all_pairs ? ceil_div(sp.N, kRealThreads) : P->ncell;
+  P->nreal = homes * P->tile.rsplit;
+  if (P->ewald) P->kf_group_start = partition_quads(P->quads, P->tile.ngroup);
   return MDG_OK;
 }
 
-// ------------------------------------------------------------------ boxes
+// ------------------------------------------------------------------ layout
+struct Layout {
+  size_t ion[12], typ, rpf, cell_of, cnt, start, sorted, s[3], styp;
+  size_t cols, segs, quads, khz, kslot, kce, kcf, kab, kab32, kpot, tx, ty, tw, skp, fp;
+  size_t tab, tab32, rpart, rpairs, scal, err, total;
+};
+
 static size_t take(size_t& off, size_t bytes) {
   const size_t o = (off + 255) & ~(size_t)255;
   off = o + bytes;
   return o;
 }
 
+static Layout layout(const BoxPlan& P, int precision) {
+  Layout L;
+  memset(&L, 0, sizeof(L));
+  const BoxSpec& sp = P.spec;
+  const long N = sp.N;
+  const bool mixed = precision == MDG_MIXED;
+  const size_t cplx = mixed ? sizeof(float2) : sizeof(double2);
+  const int ngroup = P.ewald ? (int)P.kf_group_start.size() - 1 : 0;
+  size_t off = 0;
+  for (int k = 0; k < 12; ++k) L.ion[k] = take(off, sizeof(double) * N);
+  L.typ = take(off, sizeof(int) * N);
+  if (P.tile.rsplit > 1) L.rpf = take(off, sizeof(double) * 3 * (size_t)P.tile.rsplit * N);
+  if (!P.all_pairs) {
+    L.cell_of = take(off, sizeof(int) * N);
+    L.cnt = take(off, sizeof(int) * P.ncell);
+    L.start = take(off, sizeof(int) * (P.ncell + 1));
+    L.sorted = take(off, sizeof(int) * N);
+    for (int d = 0; d < 3; ++d) L.s[d] = take(off, sizeof(double) * N);
+    L.styp = take(off, sizeof(int) * N);
+  }
+  if (P.ewald) {
+    const size_t H = P.hmax + 1;
+    L.cols = take(off, sizeof(KCol) * P.ncol);
+    L.segs = take(off, sizeof(KSeg) * P.nseg);
+    L.quads = take(off, sizeof(KQuad) * P.nquad);
+    L.khz = take(off, sizeof(int) * P.nk);
+    L.kslot = take(off, sizeof(int) * P.nk);
+    L.kce = take(off, sizeof(double) * P.nk);
+    L.kcf = take(off, sizeof(double) * P.nk);
+    L.kab = take(off, sizeof(double) * 4 * (size_t)P.nslot);
+    if (mixed) L.kab32 = take(off, sizeof(float) * 4 * (size_t)P.nslot);
+    L.kpot = take(off, sizeof(double) * P.nk);
+    L.tx = take(off, cplx * H * N);
+    L.ty = take(off, cplx * H * N);
+    L.tw = take(off, cplx * H * N);
+    L.skp = take(off, sizeof(double2) * (size_t)P.tile.nchunk * P.nk);
+    L.fp = take(off, sizeof(double) * 3 * (size_t)ngroup * N);
+  }
+  const size_t ntab = (size_t)sp.table_ntypes * sp.table_nb;
+  if (sp.real_mode == MDG_TABLE) {
+    if (mixed) L.tab32 = take(off, sizeof(float2) * ntab);
+    else L.tab = take(off, sizeof(double2) * ntab);
+  }
+  L.rpart = take(off, sizeof(double) * 4 * P.nreal);
+  L.rpairs = take(off, sizeof(unsigned long long) * P.nreal);
+  L.scal = take(off, sizeof(double) * kScalCount);
+  L.err = take(off, sizeof(int));
+  L.total = (off + 255) & ~(size_t)255;
+  return L;
+}
+
+size_t box_bytes_for(const BoxSpec& spec, int precision) {
+  BoxPlan P;
+  if (make_plan(spec, precision, &P, nullptr) != MDG_OK) return 0;
+  return layout(P, precision).total;
+}
+
+// ------------------------------------------------------------------ boxes
 int box_create(const BoxSpec& spec, int precision, Box** out, std::string* why) {
   Box* b = new Box();
   b->precision = precision;
@@ -159,47 +282,8 @@ int box_create(const BoxSpec& spec, int precision, Box** out, std::string* why) 
   const BoxPlan& P = b->plan;
   const int N = spec.N;
   const bool mixed = precision == MDG_MIXED;
-  const size_t cplx = mixed ? sizeof(float2) : sizeof(double2);
-  const int ngroup = P.ewald ? (int)P.kf_group_start.size() - 1 : 0;
-  size_t off = 0;
-  size_t o_ion[12];
-  for (int k = 0; k < 12; ++k) o_ion[k] = take(off, sizeof(double) * N);
-  const size_t o_typ = take(off, sizeof(int) * N);
-  size_t o_cell_of = 0, o_cnt = 0, o_start = 0, o_sorted = 0, o_s[3] = {0, 0, 0}, o_styp = 0;
-  if (!P.all_pairs) {
-    o_cell_of = take(off, sizeof(int) * N);
-    o_cnt = take(off, sizeof(int) * P.ncell);
-    o_start = take(off, sizeof(int) * (P.ncell + 1));
-    o_sorted = take(off, sizeof(int) * N);
-    for (int d = 0; d < 3; ++d) o_s[d] = take(off, sizeof(double) * N);
-    o_styp = take(off, sizeof(int) * N);
-  }
-  size_t o_cols = 0, o_kce = 0, o_kcf = 0, o_kab = 0, o_kab32 = 0, o_kpot = 0, o_tx = 0,
-         o_ty = 0, o_tw = 0, o_skp = 0, o_fp = 0;
-  if (P.ewald) {
-    o_cols = take(off, sizeof(KCol) * P.ncol);
-    o_kce = take(off, sizeof(double) * P.nk);
-    o_kcf = take(off, sizeof(double) * P.nk);
-    o_kab = take(off, sizeof(double) * 2 * P.nk);
-    if (mixed) o_kab32 = take(off, sizeof(float) * 2 * P.nk);
-    o_kpot = take(off, sizeof(double) * P.nk);
-    o_tx = take(off, cplx * (size_t)(P.hmax + 1) * N);
-    o_ty = take(off, cplx * (size_t)(P.hmax + 1) * N);
-    o_tw = take(off, cplx * (size_t)N);
-    o_skp = take(off, sizeof(double2) * (size_t)P.nchunk * P.nk);
-    o_fp = take(off, sizeof(double) * 3 * (size_t)ngroup * N);
-  }
-  size_t o_tab = 0, o_tab32 = 0;
-  const int ntab = spec.table_ntypes * spec.table_nb;
-  if (spec.real_mode == MDG_TABLE) {
-    if (mixed) o_tab32 = take(off, sizeof(float2) * ntab);
-    else o_tab = take(off, sizeof(double2) * ntab);
-  }
-  const size_t o_rpart = take(off, sizeof(double) * 4 * P.nreal);
-  const size_t o_rpairs = take(off, sizeof(unsigned long long) * P.nreal);
-  const size_t o_scal = take(off, sizeof(double) * kScalCount);
-  const size_t o_err = take(off, sizeof(int));
-  b->bytes = (off + 255) & ~(size_t)255;
+  const Layout L = layout(P, precision);
+  b->bytes = L.total;
   if (hipMalloc(&b->dmem, b->bytes) != hipSuccess) {
     (void)hipGetLastError();
     if (why) *why = "device allocation failed";
@@ -216,8 +300,8 @@ int box_create(const BoxSpec& spec, int precision, Box** out, std::string* why) 
   memset(&D, 0, sizeof(D));
   double** ion_ptrs[12] = {&D.rx, &D.ry, &D.rz, &D.vx, &D.vy, &D.vz,
                            &D.fx, &D.fy, &D.fz, &D.fx0, &D.fy0, &D.fz0};
-  for (int k = 0; k < 12; ++k) *ion_ptrs[k] = (double*)(base + o_ion[k]);
-  D.typ = (int*)(base + o_typ);
+  for (int k = 0; k < 12; ++k) *ion_ptrs[k] = (double*)(base + L.ion[k]);
+  D.typ = (int*)(base + L.typ);
   D.N = N;
   D.ewald = P.ewald ? 1 : 0;
   D.real_mode = spec.real_mode;
@@ -247,69 +331,80 @@ int box_create(const BoxSpec& spec, int precision, Box** out, std::string* why) 
   }
   D.rho = spec.pot.rho;
   D.table_nb = spec.table_nb;
+  D.rsplit = P.tile.rsplit;
+  D.rpf = P.tile.rsplit > 1 ? (double*)(base + L.rpf) : nullptr;
   if (!P.all_pairs) {
     for (int d = 0; d < 3; ++d) {
       D.nc[d] = P.nc[d];
       D.cellscale[d] = (double)P.nc[d] / spec.L[d];
     }
     D.ncell = P.ncell;
-    D.cell_of = (int*)(base + o_cell_of);
-    D.cell_count = (int*)(base + o_cnt);
-    D.cell_start = (int*)(base + o_start);
-    D.sorted = (int*)(base + o_sorted);
-    D.sx = (double*)(base + o_s[0]);
-    D.sy = (double*)(base + o_s[1]);
-    D.sz = (double*)(base + o_s[2]);
-    D.styp = (int*)(base + o_styp);
+    D.cell_of = (int*)(base + L.cell_of);
+    D.cell_count = (int*)(base + L.cnt);
+    D.cell_start = (int*)(base + L.start);
+    D.sorted = (int*)(base + L.sorted);
+    D.sx = (double*)(base + L.s[0]);
+    D.sy = (double*)(base + L.s[1]);
+    D.sz = (double*)(base + L.s[2]);
+    D.styp = (int*)(base + L.styp);
   }
   int err = MDG_OK;
+  auto up = [&](size_t o, const void* src, size_t n) {
+    if (n && err == MDG_OK && hipMemcpy(base + o, src, n, hipMemcpyHostToDevice) != hipSuccess)
+      err = MDG_E_HIP;
+  };
   if (P.ewald) {
     D.nk = P.nk;
     D.ncol = P.ncol;
+    D.nseg = P.nseg;
     D.hmax = P.hmax;
-    D.cols = (const KCol*)(base + o_cols);
-    D.kce = (const double*)(base + o_kce);
-    D.kcf = (const double*)(base + o_kcf);
-    D.kab = (double*)(base + o_kab);
-    D.kab32 = mixed ? (float*)(base + o_kab32) : nullptr;
-    D.kpot = (double*)(base + o_kpot);
-    D.tabx = base + o_tx;
-    D.taby = base + o_ty;
-    D.tabw = base + o_tw;
-    D.skpart = (double2*)(base + o_skp);
-    D.nchunk = P.nchunk;
-    D.fpart = (double*)(base + o_fp);
-    D.ngroup = ngroup;
-    if (hipMemcpy(base + o_cols, P.cols.data(), sizeof(KCol) * P.ncol, hipMemcpyHostToDevice) !=
-            hipSuccess ||
-        hipMemcpy(base + o_kce, P.kce.data(), sizeof(double) * P.nk, hipMemcpyHostToDevice) !=
-            hipSuccess ||
-        hipMemcpy(base + o_kcf, P.kcf.data(), sizeof(double) * P.nk, hipMemcpyHostToDevice) !=
-            hipSuccess)
-      err = MDG_E_HIP;
+    D.cols = (const KCol*)(base + L.cols);
+    D.segs = (const KSeg*)(base + L.segs);
+    D.nquad = P.nquad;
+    D.nslot = P.nslot;
+    D.quads = (const KQuad*)(base + L.quads);
+    D.kslot = (const int*)(base + L.kslot);
+    up(L.quads, P.quads.data(), sizeof(KQuad) * P.nquad);
+    up(L.kslot, P.kslot.data(), sizeof(int) * P.nk);
+    D.khz = (const int*)(base + L.khz);
+    D.kce = (const double*)(base + L.kce);
+    D.kcf = (const double*)(base + L.kcf);
+    D.kab = (double*)(base + L.kab);
+    D.kab32 = mixed ? (float*)(base + L.kab32) : nullptr;
+    D.kpot = (double*)(base + L.kpot);
+    D.tabx = base + L.tx;
+    D.taby = base + L.ty;
+    D.tabw = base + L.tw;
+    D.skpart = (double2*)(base + L.skp);
+    D.nchunk = P.tile.nchunk;
+    D.chunk = P.tile.chunk;
+    D.fpart = (double*)(base + L.fp);
+    D.ngroup = (int)P.kf_group_start.size() - 1;
+    up(L.cols, P.cols.data(), sizeof(KCol) * P.ncol);
+    up(L.segs, P.segs.data(), sizeof(KSeg) * P.nseg);
+    up(L.khz, P.khz.data(), sizeof(int) * P.nk);
+    up(L.kce, P.kce.data(), sizeof(double) * P.nk);
+    up(L.kcf, P.kcf.data(), sizeof(double) * P.nk);
   }
-  if (spec.real_mode == MDG_TABLE && err == MDG_OK) {
+  const int ntab = spec.table_ntypes * spec.table_nb;
+  if (spec.real_mode == MDG_TABLE) {
     if (mixed) {
       std::vector<float2> t(ntab);
       for (int k = 0; k < ntab; ++k) t[k] = make_float2((float)spec.pe[k], (float)spec.fe[k]);
-      if (hipMemcpy(base + o_tab32, t.data(), sizeof(float2) * ntab, hipMemcpyHostToDevice) !=
-          hipSuccess)
-        err = MDG_E_HIP;
-      D.table32 = (const float2*)(base + o_tab32);
+      up(L.tab32, t.data(), sizeof(float2) * ntab);
+      D.table32 = (const float2*)(base + L.tab32);
     } else {
       std::vector<double2> t(ntab);
       for (int k = 0; k < ntab; ++k) t[k] = make_double2(spec.pe[k], spec.fe[k]);
-      if (hipMemcpy(base + o_tab, t.data(), sizeof(double2) * ntab, hipMemcpyHostToDevice) !=
-          hipSuccess)
-        err = MDG_E_HIP;
-      D.table = (const double2*)(base + o_tab);
+      up(L.tab, t.data(), sizeof(double2) * ntab);
+      D.table = (const double2*)(base + L.tab);
     }
   }
-  D.rpart = (double*)(base + o_rpart);
-  D.rpairs = (unsigned long long*)(base + o_rpairs);
+  D.rpart = (double*)(base + L.rpart);
+  D.rpairs = (unsigned long long*)(base + L.rpairs);
   D.nreal = P.nreal;
-  D.scal = (double*)(base + o_scal);
-  D.err = (int*)(base + o_err);
+  D.scal = (double*)(base + L.scal);
+  D.err = (int*)(base + L.err);
   if (err != MDG_OK) {
     if (why) *why = "uploading box constants failed";
     box_destroy(b);
@@ -419,26 +514,30 @@ int BoxSet::set_boxes(const std::vector<Box*>& boxes) {
                    [&](int a, int b) { return boxes_[a]->desc.N > boxes_[b]->desc.N; });
   std::vector<Item> real, tab, sk, fin, kf, ion;
   std::vector<int> cellid, allid;
+  int hmax = 0;
   for (int bi : order) {
     const Box* B = boxes_[bi];
     const BoxPlan& P = B->plan;
     const int N = B->desc.N;
+    const int rs = P.tile.rsplit;
     allid.push_back(bi);
     if (P.all_pairs) {
       int slot = 0;
-      for (int i0 = 0; i0 < N; i0 += kRealThreads) real.push_back({bi, i0, slot++, 0, 0});
+      for (int i0 = 0; i0 < N; i0 += kRealThreads)
+        for (int s = 0; s < rs; ++s) real.push_back({bi, i0, slot++, s, 0});
     } else {
       cellid.push_back(bi);
-      for (int c = 0; c < P.ncell; ++c) real.push_back({bi, c, c, 0, 0});
+      int slot = 0;
+      for (int c = 0; c < P.ncell; ++c)
+        for (int s = 0; s < rs; ++s) real.push_back({bi, c, slot++, s, 0});
     }
     for (int i0 = 0; i0 < N; i0 += kIonThreads) ion.push_back({bi, i0, 0, 0, 0});
     if (P.ewald) {
+      hmax = std::max(hmax, P.hmax);
       for (int i0 = 0; i0 < N; i0 += kIonThreads) tab.push_back({bi, i0, 0, 0, 0});
-      const int gs = (int)P.sk_group_start.size() - 1;
-      for (int ch = 0; ch < P.nchunk; ++ch)
-        for (int g = 0; g < gs; ++g)
-          sk.push_back({bi, ch, P.sk_group_start[g],
-                        P.sk_group_start[g + 1] - P.sk_group_start[g], g});
+      for (int ch = 0; ch < P.tile.nchunk; ++ch)
+        for (int s0 = 0; s0 < P.nseg; s0 += kSkThreads)
+          sk.push_back({bi, ch, s0, std::min(kSkThreads, P.nseg - s0), 0});
       for (int k0 = 0; k0 < P.nk; k0 += kIonThreads) fin.push_back({bi, k0, 0, 0, 0});
       const int gk = (int)P.kf_group_start.size() - 1;
       for (int i0 = 0; i0 < N; i0 += kKfThreads)
@@ -447,6 +546,10 @@ int BoxSet::set_boxes(const std::vector<Box*>& boxes) {
                         P.kf_group_start[g + 1] - P.kf_group_start[g], g});
     }
   }
+  const int H = hmax + 1;
+  hs_ = H | 1;                      // odd strides spread lanes over LDS banks
+  hsw_ = (H + kSegLen - 1) | 1;
+  hz_ = kforce_hz_for(hmax);
   std::vector<Item> items;
   auto app = [&](std::vector<Item>& v, int& off, int& n) {
     off = (int)items.size();
@@ -514,35 +617,32 @@ void BoxSet::launch_real(hipStream_t s, int mode) {
     launch_real64(analytic, d_desc_, d_items_ + off_real_, n_real_, mode, s);
 }
 
-void BoxSet::enqueue_forces_parts(hipStream_t s, bool real, int real_write_mode, bool recip) {
+// phase tables, partial structure factors, finalisation, back-projection
+void BoxSet::launch_recip(hipStream_t s) {
   const bool mixed = precision_ == MDG_MIXED;
+  launch_phase_tables(mixed, d_desc_, d_items_ + off_tab_, n_tab_, s);
+  if (mixed) launch_sk32(d_desc_, d_items_ + off_sk_, n_sk_, hs_, hsw_, s);
+  else launch_sk64(d_desc_, d_items_ + off_sk_, n_sk_, hs_, hsw_, s);
+  launch_sk_finalize(d_desc_, d_items_ + off_fin_, n_fin_, s);
+  if (mixed) launch_kforce32(hz_, d_desc_, d_items_ + off_kf_, n_kf_, s);
+  else launch_kforce64(hz_, d_desc_, d_items_ + off_kf_, n_kf_, s);
+}
+
+void BoxSet::enqueue_forces_parts(hipStream_t s, bool real, int real_write_mode, bool recip) {
   if (real) {
     launch_cell_build(d_desc_, d_ids_ + off_cellid_, n_cellid_, s);
     launch_real(s, real_write_mode);
   }
   if (recip && n_sk_ > 0) {
-    launch_phase_tables(mixed, d_desc_, d_items_ + off_tab_, n_tab_, s);
-    if (mixed) launch_sk_partial32(d_desc_, d_items_ + off_sk_, n_sk_, s);
-    else launch_sk_partial64(d_desc_, d_items_ + off_sk_, n_sk_, s);
-    launch_sk_finalize(d_desc_, d_items_ + off_fin_, n_fin_, s);
-    if (mixed) launch_kforce32(d_desc_, d_items_ + off_kf_, n_kf_, s);
-    else launch_kforce64(d_desc_, d_items_ + off_kf_, n_kf_, s);
+    launch_recip(s);
     launch_force_vv(false, d_desc_, d_items_ + off_ion_, n_ion_, s);
   }
 }
 
 void BoxSet::enqueue_step(hipStream_t s, bool vv) {
-  const bool mixed = precision_ == MDG_MIXED;
   launch_cell_build(d_desc_, d_ids_ + off_cellid_, n_cellid_, s);
-  if (n_sk_ > 0) launch_phase_tables(mixed, d_desc_, d_items_ + off_tab_, n_tab_, s);
   launch_real(s, 0);
-  if (n_sk_ > 0) {
-    if (mixed) launch_sk_partial32(d_desc_, d_items_ + off_sk_, n_sk_, s);
-    else launch_sk_partial64(d_desc_, d_items_ + off_sk_, n_sk_, s);
-    launch_sk_finalize(d_desc_, d_items_ + off_fin_, n_fin_, s);
-    if (mixed) launch_kforce32(d_desc_, d_items_ + off_kf_, n_kf_, s);
-    else launch_kforce64(d_desc_, d_items_ + off_kf_, n_kf_, s);
-  }
+  if (n_sk_ > 0) launch_recip(s);
   launch_force_vv(vv, d_desc_, d_items_ + off_ion_, n_ion_, s);
   launch_box_scalars(d_desc_, d_ids_ + off_allid_, n_allid_, vv ? 1 : 0, s);
 }
@@ -591,34 +691,33 @@ int BoxSet::profile(int nsteps, hipStream_t s, std::vector<std::string>* names,
                        "sk_finalize", "kforce",      "force_vv",   "box_scalars"};
   std::vector<hipEvent_t> ev(2 * 8 * std::max(1, nsteps));
   for (auto& e : ev) MDG_HIP(hipEventCreate(&e));
-  std::vector<int> used(8, 0);
+  std::vector<int> used(8, 0), which;
   int e = 0;
   auto rec = [&](int k, auto&& fn) {
-    MDG_HIP(hipEventRecord(ev[e], s));
+    (void)hipEventRecord(ev[e], s);
     fn();
-    MDG_HIP(hipEventRecord(ev[e + 1], s));
+    (void)hipEventRecord(ev[e + 1], s);
     used[k] = 1;
+    which.push_back(k);
     e += 2;
-    return MDG_OK;
   };
-  std::vector<int> which;
   for (int st = 0; st < nsteps; ++st) {
-    if (n_cellid_) { rec(0, [&] { launch_cell_build(d_desc_, d_ids_ + off_cellid_, n_cellid_, s); }); which.push_back(0); }
-    if (n_sk_) { rec(1, [&] { launch_phase_tables(mixed, d_desc_, d_items_ + off_tab_, n_tab_, s); }); which.push_back(1); }
-    rec(2, [&] { launch_real(s, 0); }); which.push_back(2);
+    if (n_cellid_) rec(0, [&] { launch_cell_build(d_desc_, d_ids_ + off_cellid_, n_cellid_, s); });
+    rec(2, [&] { launch_real(s, 0); });
     if (n_sk_) {
+      rec(1, [&] { launch_phase_tables(mixed, d_desc_, d_items_ + off_tab_, n_tab_, s); });
       rec(3, [&] {
-        if (mixed) launch_sk_partial32(d_desc_, d_items_ + off_sk_, n_sk_, s);
-        else launch_sk_partial64(d_desc_, d_items_ + off_sk_, n_sk_, s);
-      }); which.push_back(3);
-      rec(4, [&] { launch_sk_finalize(d_desc_, d_items_ + off_fin_, n_fin_, s); }); which.push_back(4);
+        if (mixed) launch_sk32(d_desc_, d_items_ + off_sk_, n_sk_, hs_, hsw_, s);
+        else launch_sk64(d_desc_, d_items_ + off_sk_, n_sk_, hs_, hsw_, s);
+      });
+      rec(4, [&] { launch_sk_finalize(d_desc_, d_items_ + off_fin_, n_fin_, s); });
       rec(5, [&] {
-        if (mixed) launch_kforce32(d_desc_, d_items_ + off_kf_, n_kf_, s);
-        else launch_kforce64(d_desc_, d_items_ + off_kf_, n_kf_, s);
-      }); which.push_back(5);
+        if (mixed) launch_kforce32(hz_, d_desc_, d_items_ + off_kf_, n_kf_, s);
+        else launch_kforce64(hz_, d_desc_, d_items_ + off_kf_, n_kf_, s);
+      });
     }
-    rec(6, [&] { launch_force_vv(true, d_desc_, d_items_ + off_ion_, n_ion_, s); }); which.push_back(6);
-    rec(7, [&] { launch_box_scalars(d_desc_, d_ids_ + off_allid_, n_allid_, 1, s); }); which.push_back(7);
+    rec(6, [&] { launch_force_vv(true, d_desc_, d_items_ + off_ion_, n_ion_, s); });
+    rec(7, [&] { launch_box_scalars(d_desc_, d_ids_ + off_allid_, n_allid_, 1, s); });
   }
   MDG_HIP(hipStreamSynchronize(s));
   std::vector<double> tot(8, 0.0);

@@ -25,6 +25,7 @@ struct BoxSpec {
   double dt = 0, dt2 = 0; // mdc time units
   int real_mode = MDG_TABLE;
   int cells = MDG_CELLS_AUTO;
+  bool ordered = false;   // real space in mdc's partner order (no splitting)
   int kinds = 2;
   double Z[4] = {0, 0, 0, 0}, m[4] = {0, 0, 0, 0};
   int lookup[16] = {0, 1, 1, 2};
@@ -35,28 +36,31 @@ struct BoxSpec {
   int table_nb = 0, table_ntypes = 3;
 };
 
-// Tile sizes of one box class; chosen per box from its own size only.
+// Tile sizes of one box; a function of the box alone (and its precision),
+// so a box computes the same bits alone and in any batch.
 struct Tiling {
-  int sk_items_target = 1024;  // structure-factor items a lone box aims for
-  int kf_items_target = 2048;  // back-projection items a lone box aims for
-  int sk_min_k = 16;           // k vectors per structure-factor item, at least
-  int kf_min_k = 64;           // k vectors per back-projection item, at least
-  int cells_min_n = 1500;      // MDG_CELLS_AUTO uses cell lists from this N
+  int rsplit = 1;        // real-space partner splits per home unit
+  int nchunk = 1;        // structure-factor ion chunks
+  int chunk = 0;         // ions per chunk (multiple of kSkTile)
+  int ngroup = 1;        // back-projection column groups
+  bool cells = false;    // cell list (else all pairs)
 };
-
-Tiling tiling_for(int N, int nk);
 
 struct BoxPlan {
   BoxSpec spec;
   bool ewald = false;
-  int hm_sqrt = 0, hmax = 0, nk = 0, ncol = 0;
+  int hm_sqrt = 0, hmax = 0, nk = 0, ncol = 0, nseg = 0;
   std::vector<KCol> cols;
+  std::vector<KSeg> segs;
+  std::vector<KQuad> quads;
+  std::vector<int> khz, kslot;
+  int nquad = 0, nslot = 0;
   std::vector<double> kce, kcf;
   bool all_pairs = true;
   int nc[3] = {0, 0, 0}, ncell = 0;
-  int nreal = 0, nchunk = 0;
-  std::vector<int> sk_group_start;  // column index where each sk group starts (+ end)
-  std::vector<int> kf_group_start;
+  int nreal = 0;
+  Tiling tile;
+  std::vector<int> kf_group_start;  // first quad of each back-projection group (+ end)
 };
 
 int make_plan(const BoxSpec& spec, int precision, BoxPlan* out, std::string* why);
@@ -71,6 +75,8 @@ struct Box {
   bool desc_dirty = true;
 };
 
+// Device bytes a box of this spec would hold (for admission control).
+size_t box_bytes_for(const BoxSpec& spec, int precision);
 int box_create(const BoxSpec& spec, int precision, Box** out, std::string* why);
 void box_destroy(Box* b);
 int box_upload_state(Box* b, const double* r, const double* v, const int* species,
@@ -86,8 +92,8 @@ class BoxSet {
   ~BoxSet();
   int set_boxes(const std::vector<Box*>& boxes);  // rebuild work lists
   const std::vector<Box*>& boxes() const { return boxes_; }
-  // Enqueue mdc's calc_foc (real_mode_write 0: f0 <- f, f <- forces).
-  // vv: also next_v/next_r; count_step: bump the step counter.
+  // mdc's calc_foc on every box (f0 <- f, f <- forces); vv: also next_v and
+  // next_r and count the step.
   void enqueue_step(hipStream_t s, bool vv);
   void enqueue_forces_parts(hipStream_t s, bool real, int real_write_mode, bool recip);
   void enqueue_scalars(hipStream_t s, bool count_step);
@@ -112,10 +118,12 @@ class BoxSet {
   int off_real_ = 0, n_real_ = 0, off_tab_ = 0, n_tab_ = 0, off_sk_ = 0, n_sk_ = 0;
   int off_fin_ = 0, n_fin_ = 0, off_kf_ = 0, n_kf_ = 0, off_ion_ = 0, n_ion_ = 0;
   int off_cellid_ = 0, n_cellid_ = 0, off_allid_ = 0, n_allid_ = 0;
+  int hs_ = 1, hsw_ = 8, hz_ = 8;  // LDS strides / register row of the k-space kernels
   hipGraphExec_t graph_exec_ = nullptr;
   hipGraph_t graph_ = nullptr;
   int graph_steps_ = 0;
   void launch_real(hipStream_t s, int mode);
+  void launch_recip(hipStream_t s);
 };
 
 void set_last_hip_error(hipError_t e, const char* what);

@@ -1,13 +1,22 @@
 // k_fp64.hip -- FP64 kernels of the plain path and the FP64 kernels both
 // precision modes share.  Built with -ffp-contract=off: products and sums
 // written unfused round as mdc's SSE2 code does (explicit fma() calls in
-// the k-space recurrences stay fused).
+// the k-space products stay fused).
 #include "kernels_common.h"
 #include "launch.h"
 
 namespace mdg {
 
 #define MDG_GRID(n, t) dim3((unsigned)(n)), dim3((unsigned)(t)), 0, s
+#define MDG_GRID_LDS(n, t, b) dim3((unsigned)(n)), dim3((unsigned)(t)), (unsigned)(b), s
+
+int kforce_hz_for(int hmax) {
+  const int need = hmax + 1;
+  const int opts[6] = {8, 16, 24, 32, 48, 64};
+  for (int h : opts)
+    if (need <= h) return h;
+  return -1;
+}
 
 void launch_cell_build(const BoxDesc* boxes, const int* ids, int n, hipStream_t s) {
   if (n > 0) hipLaunchKernelGGL(k_cell_build, MDG_GRID(n, kCellThreads), boxes, ids);
@@ -29,12 +38,28 @@ void launch_real64(bool analytic, const BoxDesc* boxes, const Item* items, int n
     hipLaunchKernelGGL(k_real<kRealTable64>, MDG_GRID(n, kRealThreads), boxes, items, mode);
 }
 
-void launch_sk_partial64(const BoxDesc* boxes, const Item* items, int n, hipStream_t s) {
-  if (n > 0) hipLaunchKernelGGL(k_sk_partial<double>, MDG_GRID(n, kSkThreads), boxes, items);
+void launch_sk64(const BoxDesc* boxes, const Item* items, int n, int hs, int hsw, hipStream_t s) {
+  if (n <= 0) return;
+  const size_t lds = (size_t)kSkTile * (2 * hs + hsw) * sizeof(double2);
+  static bool raised = false;  // allow beyond 64 KB of dynamic LDS (gfx950 has 160 KB)
+  if (lds > 65536 && !raised) {
+    (void)hipFuncSetAttribute((const void*)k_sk_seg<double>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+    raised = true;
+  }
+  hipLaunchKernelGGL(k_sk_seg<double>, MDG_GRID_LDS(n, kSkThreads, lds), boxes, items, hs, hsw);
 }
 
-void launch_kforce64(const BoxDesc* boxes, const Item* items, int n, hipStream_t s) {
-  if (n > 0) hipLaunchKernelGGL(k_kforce<double>, MDG_GRID(n, kKfThreads), boxes, items);
+void launch_kforce64(int hz, const BoxDesc* boxes, const Item* items, int n, hipStream_t s) {
+  if (n <= 0) return;
+  switch (hz) {
+    case 8: hipLaunchKernelGGL((k_kforce<double, 8>), MDG_GRID(n, kKfThreads), boxes, items); break;
+    case 16: hipLaunchKernelGGL((k_kforce<double, 16>), MDG_GRID(n, kKfThreads), boxes, items); break;
+    case 24: hipLaunchKernelGGL((k_kforce<double, 24>), MDG_GRID(n, kKfThreads), boxes, items); break;
+    case 32: hipLaunchKernelGGL((k_kforce<double, 32>), MDG_GRID(n, kKfThreads), boxes, items); break;
+    case 48: hipLaunchKernelGGL((k_kforce<double, 48>), MDG_GRID(n, kKfThreads), boxes, items); break;
+    default: hipLaunchKernelGGL((k_kforce<double, 64>), MDG_GRID(n, kKfThreads), boxes, items); break;
+  }
 }
 
 void launch_sk_finalize(const BoxDesc* boxes, const Item* items, int n, hipStream_t s) {

@@ -8,6 +8,8 @@
 // identical bits alone or inside any batch.
 #pragma once
 
+#include <type_traits>
+
 #include "mdg_internal.h"
 
 namespace mdg {
@@ -21,6 +23,25 @@ __device__ __forceinline__ Cx<double> cmul(Cx<double> a, Cx<double> b) {
 __device__ __forceinline__ Cx<float> cmul(Cx<float> a, Cx<float> b) {
   return {fmaf(a.x, b.x, -a.y * b.y), fmaf(a.x, b.y, a.y * b.x)};
 }
+// acc += a * b (complex), four fused multiply-adds
+__device__ __forceinline__ void cmac(Cx<double>& acc, Cx<double> a, Cx<double> b) {
+  acc.x = fma(a.x, b.x, acc.x); acc.x = fma(-a.y, b.y, acc.x);
+  acc.y = fma(a.x, b.y, acc.y); acc.y = fma(a.y, b.x, acc.y);
+}
+__device__ __forceinline__ void cmac(Cx<float>& acc, Cx<float> a, Cx<float> b) {
+  acc.x = fmaf(a.x, b.x, acc.x); acc.x = fmaf(-a.y, b.y, acc.x);
+  acc.y = fmaf(a.x, b.y, acc.y); acc.y = fmaf(a.y, b.x, acc.y);
+}
+// Read-only per-box tables (k columns, segments, per-k weights) are never
+// written while a kernel that reads them runs; reading them through the
+// constant address space lets uniform indices become scalar-cache loads.
+template <typename T>
+using cptr = const __attribute__((address_space(4))) T*;
+template <typename T>
+__device__ __forceinline__ cptr<T> as_const(const T* p) {
+  return (cptr<T>)p;
+}
+
 template <typename R> __device__ __forceinline__ Cx<R> conj_if(Cx<R> a, bool c) {
   return {a.x, c ? -a.y : a.y};
 }
@@ -71,47 +92,6 @@ __device__ __forceinline__ float erfc_cheb(float z) {
   return t * __expf(fmaf(-z, z, p));
 }
 
-// ------------------------------------------------------------ pair terms
-// -U'(r)/r and U(r) of erfc-screened Coulomb + Born-Mayer-Huggins at exact r,
-// operation order of mdc's mk_table (mdc/program/control.c:219-240); used by
-// the FP64 analytic mode as pr-mdc's ewald1 evaluates erfc per pair
-// (pr-mdc/program/ewald.c:37-43).
-__device__ __forceinline__ void pair_analytic64(const BoxDesc& B, int t, double ZZ, double dr,
-                                                double& p, double& f) {
-  double dr6 = dr * dr * dr * dr * dr * dr;
-  double dr8 = dr6 * dr * dr;
-  double adr = B.alpha * dr;
-  double erfc_per_dr = mori_erfc(adr) / dr;
-  double ebmh = B.cb[t] * exp((B.sig[t] - dr) / B.rho);
-  p = ZZ * erfc_per_dr + (ebmh - B.Cd[t] / dr6 - B.Dd[t] / dr8);
-  double fe = ZZ * (2.0 * B.a3 * exp(-adr * adr) + erfc_per_dr) / (dr * dr);
-  double fr = (ebmh / B.rho - 6.0 * B.Cd[t] / (dr6 * dr) - 8.0 * B.Dd[t] / (dr8 * dr)) / dr;
-  f = fe + fr;
-}
-
-// FP32 constants of the pair function per type, staged once per work item.
-struct Pair32 {
-  float rc2, alpha, two_a3;
-  float zz[3], cb[3], sig[3], inv_rho, c6[3], d8[3];
-};
-
-__device__ __forceinline__ void pair_analytic32(const Pair32& P, int t, float r2, float& p,
-                                                float& f) {
-  const float kLog2e = 1.4426950408889634f;
-  float rinv = __frsqrt_rn(r2);
-  float r = r2 * rinv;
-  float rinv2 = rinv * rinv;
-  float z = P.alpha * r;
-  float erfc_r = erfc_cheb(z) * rinv;
-  float g = exp2f(-z * z * kLog2e);
-  float eb = P.cb[t] * exp2f((P.sig[t] - r) * P.inv_rho * kLog2e);
-  float r6 = rinv2 * rinv2 * rinv2;
-  float r8 = r6 * rinv2;
-  p = P.zz[t] * erfc_r + eb - P.c6[t] * r6 - P.d8[t] * r8;
-  f = P.zz[t] * (P.two_a3 * g + erfc_r) * rinv2 +
-      (eb * P.inv_rho * rinv - 6.0f * P.c6[t] * r8 - 8.0f * P.d8[t] * r8 * rinv2);
-}
-
 // ------------------------------------------------------------ reductions
 // Fixed-order sum over the 64 lanes of a wave.
 __device__ __forceinline__ double wave_sum(double v) {
@@ -130,105 +110,233 @@ __device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long v)
 //
 // One wave per item.  The force on an ion is gathered by its own lane
 // (no scatter, no atomics); partner ions are staged 64 at a time in LDS and
-// read by all lanes at once.  All-pairs items visit partners in ascending
-// index, which is the order in which mdc's i<j loop adds contributions to
-// any given ion, so table-mode forces are bit-identical to mdc's.  Energy
-// and virial count each unordered pair once (partner index > own index).
+// read by all lanes at once.  An item covers one home unit (64 consecutive
+// ions, or one cell) and one split of its partners (a contiguous index
+// range, or a contiguous run of the 27 neighbour cells).  With one split
+// and all pairs, partners are visited in ascending index -- the order in
+// which mdc's i<j loop adds contributions to any given ion -- so
+// table-mode forces are bit-identical to mdc's.  Energy and virial count
+// each unordered pair once (partner index > own index).
 //
-// MODE 0: f0 <- f, f <- real-space force (mdc's clear_foc then real_space).
-// MODE 1: f += real-space force (real_space() after a caller's clear_foc).
+// rsplit == 1, MODE 0: f0 <- f, f <- real-space force (clear_foc + real_space)
+// rsplit == 1, MODE 1: f += real-space force (real_space() after clear_foc)
+// rsplit > 1: the split's partial force goes to rpf; k_force_vv sums them.
 // ======================================================================
 enum { kRealTable64 = 0, kRealAnalytic64 = 1, kRealAnalytic32 = 2, kRealTable32 = 3 };
 
-template <int KIND>
 struct RealAcc {
   double fx = 0, fy = 0, fz = 0, pot = 0, vx = 0, vy = 0, vz = 0;
   unsigned long long np = 0;
 };
 
+// Pair-function constants of a home ion against partner species 0 and 1
+// (analytic FP32); selected per pair by the partner's species.
+struct Row32 {
+  float zz, cb, sig, c6, d8;
+};
+
+struct HomeCtx {
+  int i, ti;
+  bool valid;
+  double xi, yi, zi;
+  const double2* t64[4];  // table rows against partner species 0..3
+  const float2* t32[4];
+  double Zi;
+  int ta[4];              // analytic64 pair type against partner species
+  Row32 row[2];
+};
+
+struct BoxCtx {
+  double hx, hy, hz, Lx, Ly, Lz, rc2;
+  int nb;
+  float rc2f, alpha, two_a3, inv_rho;
+};
+
 template <int KIND>
-__device__ __forceinline__ void real_tile(const BoxDesc& B, const Pair32& P, int i, int ti,
-                                          double xi, double yi, double zi, bool valid,
+__device__ __forceinline__ void make_home(const BoxDesc& B, int i, int ti, bool valid, double x,
+                                          double y, double z, HomeCtx& h) {
+  h.i = i; h.ti = ti; h.valid = valid; h.xi = x; h.yi = y; h.zi = z;
+  h.Zi = B.Z[ti];
+  for (int s = 0; s < 4; ++s) {
+    const int t = s < B.kinds ? B.lookup[ti * B.kinds + s] : 0;
+    if constexpr (KIND == kRealTable64) h.t64[s] = B.table + (size_t)t * B.table_nb;
+    if constexpr (KIND == kRealTable32) h.t32[s] = B.table32 + (size_t)t * B.table_nb;
+    h.ta[s] = t;
+  }
+  if constexpr (KIND == kRealAnalytic32) {
+    for (int s = 0; s < 2; ++s) {
+      const int t = B.lookup[ti * B.kinds + s];
+      h.row[s].zz = (float)(B.Z[ti] * B.Z[s]);
+      h.row[s].cb = (float)B.cb[t];
+      h.row[s].sig = (float)B.sig[t];
+      h.row[s].c6 = (float)B.Cd[t];
+      h.row[s].d8 = (float)B.Dd[t];
+    }
+  }
+}
+
+// mdc's single minimum-image fold: '>' on every axis first, then '<'
+// (mdc/program/real.c:43-49)
+__device__ __forceinline__ void fold(const BoxCtx& c, double& dx, double& dy, double& dz) {
+  if (dx > c.hx) dx -= c.Lx;
+  if (dy > c.hy) dy -= c.Ly;
+  if (dz > c.hz) dz -= c.Lz;
+  if (dx < -c.hx) dx += c.Lx;
+  if (dy < -c.hy) dy += c.Ly;
+  if (dz < -c.hz) dz += c.Lz;
+}
+
+// FP32 analytic pair function: -U'(r)/r and U(r) of erfc Coulomb + BMH.
+__device__ __forceinline__ void pair32(const BoxCtx& c, const Row32& w, float r2, float& p,
+                                       float& f) {
+  const float kLog2e = 1.4426950408889634f;
+  const float rinv = __frsqrt_rn(r2);
+  const float r = r2 * rinv;
+  const float rinv2 = rinv * rinv;
+  const float z = c.alpha * r;
+  const float erfc_r = erfc_cheb(z) * rinv;
+  const float g = exp2f(-z * z * kLog2e);
+  const float eb = w.cb * exp2f((w.sig - r) * c.inv_rho * kLog2e);
+  const float r6 = rinv2 * rinv2 * rinv2;
+  const float r8 = r6 * rinv2;
+  p = w.zz * erfc_r + eb - w.c6 * r6 - w.d8 * r8;
+  f = w.zz * (c.two_a3 * g + erfc_r) * rinv2 +
+      (eb * c.inv_rho * rinv - 6.0f * w.c6 * r8 - 8.0f * w.d8 * r8 * rinv2);
+}
+
+// FP64 analytic pair function in the operation order of mdc's mk_table
+// (mdc/program/control.c:219-240), erfc per pair as pr-mdc's ewald1 does
+// (pr-mdc/program/ewald.c:37-43).
+__device__ __forceinline__ void pair64(const BoxDesc& B, int t, double ZZ, double dr, double& p,
+                                       double& f) {
+  const double dr6 = dr * dr * dr * dr * dr * dr;
+  const double dr8 = dr6 * dr * dr;
+  const double adr = B.alpha * dr;
+  const double erfc_per_dr = mori_erfc(adr) / dr;
+  const double ebmh = B.cb[t] * exp((B.sig[t] - dr) / B.rho);
+  p = ZZ * erfc_per_dr + (ebmh - B.Cd[t] / dr6 - B.Dd[t] / dr8);
+  const double fe = ZZ * (2.0 * B.a3 * exp(-adr * adr) + erfc_per_dr) / (dr * dr);
+  const double fr = (ebmh / B.rho - 6.0 * B.Cd[t] / (dr6 * dr) - 8.0 * B.Dd[t] / (dr8 * dr)) / dr;
+  f = fe + fr;
+}
+
+// Partners u in [0, cnt) of one LDS tile, in order.
+template <int KIND>
+__device__ __forceinline__ void real_tile(const BoxDesc& B, const BoxCtx& c, const HomeCtx& h,
                                           const double* jx, const double* jy, const double* jz,
-                                          const int* jt, const int* jid, int cnt,
-                                          RealAcc<KIND>& a, int& err) {
-  const double hx = B.halfL[0], hy = B.halfL[1], hz = B.halfL[2];
-  const double Lx = B.L[0], Ly = B.L[1], Lz = B.L[2];
-  float ffx = 0.f, ffy = 0.f, ffz = 0.f, fpot = 0.f, fvx = 0.f, fvy = 0.f, fvz = 0.f;
-  for (int u = 0; u < cnt; ++u) {
-    const int j = jid[u];
-    double dx = xi - jx[u], dy = yi - jy[u], dz = zi - jz[u];
-    // mdc's single fold, '>' on all axes first, then '<' (real.c:43-49)
-    if (dx > hx) dx -= Lx;
-    if (dy > hy) dy -= Ly;
-    if (dz > hz) dz -= Lz;
-    if (dx < -hx) dx += Lx;
-    if (dy < -hy) dy += Ly;
-    if (dz < -hz) dz += Lz;
-    const bool upper = j > i;
-    if constexpr (KIND == kRealTable64 || KIND == kRealAnalytic64 || KIND == kRealTable32) {
-      const double dr = dx * dx + dy * dy + dz * dz;  // unfused, as mdc
-      if (!valid || j == i || !(dr <= B.rc2)) continue;
-      const int t = B.lookup[ti * B.kinds + jt[u]];
-      if constexpr (KIND == kRealTable64 || KIND == kRealTable32) {
-        const int bin = (int)((sqrt(dr) - 0.5) * 1000.0);  // truncating, real.c:62
-        if (bin < 0 || bin >= B.table_nb) { err |= kErrRange; continue; }
-        if constexpr (KIND == kRealTable64) {
-          const double2 pf = B.table[t * B.table_nb + bin];
-          a.fx += pf.y * dx; a.fy += pf.y * dy; a.fz += pf.y * dz;
-          if (upper) {
-            a.pot += pf.x;
-            a.vx += pf.y * dx * dx; a.vy += pf.y * dy * dy; a.vz += pf.y * dz * dz;
-            ++a.np;
-          }
-        } else {
-          const float2 pf = B.table32[t * B.table_nb + bin];
-          const float fdx = (float)dx, fdy = (float)dy, fdz = (float)dz;
-          ffx = fmaf(pf.y, fdx, ffx); ffy = fmaf(pf.y, fdy, ffy); ffz = fmaf(pf.y, fdz, ffz);
-          if (upper) {
-            fpot += pf.x;
-            fvx = fmaf(pf.y * fdx, fdx, fvx); fvy = fmaf(pf.y * fdy, fdy, fvy);
-            fvz = fmaf(pf.y * fdz, fdz, fvz);
-            ++a.np;
-          }
-        }
-      } else {
-        double p, f;
-        pair_analytic64(B, t, B.Z[ti] * B.Z[jt[u]], sqrt(dr), p, f);
-        a.fx += f * dx; a.fy += f * dy; a.fz += f * dz;
-        if (upper) {
-          a.pot += p;
-          a.vx += f * dx * dx; a.vy += f * dy * dy; a.vz += f * dz * dz;
+                                          const int* jt, const int* jid, int cnt, RealAcc& a,
+                                          int& err) {
+  if constexpr (KIND == kRealTable64) {
+    // 8 partners at a time: the table gathers are issued together, the
+    // accumulation stays in partner order
+    int u = 0;
+    for (; u + 8 <= cnt; u += 8) {
+      double dx[8], dy[8], dz[8];
+      bool ok[8];
+      double2 pf[8];
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        dx[q] = h.xi - jx[u + q]; dy[q] = h.yi - jy[u + q]; dz[q] = h.zi - jz[u + q];
+        fold(c, dx[q], dy[q], dz[q]);
+        const double dr = dx[q] * dx[q] + dy[q] * dy[q] + dz[q] * dz[q];  // unfused, as mdc
+        ok[q] = h.valid && jid[u + q] != h.i && dr <= c.rc2;
+        int bin = (int)((sqrt(dr) - 0.5) * 1000.0);  // truncating, real.c:62
+        if (ok[q] && (bin < 0 || bin >= c.nb)) { err |= kErrRange; ok[q] = false; }
+        bin = ok[q] ? bin : 0;
+        const int s = jt[u + q];
+        const double2* row = s == 0 ? h.t64[0] : (s == 1 ? h.t64[1] : (s == 2 ? h.t64[2] : h.t64[3]));
+        pf[q] = row[bin];
+      }
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        if (!ok[q]) continue;
+        a.fx += pf[q].y * dx[q]; a.fy += pf[q].y * dy[q]; a.fz += pf[q].y * dz[q];
+        if (jid[u + q] > h.i) {
+          a.pot += pf[q].x;
+          a.vx += pf[q].y * dx[q] * dx[q]; a.vy += pf[q].y * dy[q] * dy[q];
+          a.vz += pf[q].y * dz[q] * dz[q];
           ++a.np;
         }
       }
-    } else {  // kRealAnalytic32: fold in FP64, pair arithmetic in FP32
-      const float fdx = (float)dx, fdy = (float)dy, fdz = (float)dz;
-      const float r2 = fmaf(fdx, fdx, fmaf(fdy, fdy, fdz * fdz));
-      if (!valid || j == i || !(r2 <= P.rc2)) continue;
-      const int t = B.lookup[ti * B.kinds + jt[u]];
-      float p, f;
-      pair_analytic32(P, t, r2, p, f);
+    }
+    for (; u < cnt; ++u) {
+      double dx = h.xi - jx[u], dy = h.yi - jy[u], dz = h.zi - jz[u];
+      fold(c, dx, dy, dz);
+      const double dr = dx * dx + dy * dy + dz * dz;
+      if (!h.valid || jid[u] == h.i || !(dr <= c.rc2)) continue;
+      const int bin = (int)((sqrt(dr) - 0.5) * 1000.0);
+      if (bin < 0 || bin >= c.nb) { err |= kErrRange; continue; }
+      const int s = jt[u];
+      const double2* row = s == 0 ? h.t64[0] : (s == 1 ? h.t64[1] : (s == 2 ? h.t64[2] : h.t64[3]));
+      const double2 pf = row[bin];
+      a.fx += pf.y * dx; a.fy += pf.y * dy; a.fz += pf.y * dz;
+      if (jid[u] > h.i) {
+        a.pot += pf.x;
+        a.vx += pf.y * dx * dx; a.vy += pf.y * dy * dy; a.vz += pf.y * dz * dz;
+        ++a.np;
+      }
+    }
+  } else if constexpr (KIND == kRealAnalytic64) {
+    for (int u = 0; u < cnt; ++u) {
+      double dx = h.xi - jx[u], dy = h.yi - jy[u], dz = h.zi - jz[u];
+      fold(c, dx, dy, dz);
+      const double dr = dx * dx + dy * dy + dz * dz;
+      if (!h.valid || jid[u] == h.i || !(dr <= c.rc2)) continue;
+      const int s = jt[u];
+      const int t = s == 0 ? h.ta[0] : (s == 1 ? h.ta[1] : (s == 2 ? h.ta[2] : h.ta[3]));
+      double p, f;
+      pair64(B, t, h.Zi * B.Z[s], sqrt(dr), p, f);
+      a.fx += f * dx; a.fy += f * dy; a.fz += f * dz;
+      if (jid[u] > h.i) {
+        a.pot += p;
+        a.vx += f * dx * dx; a.vy += f * dy * dy; a.vz += f * dz * dz;
+        ++a.np;
+      }
+    }
+  } else {
+    // FP32 pair arithmetic; the fold stays FP64 (positions are FP64).
+    // Tile sums (at most 64 pairs) in FP32, running sums in FP64.
+    float ffx = 0.f, ffy = 0.f, ffz = 0.f, fpot = 0.f, fvx = 0.f, fvy = 0.f, fvz = 0.f;
+    for (int u = 0; u < cnt; ++u) {
+      double dx = h.xi - jx[u], dy = h.yi - jy[u], dz = h.zi - jz[u];
+      fold(c, dx, dy, dz);
+      float p, f, fdx, fdy, fdz;
+      if constexpr (KIND == kRealAnalytic32) {
+        fdx = (float)dx; fdy = (float)dy; fdz = (float)dz;
+        const float r2 = fmaf(fdx, fdx, fmaf(fdy, fdy, fdz * fdz));
+        if (!h.valid || jid[u] == h.i || !(r2 <= c.rc2f)) continue;
+        pair32(c, jt[u] == 0 ? h.row[0] : h.row[1], r2, p, f);
+      } else {  // kRealTable32: FP64 bin as mdc, FP32 table values
+        const double dr = dx * dx + dy * dy + dz * dz;
+        if (!h.valid || jid[u] == h.i || !(dr <= c.rc2)) continue;
+        const int bin = (int)((sqrt(dr) - 0.5) * 1000.0);
+        if (bin < 0 || bin >= c.nb) { err |= kErrRange; continue; }
+        const int s = jt[u];
+        const float2* row = s == 0 ? h.t32[0] : (s == 1 ? h.t32[1] : (s == 2 ? h.t32[2] : h.t32[3]));
+        const float2 pf = row[bin];
+        p = pf.x; f = pf.y;
+        fdx = (float)dx; fdy = (float)dy; fdz = (float)dz;
+      }
       ffx = fmaf(f, fdx, ffx); ffy = fmaf(f, fdy, ffy); ffz = fmaf(f, fdz, ffz);
-      if (upper) {
+      if (jid[u] > h.i) {
         fpot += p;
         fvx = fmaf(f * fdx, fdx, fvx); fvy = fmaf(f * fdy, fdy, fvy); fvz = fmaf(f * fdz, fdz, fvz);
         ++a.np;
       }
     }
-  }
-  if constexpr (KIND == kRealAnalytic32 || KIND == kRealTable32) {
-    // tile sums (at most 64 pairs) in FP32, running sums in FP64
     a.fx += ffx; a.fy += ffy; a.fz += ffz; a.pot += fpot;
     a.vx += fvx; a.vy += fvy; a.vz += fvz;
   }
 }
 
-template <int KIND>
-__device__ __forceinline__ void real_store(const BoxDesc& B, int i, bool valid, int mode,
-                                           const RealAcc<KIND>& a) {
-  if (!valid) return;
-  if (mode == 0) {
+__device__ __forceinline__ void real_store(const BoxDesc& B, const HomeCtx& h, int split,
+                                           int mode, const RealAcc& a) {
+  if (!h.valid) return;
+  const int i = h.i;
+  if (B.rsplit > 1) {
+    double* p = B.rpf + (size_t)split * 3 * B.N;
+    p[i] = a.fx; p[B.N + i] = a.fy; p[2 * B.N + i] = a.fz;
+  } else if (mode == 0) {
     B.fx0[i] = B.fx[i]; B.fy0[i] = B.fy[i]; B.fz0[i] = B.fz[i];
     B.fx[i] = a.fx; B.fy[i] = a.fy; B.fz[i] = a.fz;
   } else {
@@ -244,78 +352,67 @@ __global__ __launch_bounds__(kRealThreads) void k_real(const BoxDesc* __restrict
   const Item it = items[blockIdx.x];
   const BoxDesc& B = boxes[it.box];
   const int lane = threadIdx.x;
+  const int N = B.N, split = it.c, nsplit = B.rsplit;
   int err = 0;
-  Pair32 P;
-  if constexpr (KIND == kRealAnalytic32) {
-    P.rc2 = (float)B.rc2; P.alpha = (float)B.alpha; P.two_a3 = (float)(2.0 * B.a3);
-    P.inv_rho = (float)(1.0 / B.rho);
-    for (int ti = 0; ti < 2; ++ti)
-      for (int tj = ti; tj < 2; ++tj) {
-        const int t = B.lookup[ti * B.kinds + tj];
-        P.zz[t] = (float)(B.Z[ti] * B.Z[tj]);
-      }
-    for (int t = 0; t < 3; ++t) {
-      P.cb[t] = (float)B.cb[t]; P.sig[t] = (float)B.sig[t];
-      P.c6[t] = (float)B.Cd[t]; P.d8[t] = (float)B.Dd[t];
-    }
-  }
-  RealAcc<KIND> tot;  // item totals of pot / virial (per lane, then wave)
+  BoxCtx c;
+  c.hx = B.halfL[0]; c.hy = B.halfL[1]; c.hz = B.halfL[2];
+  c.Lx = B.L[0]; c.Ly = B.L[1]; c.Lz = B.L[2];
+  c.rc2 = B.rc2;
+  c.nb = B.table_nb;
+  c.rc2f = (float)B.rc2; c.alpha = (float)B.alpha; c.two_a3 = (float)(2.0 * B.a3);
+  c.inv_rho = (float)(1.0 / B.rho);
+  RealAcc tot;  // pot / virial / pair count of the item
   if (B.all_pairs) {
     const int i = it.a + lane;
-    const bool valid = i < B.N;
+    const bool valid = i < N;
     const int ii = valid ? i : 0;
-    const double xi = B.rx[ii], yi = B.ry[ii], zi = B.rz[ii];
-    const int ti = B.typ[ii];
-    RealAcc<KIND> a;
-    for (int j0 = 0; j0 < B.N; j0 += kRealThreads) {
+    HomeCtx h;
+    make_home<KIND>(B, i, B.typ[ii], valid, B.rx[ii], B.ry[ii], B.rz[ii], h);
+    const int js = (int)((long)split * N / nsplit), je = (int)((long)(split + 1) * N / nsplit);
+    RealAcc a;
+    for (int j0 = js; j0 < je; j0 += kRealThreads) {
       const int j = j0 + lane;
-      if (j < B.N) {
+      if (j < je) {
         jx[lane] = B.rx[j]; jy[lane] = B.ry[j]; jz[lane] = B.rz[j];
         jt[lane] = B.typ[j]; jid[lane] = j;
       }
       __syncthreads();
-      const int cnt = min(kRealThreads, B.N - j0);
-      real_tile<KIND>(B, P, i, ti, xi, yi, zi, valid, jx, jy, jz, jt, jid, cnt, a, err);
+      real_tile<KIND>(B, c, h, jx, jy, jz, jt, jid, min(kRealThreads, je - j0), a, err);
       __syncthreads();
     }
-    real_store<KIND>(B, i, valid, mode, a);
+    real_store(B, h, split, mode, a);
     tot = a;
   } else {
-    const int c = it.a;
+    const int cell = it.a;
     const int ncx = B.nc[0], ncy = B.nc[1], ncz = B.nc[2];
-    const int cx = c % ncx, cy = (c / ncx) % ncy, cz = c / (ncx * ncy);
-    const int hs = B.cell_start[c], he = B.cell_start[c + 1];
+    const int cx = cell % ncx, cy = (cell / ncx) % ncy, cz = cell / (ncx * ncy);
+    const int hs = B.cell_start[cell], he = B.cell_start[cell + 1];
+    const int n0 = split * 27 / nsplit, n1 = (split + 1) * 27 / nsplit;
     for (int h0 = hs; h0 < he; h0 += kRealThreads) {
       const int hp = h0 + lane;
       const bool valid = hp < he;
       const int hq = valid ? hp : hs;
-      const int i = B.sorted[hq];
-      const double xi = B.sx[hq], yi = B.sy[hq], zi = B.sz[hq];
-      const int ti = B.styp[hq];
-      RealAcc<KIND> a;
-      for (int oz = -1; oz <= 1; ++oz) {
-        const int nz = (cz + oz + ncz) % ncz;
-        for (int oy = -1; oy <= 1; ++oy) {
-          const int ny = (cy + oy + ncy) % ncy;
-          for (int ox = -1; ox <= 1; ++ox) {
-            const int nx = (cx + ox + ncx) % ncx;
-            const int nb = (nz * ncy + ny) * ncx + nx;
-            const int ns = B.cell_start[nb], ne = B.cell_start[nb + 1];
-            for (int j0 = ns; j0 < ne; j0 += kRealThreads) {
-              const int jp = j0 + lane;
-              if (jp < ne) {
-                jx[lane] = B.sx[jp]; jy[lane] = B.sy[jp]; jz[lane] = B.sz[jp];
-                jt[lane] = B.styp[jp]; jid[lane] = B.sorted[jp];
-              }
-              __syncthreads();
-              const int cnt = min(kRealThreads, ne - j0);
-              real_tile<KIND>(B, P, i, ti, xi, yi, zi, valid, jx, jy, jz, jt, jid, cnt, a, err);
-              __syncthreads();
-            }
+      HomeCtx h;
+      make_home<KIND>(B, B.sorted[hq], B.styp[hq], valid, B.sx[hq], B.sy[hq], B.sz[hq], h);
+      RealAcc a;
+      for (int n = n0; n < n1; ++n) {
+        const int oz = n / 9 - 1, oy = (n / 3) % 3 - 1, ox = n % 3 - 1;
+        const int nzc = (cz + oz + ncz) % ncz, nyc = (cy + oy + ncy) % ncy,
+                  nxc = (cx + ox + ncx) % ncx;
+        const int nb = (nzc * ncy + nyc) * ncx + nxc;
+        const int ns = B.cell_start[nb], ne = B.cell_start[nb + 1];
+        for (int j0 = ns; j0 < ne; j0 += kRealThreads) {
+          const int jp = j0 + lane;
+          if (jp < ne) {
+            jx[lane] = B.sx[jp]; jy[lane] = B.sy[jp]; jz[lane] = B.sz[jp];
+            jt[lane] = B.styp[jp]; jid[lane] = B.sorted[jp];
           }
+          __syncthreads();
+          real_tile<KIND>(B, c, h, jx, jy, jz, jt, jid, min(kRealThreads, ne - j0), a, err);
+          __syncthreads();
         }
       }
-      real_store<KIND>(B, i, valid, mode, a);
+      real_store(B, h, split, mode, a);
       tot.pot += a.pot; tot.vx += a.vx; tot.vy += a.vy; tot.vz += a.vz; tot.np += a.np;
     }
   }
@@ -331,123 +428,177 @@ __global__ __launch_bounds__(kRealThreads) void k_real(const BoxDesc* __restrict
 }
 
 // ======================================================================
-// Reciprocal space, pass A: partial structure factors
-//   S(k) = sum_j Z_j exp(2 pi i h.r_j / L)  over one chunk of ions
-// (mdc/program/rcprcl.c:59-78).  A thread owns kSkIpt ions; for each k
-// column (hx, hy) it forms Z X[hx] Y[hy] (X, Y: per-ion phase tables) and
-// steps hz with one complex multiply per k.  Sub-blocks of kSkSub k
-// vectors are summed over the workgroup through LDS in a fixed order.
+// Reciprocal space, pass A: partial structure factors over one ion chunk
+//   S(k) = sum_j Z_j X_j[hx] Y_j[hy] W_j[hz],   X_j[h] = e^{2 pi i h x_j/Lx} ...
+// (the sum of mdc/program/rcprcl.c:59-78 in separable form).  One wave per
+// item; lane s owns one k segment (up to 8 consecutive hz of one column)
+// and keeps its 8 sums in registers over the whole chunk: per (k, ion) one
+// complex multiply-add, no cross-lane reduction.  Ion tiles of X, Y and
+// Z*W rows are staged in LDS ([ion][h] layout, so lanes at different h
+// read different banks).  Tile sums in R, chunk sums in FP64.
 // ======================================================================
 template <typename R>
-__global__ __launch_bounds__(kSkThreads) void k_sk_partial(const BoxDesc* __restrict__ boxes,
-                                                           const Item* __restrict__ items) {
-  __shared__ double red[2 * kSkSub][kSkThreads + 2];
+__global__ __launch_bounds__(kSkThreads) void k_sk_seg(const BoxDesc* __restrict__ boxes,
+                                                       const Item* __restrict__ items, int hs,
+                                                       int hsw) {
+  extern __shared__ unsigned char smem[];
+  Cx<R>* Xt = reinterpret_cast<Cx<R>*>(smem);    // [kSkTile][hs]
+  Cx<R>* Yt = Xt + kSkTile * hs;                  // [kSkTile][hs]
+  Cx<R>* Wt = Yt + kSkTile * hs;                  // [kSkTile][hsw], hsw >= H + kSegLen - 1
   const Item it = items[blockIdx.x];
   const BoxDesc& B = boxes[it.box];
-  const int N = B.N, tid = threadIdx.x;
-  const void* tx = B.tabx;
-  const void* ty = B.taby;
-  R zq[kSkIpt];
-  Cx<R> w1[kSkIpt];
-  int ion[kSkIpt];
-#pragma unroll
-  for (int q = 0; q < kSkIpt; ++q) {
-    const int i = it.a * kSkChunk + q * kSkThreads + tid;
-    const bool v = i < N;
-    ion[q] = v ? i : 0;
-    zq[q] = v ? (R)B.Z[B.typ[i]] : (R)0;
-    w1[q] = ld_cx<R>(B.tabw, ion[q]);
+  const int N = B.N, H = B.hmax + 1, tid = threadIdx.x;
+  const bool sv = tid < it.c;
+  KSeg sg = {0, 0, 0, 0, 0, {0, 0, 0}};
+  if (sv) {
+    const cptr<KSeg> sp = as_const(B.segs) + it.b + tid;
+    sg.hx = sp->hx; sg.hy = sp->hy; sg.hz0 = sp->hz0; sg.n = sp->n; sg.k0 = sp->k0;
   }
-  const int v = tid >> 4, seg = tid & 15;
-  for (int c = it.b; c < it.b + it.c; ++c) {
-    const KCol col = B.cols[c];
-    const int ax = col.hx < 0 ? -col.hx : col.hx, ay = col.hy < 0 ? -col.hy : col.hy;
-    Cx<R> t[kSkIpt];
+  const int ax = sg.hx < 0 ? -sg.hx : sg.hx, ay = sg.hy < 0 ? -sg.hy : sg.hy;
+  const bool cxn = sg.hx < 0, cyn = sg.hy < 0;
+  Cx<double> acc[kSegLen];
 #pragma unroll
-    for (int q = 0; q < kSkIpt; ++q) {
-      Cx<R> X = conj_if(ld_cx<R>(tx, (long)ax * N + ion[q]), col.hx < 0);
-      Cx<R> Y = conj_if(ld_cx<R>(ty, (long)ay * N + ion[q]), col.hy < 0);
-      Cx<R> a = cmul(X, Y);
-      if (col.hz0) a = cmul(a, w1[q]);
-      t[q] = {a.x * zq[q], a.y * zq[q]};
-    }
-    for (int sb = 0; sb < col.len; sb += kSkSub) {
-      const int nsub = min(kSkSub, col.len - sb);
-      R ax_[kSkSub], ay_[kSkSub];
+  for (int q = 0; q < kSegLen; ++q) acc[q] = {0.0, 0.0};
+  const int jb = it.a * B.chunk, je = min(N, jb + B.chunk);
+  for (int j0 = jb; j0 < je; j0 += kSkTile) {
+    const int cnt = min(kSkTile, je - j0);
+    // stage the tile: element e = h * kSkTile + u of each table, four
+    // elements per thread in flight
+    const int E = H * kSkTile;
+    for (int e0 = tid; e0 < E; e0 += 4 * kSkThreads) {
+      Cx<R> xv[4], yv[4], wv[4];
+      int eu[4], eh[4];
 #pragma unroll
-      for (int s = 0; s < kSkSub; ++s) { ax_[s] = 0; ay_[s] = 0; }
+      for (int q = 0; q < 4; ++q) {
+        const int e = e0 + q * kSkThreads;
+        eu[q] = e % kSkTile;
+        eh[q] = e / kSkTile;
+        const bool ok = e < E && eu[q] < cnt;
+        const long g = (long)(ok ? eh[q] : 0) * N + j0 + (ok ? eu[q] : 0);
+        xv[q] = ld_cx<R>(B.tabx, g);
+        yv[q] = ld_cx<R>(B.taby, g);
+        wv[q] = ld_cx<R>(B.tabw, g);
+      }
 #pragma unroll
-      for (int q = 0; q < kSkIpt; ++q) {
-#pragma unroll
-        for (int s = 0; s < kSkSub; ++s) {
-          if (s < nsub) {
-            ax_[s] += t[q].x; ay_[s] += t[q].y;
-            t[q] = cmul(t[q], w1[q]);
-          }
+      for (int q = 0; q < 4; ++q) {
+        const int e = e0 + q * kSkThreads;
+        if (e < E && eu[q] < cnt) {
+          const R zq = (R)B.Z[B.typ[j0 + eu[q]]];
+          Xt[eu[q] * hs + eh[q]] = xv[q];
+          Yt[eu[q] * hs + eh[q]] = yv[q];
+          Wt[eu[q] * hsw + eh[q]] = {zq * wv[q].x, zq * wv[q].y};
         }
       }
-#pragma unroll
-      for (int s = 0; s < kSkSub; ++s) {
-        red[2 * s][tid] = (double)ax_[s];
-        red[2 * s + 1][tid] = (double)ay_[s];
-      }
-      __syncthreads();
-      double acc = 0.0;
-#pragma unroll
-      for (int u = 0; u < 16; ++u) acc += red[v][u * 16 + seg];
-#pragma unroll
-      for (int off = 8; off >= 1; off >>= 1) acc += __shfl_xor(acc, off, 16);
-      if (seg == 0 && (v >> 1) < nsub) {
-        double* dst = reinterpret_cast<double*>(B.skpart + (size_t)it.a * B.nk + col.k0 + sb + (v >> 1));
-        dst[v & 1] = acc;
-      }
-      __syncthreads();
     }
+    __syncthreads();
+    if (sv) {
+      Cx<R> t[kSegLen];
+#pragma unroll
+      for (int q = 0; q < kSegLen; ++q) t[q] = {(R)0, (R)0};
+      for (int u = 0; u < cnt; ++u) {
+        const Cx<R> p = cmul(conj_if(Xt[u * hs + ax], cxn), conj_if(Yt[u * hs + ay], cyn));
+        const Cx<R>* w = Wt + u * hsw + sg.hz0;
+#pragma unroll
+        for (int q = 0; q < kSegLen; ++q) cmac(t[q], p, w[q]);
+      }
+#pragma unroll
+      for (int q = 0; q < kSegLen; ++q) {
+        acc[q].x += (double)t[q].x;
+        acc[q].y += (double)t[q].y;
+      }
+    }
+    __syncthreads();
+  }
+  if (sv) {
+    double2* dst = B.skpart + (size_t)it.a * B.nk + sg.k0;
+#pragma unroll
+    for (int q = 0; q < kSegLen; ++q)
+      if (q < sg.n) dst[q] = make_double2(acc[q].x, acc[q].y);
   }
 }
 
 // ======================================================================
-// Reciprocal space, pass B: back-projection of S(k) onto each ion
+// Reciprocal space, pass B: back-projection onto each ion
 //   f_i += -(2/V) sum_k e^{-pi^2 m^2/alpha^2}/m^2 (c_i Ss - s_i Sc) m
-// (mdc/program/rcprcl.c:84-92), with c_i, s_i the ion's Z cos / Z sin.
-// One thread per ion over a group of k columns; A = cf 2Ss and B = cf 2Sc
-// are the same for all lanes and come through the scalar cache.  Column
-// sums run in R, the cross-column sums in FP64.
+// (mdc/program/rcprcl.c:84-92), c_i, s_i = Z_i cos, Z_i sin.  With
+// T_k = cf_k (2Ss_k + i 2Sc_k) the per-ion sum over one column (hx, hy) is
+//   sum_hz Re(X[hx] Y[hy] W[hz] T_k) = Re(Q U),  U = sum_hz W[hz] T_k,
+// and the z component needs V = sum_hz hz W[hz] T_k.  Lane = ion; its
+// W[0..HZ) row lives in registers; T_k comes through the scalar cache:
+// two complex multiply-adds per (k, ion).  Column sums in R, sums over
+// columns in FP64.
 // ======================================================================
-template <typename R>
+template <typename R, int HZ>
 __global__ __launch_bounds__(kKfThreads) void k_kforce(const BoxDesc* __restrict__ boxes,
                                                        const Item* __restrict__ items) {
   const Item it = items[blockIdx.x];
   const BoxDesc& B = boxes[it.box];
-  const int N = B.N;
+  const int N = B.N, H = B.hmax + 1;
   const int i = it.a + threadIdx.x;
   const bool valid = i < N;
   const int ii = valid ? i : 0;
-  const Cx<R> w1 = ld_cx<R>(B.tabw, ii);
-  const R* ab;
-  if constexpr (sizeof(R) == 8) ab = reinterpret_cast<const R*>(B.kab);
-  else ab = reinterpret_cast<const R*>(B.kab32);
+  Cx<R> W[HZ];
+#pragma unroll
+  for (int h = 0; h < HZ; ++h) W[h] = h < H ? ld_cx<R>(B.tabw, (long)h * N + ii) : Cx<R>{0, 0};
+  // Weight rows (4 columns x (A, B, hz A, hz B)) staged in LDS, up to
+  // kKfRows rows at a time, copied contiguously; lanes then read each row
+  // as a broadcast.  Per hz row one W[hz] register feeds four independent
+  // complex accumulations.
+  using R4 = typename std::conditional<sizeof(R) == 8, double4, float4>::type;
+  __shared__ R4 Tl[kKfRows * 4];
+  const R4* T;
+  if constexpr (sizeof(R) == 8) T = reinterpret_cast<const R4*>(B.kab);
+  else T = reinterpret_cast<const R4*>(B.kab32);
+  const cptr<KQuad> quads = as_const(B.quads);
   double Fx = 0.0, Fy = 0.0, Fz = 0.0;
-  const int cend = it.b + it.c;
-  for (int c = it.b; c < cend; ++c) {
-    const KCol col = B.cols[c];
-    const int ax = col.hx < 0 ? -col.hx : col.hx, ay = col.hy < 0 ? -col.hy : col.hy;
-    Cx<R> t = cmul(conj_if(ld_cx<R>(B.tabx, (long)ax * N + ii), col.hx < 0),
-                   conj_if(ld_cx<R>(B.taby, (long)ay * N + ii), col.hy < 0));
-    if (col.hz0) t = cmul(t, w1);
-    R G = 0, Gz = 0, hz = (R)col.hz0;
-    const R* p = ab + 2 * (size_t)col.k0;
-    for (int j = 0; j < col.len; ++j) {
-      const R A = p[2 * j], Bv = p[2 * j + 1];
-      const R g = t.x * A - t.y * Bv;
-      G += g;
-      Gz += g * hz;
-      hz += (R)1;
-      t = cmul(t, w1);
+  const int qend = it.b + it.c;
+  int q = it.b;
+  while (q < qend) {
+    const int r0 = quads[q].row0;
+    int qe = q + 1;
+    while (qe < qend && quads[qe].row0 + quads[qe].nrow - r0 <= kKfRows) ++qe;
+    const int nslots = 4 * (quads[qe - 1].row0 + quads[qe - 1].nrow - r0);
+    for (int e = threadIdx.x; e < nslots; e += kKfThreads) Tl[e] = T[4 * (size_t)r0 + e];
+    __syncthreads();
+    for (int qq = q; qq < qe; ++qq) {
+      const int lo = quads[qq].lo, hi = lo + quads[qq].nrow, nc = quads[qq].ncol;
+      int hxs[4], hys[4];
+      Cx<R> Q[4], U[4], V[4];
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        hxs[c] = quads[qq].hx[c];
+        hys[c] = quads[qq].hy[c];
+        U[c] = {0, 0};
+        V[c] = {0, 0};
+        Q[c] = {0, 0};
+        if (c < nc) {
+          const int ax = hxs[c] < 0 ? -hxs[c] : hxs[c], ay = hys[c] < 0 ? -hys[c] : hys[c];
+          Q[c] = cmul(conj_if(ld_cx<R>(B.tabx, (long)ax * N + ii), hxs[c] < 0),
+                      conj_if(ld_cx<R>(B.taby, (long)ay * N + ii), hys[c] < 0));
+        }
+      }
+      const R4* t = Tl + 4 * (quads[qq].row0 - r0 - lo);
+#pragma unroll
+      for (int hz = 0; hz < HZ; ++hz) {
+        if (hz >= lo && hz < hi) {
+          const R4 t0 = t[4 * hz], t1 = t[4 * hz + 1], t2 = t[4 * hz + 2], t3 = t[4 * hz + 3];
+          cmac(U[0], Cx<R>{t0.x, t0.y}, W[hz]); cmac(V[0], Cx<R>{t0.z, t0.w}, W[hz]);
+          cmac(U[1], Cx<R>{t1.x, t1.y}, W[hz]); cmac(V[1], Cx<R>{t1.z, t1.w}, W[hz]);
+          cmac(U[2], Cx<R>{t2.x, t2.y}, W[hz]); cmac(V[2], Cx<R>{t2.z, t2.w}, W[hz]);
+          cmac(U[3], Cx<R>{t3.x, t3.y}, W[hz]); cmac(V[3], Cx<R>{t3.z, t3.w}, W[hz]);
+        }
+      }
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        const R G = Q[c].x * U[c].x - Q[c].y * U[c].y;
+        const R Gz = Q[c].x * V[c].x - Q[c].y * V[c].y;
+        Fx += (double)hxs[c] * (double)G;
+        Fy += (double)hys[c] * (double)G;
+        Fz += (double)Gz;
+      }
     }
-    Fx += (double)col.hx * (double)G;
-    Fy += (double)col.hy * (double)G;
-    Fz += (double)Gz;
+    __syncthreads();
+    q = qe;
   }
   if (valid) {
     const double zq = B.Z[B.typ[i]];

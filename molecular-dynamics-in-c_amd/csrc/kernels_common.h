@@ -1,8 +1,8 @@
 // kernels_common.h -- FP64 kernels shared by both precision modes: cell
-// lists, phase tables, structure-factor finalisation, the velocity-Verlet
-// update and per-box reductions.  Included only by k_fp64.hip, which is
-// compiled with -ffp-contract=off: the integrator then rounds exactly as
-// mdc's next_v / next_r do on the host.
+// lists, phase tables, structure-factor finalisation, force assembly with
+// the velocity-Verlet update, and per-box reductions.  Included only by
+// k_fp64.hip, which is compiled with -ffp-contract=off: the integrator then
+// rounds exactly as mdc's next_v / next_r do on the host.
 #pragma once
 
 #include "kernels.h"
@@ -36,18 +36,25 @@ __device__ __forceinline__ void block_sum(double (&v)[NV]) {
 // ----------------------------------------------------------------------
 // Cell list of one box per workgroup.  Binning is integer and exact:
 // c_a = (int)(r_a * (n_a / L_a)) clamped to n_a - 1 (r_a == L_a is a legal
-// coordinate after mdc's fold, rv.c:59-64).  Counting sort by cell, then
-// each cell's list sorted by ion index, so the order is deterministic.
-// Also writes cell-ordered copies of positions and species.
+// coordinate after mdc's fold, rv.c:59-64).  Counts and starts live in LDS
+// for up to kCellLds cells.  The scatter is stable (ions of a cell keep
+// ascending index): one wave walks the ions in order, 64 at a time, and
+// ranks lanes of equal cell with ballots, so the order never depends on
+// timing.  Also writes cell-ordered copies of positions and species.
 // ----------------------------------------------------------------------
 __global__ __launch_bounds__(kCellThreads) void k_cell_build(const BoxDesc* __restrict__ boxes,
                                                              const int* __restrict__ box_ids) {
-  __shared__ int sh[kCellThreads];
+  __shared__ int s_cnt[kCellLds];
+  __shared__ int s_start[kCellLds + 1];
+  __shared__ int s_scan[kCellThreads];
   const BoxDesc& B = boxes[box_ids[blockIdx.x]];
   const int N = B.N, ncell = B.ncell, tid = threadIdx.x;
   const int ncx = B.nc[0], ncy = B.nc[1], ncz = B.nc[2];
+  const bool lds = ncell <= kCellLds;
+  int* cnt = lds ? s_cnt : B.cell_count;
+  int* start = lds ? s_start : B.cell_start;
   int err = 0;
-  for (int c = tid; c < ncell; c += kCellThreads) B.cell_count[c] = 0;
+  for (int c = tid; c < ncell; c += kCellThreads) cnt[c] = 0;
   __syncthreads();
   for (int i = tid; i < N; i += kCellThreads) {
     const double x = B.rx[i], y = B.ry[i], z = B.rz[i];
@@ -61,49 +68,61 @@ __global__ __launch_bounds__(kCellThreads) void k_cell_build(const BoxDesc* __re
     cz = min(max(cz, 0), ncz - 1);
     const int c = (cz * ncy + cy) * ncx + cx;
     B.cell_of[i] = c;
-    atomicAdd(&B.cell_count[c], 1);
+    atomicAdd(&cnt[c], 1);
   }
   __syncthreads();
-  // exclusive scan of counts -> cell_start
+  // exclusive scan of counts -> start; counts become fill cursors
   const int per = (ncell + kCellThreads - 1) / kCellThreads;
   const int c0 = min(tid * per, ncell), c1 = min(c0 + per, ncell);
   int local = 0;
-  for (int c = c0; c < c1; ++c) local += B.cell_count[c];
-  sh[tid] = local;
+  for (int c = c0; c < c1; ++c) local += cnt[c];
+  s_scan[tid] = local;
   __syncthreads();
   for (int off = 1; off < kCellThreads; off <<= 1) {
-    const int add = tid >= off ? sh[tid - off] : 0;
+    const int add = tid >= off ? s_scan[tid - off] : 0;
     __syncthreads();
-    sh[tid] += add;
+    s_scan[tid] += add;
     __syncthreads();
   }
-  int run = sh[tid] - local;
+  int run = s_scan[tid] - local;
   for (int c = c0; c < c1; ++c) {
-    B.cell_start[c] = run;
-    run += B.cell_count[c];
-    B.cell_count[c] = 0;  // becomes the fill cursor
+    start[c] = run;
+    run += cnt[c];
+    cnt[c] = 0;
   }
-  if (tid == kCellThreads - 1) B.cell_start[ncell] = sh[tid];
+  if (tid == kCellThreads - 1) start[ncell] = s_scan[tid];
   __syncthreads();
-  for (int i = tid; i < N; i += kCellThreads) {
-    const int c = B.cell_of[i];
-    const int p = B.cell_start[c] + atomicAdd(&B.cell_count[c], 1);
-    B.sorted[p] = i;
-  }
-  __syncthreads();
-  for (int c = tid; c < ncell; c += kCellThreads) {  // insertion sort per cell
-    const int s = B.cell_start[c], e = B.cell_start[c + 1];
-    for (int p = s + 1; p < e; ++p) {
-      const int key = B.sorted[p];
-      int q = p - 1;
-      while (q >= s && B.sorted[q] > key) {
-        B.sorted[q + 1] = B.sorted[q];
-        --q;
+  if (tid < kWave) {
+    const int lane = tid;
+    const unsigned long long below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    for (int base = 0; base < N; base += kWave) {
+      const int i = base + lane;
+      const bool valid = i < N;
+      const int c = valid ? B.cell_of[i] : -1;
+      unsigned long long todo = __ballot(valid);
+      int rank = 0, first = 0, cur = 0;
+      unsigned long long mine = 0;
+      while (todo) {
+        const int leader = __ffsll((long long)todo) - 1;
+        const int lc = __shfl(c, leader, kWave);
+        const unsigned long long m = __ballot(c == lc);
+        if (c == lc) {
+          mine = m;
+          rank = __popcll(m & below);
+          first = leader;
+        }
+        todo &= ~m;
       }
-      B.sorted[q + 1] = key;
+      if (valid) cur = cnt[c];            // every lane reads before any lane writes
+      __builtin_amdgcn_wave_barrier();
+      if (valid && lane == first) cnt[c] = cur + __popcll(mine);
+      if (valid) B.sorted[start[c] + cur + rank] = i;
+      __builtin_amdgcn_wave_barrier();
     }
   }
   __syncthreads();
+  if (lds)
+    for (int c = tid; c <= ncell; c += kCellThreads) B.cell_start[c] = start[c];
   for (int p = tid; p < N; p += kCellThreads) {
     const int i = B.sorted[p];
     B.sx[p] = B.rx[i]; B.sy[p] = B.ry[i]; B.sz[p] = B.rz[i];
@@ -113,38 +132,35 @@ __global__ __launch_bounds__(kCellThreads) void k_cell_build(const BoxDesc* __re
 }
 
 // ----------------------------------------------------------------------
-// Per-ion phase tables X[h] = e^{2 pi i h x/Lx}, Y[h] = e^{2 pi i h y/Ly}
-// for h = 0..hmax and W = e^{2 pi i z/Lz}, from FP64 sincospi (argument
-// reduction by exact multiples of 2); stored FP64 or rounded to FP32.
+// Per-ion phase tables X[h] = e^{2 pi i h x/Lx}, Y[h], W[h] for
+// h = 0..hmax, from FP64 sincospi (argument reduction by exact multiples
+// of 2); stored FP64 or rounded to FP32.
 // ----------------------------------------------------------------------
 template <typename CT>
 __global__ __launch_bounds__(kIonThreads) void k_phase_tables(const BoxDesc* __restrict__ boxes,
                                                               const Item* __restrict__ items) {
+  using RT = decltype(CT{}.x);
   const Item it = items[blockIdx.x];
   const BoxDesc& B = boxes[it.box];
   const int N = B.N;
   const int i = it.a + threadIdx.x;
   if (i >= N) return;
-  const double s0 = B.rx[i] * B.invL[0], s1 = B.ry[i] * B.invL[1], s2 = B.rz[i] * B.invL[2];
-  CT* tx = reinterpret_cast<CT*>(B.tabx);
-  CT* ty = reinterpret_cast<CT*>(B.taby);
-  CT* tw = reinterpret_cast<CT*>(B.tabw);
-  for (int h = 0; h <= B.hmax; ++h) {
-    double sn, cs;
-    sincospi(2.0 * (double)h * s0, &sn, &cs);
-    tx[(size_t)h * N + i] = CT{(decltype(CT{}.x))cs, (decltype(CT{}.x))sn};
-    sincospi(2.0 * (double)h * s1, &sn, &cs);
-    ty[(size_t)h * N + i] = CT{(decltype(CT{}.x))cs, (decltype(CT{}.x))sn};
-  }
-  double sn, cs;
-  sincospi(2.0 * s2, &sn, &cs);
-  tw[i] = CT{(decltype(CT{}.x))cs, (decltype(CT{}.x))sn};
+  const double s[3] = {B.rx[i] * B.invL[0], B.ry[i] * B.invL[1], B.rz[i] * B.invL[2]};
+  CT* tab[3] = {reinterpret_cast<CT*>(B.tabx), reinterpret_cast<CT*>(B.taby),
+                reinterpret_cast<CT*>(B.tabw)};
+  for (int d = 0; d < 3; ++d)
+    for (int h = 0; h <= B.hmax; ++h) {
+      double sn, cs;
+      sincospi(2.0 * (double)h * s[d], &sn, &cs);
+      tab[d][(size_t)h * N + i] = CT{(RT)cs, (RT)sn};
+    }
 }
 
 // ----------------------------------------------------------------------
 // Structure factor per k from the chunk partials (fixed chunk order), with
 // mdc's doubling of both sums (rcprcl.c:77-78), the energy term
-// vp exp(..)/m^2 (2Sc^2 + 2Ss^2) and the back-projection weights.
+// vp exp(..)/m^2 (2Sc^2 + 2Ss^2) and the back-projection weights
+// T = cf (2Ss, 2Sc) and hz T.
 // ----------------------------------------------------------------------
 __global__ __launch_bounds__(kIonThreads) void k_sk_finalize(const BoxDesc* __restrict__ boxes,
                                                              const Item* __restrict__ items) {
@@ -161,19 +177,21 @@ __global__ __launch_bounds__(kIonThreads) void k_sk_finalize(const BoxDesc* __re
   sc *= 2.0;
   ss *= 2.0;
   B.kpot[k] = B.kce[k] * (sc * sc + ss * ss);
-  const double a = B.kcf[k] * ss, b = B.kcf[k] * sc;
-  B.kab[2 * k] = a;
-  B.kab[2 * k + 1] = b;
+  const double a = B.kcf[k] * ss, b = B.kcf[k] * sc, hz = (double)B.khz[k];
+  const size_t slot = (size_t)B.kslot[k];
+  double* t = B.kab + 4 * slot;
+  t[0] = a; t[1] = b; t[2] = hz * a; t[3] = hz * b;
   if (B.kab32) {
-    B.kab32[2 * k] = (float)a;
-    B.kab32[2 * k + 1] = (float)b;
+    float* u = B.kab32 + 4 * slot;
+    u[0] = (float)a; u[1] = (float)b; u[2] = (float)(hz * a); u[3] = (float)(hz * b);
   }
 }
 
 // ----------------------------------------------------------------------
-// Reciprocal forces added to f (groups in index order); with VV also
-// mdc's next_v then next_r (rv.c:33-66) in mdc's operation order, with the
-// one-fold strict-comparison wrap.
+// Force assembly: with split real space, f0 <- f and f <- sum of the
+// partials (split order); then the reciprocal partials are added (group
+// order).  With VV also mdc's next_v then next_r (rv.c:33-66) in mdc's
+// operation order, with the one-fold strict-comparison wrap.
 // ----------------------------------------------------------------------
 template <int VV>
 __global__ __launch_bounds__(kIonThreads) void k_force_vv(const BoxDesc* __restrict__ boxes,
@@ -183,7 +201,19 @@ __global__ __launch_bounds__(kIonThreads) void k_force_vv(const BoxDesc* __restr
   const int N = B.N;
   const int i = it.a + threadIdx.x;
   if (i >= N) return;
-  double fx = B.fx[i], fy = B.fy[i], fz = B.fz[i];
+  double fx, fy, fz;
+  if (B.rsplit > 1) {
+    fx = 0.0; fy = 0.0; fz = 0.0;
+    for (int s = 0; s < B.rsplit; ++s) {
+      const double* p = B.rpf + (size_t)s * 3 * N;
+      fx += p[i];
+      fy += p[N + i];
+      fz += p[2 * N + i];
+    }
+    B.fx0[i] = B.fx[i]; B.fy0[i] = B.fy[i]; B.fz0[i] = B.fz[i];
+  } else {
+    fx = B.fx[i]; fy = B.fy[i]; fz = B.fz[i];
+  }
   if (B.ewald) {
     for (int g = 0; g < B.ngroup; ++g) {
       const double* fp = B.fpart + (size_t)g * 3 * N;
@@ -191,8 +221,8 @@ __global__ __launch_bounds__(kIonThreads) void k_force_vv(const BoxDesc* __restr
       fy += fp[N + i];
       fz += fp[2 * N + i];
     }
-    B.fx[i] = fx; B.fy[i] = fy; B.fz[i] = fz;
   }
+  B.fx[i] = fx; B.fy[i] = fy; B.fz[i] = fz;
   if (VV) {
     const double m2 = 2.0 * B.m[B.typ[i]];
     const double vx = B.vx[i] + B.dt * (fx + B.fx0[i]) / m2;
@@ -262,7 +292,7 @@ __global__ __launch_bounds__(kIonThreads) void k_next_r(const BoxDesc* __restric
 // real-space pot and diagonal virial from the item partials, reciprocal
 // energy from the per-k terms, kinetic energy as mdc's calc_kin
 // (rv.c:68-86), pressure as calc_press (pt.c:8-21).
-// flags: bit0 count a completed step, bit1 include reciprocal/self terms.
+// flags bit0: count a completed step.
 // ----------------------------------------------------------------------
 __global__ __launch_bounds__(kScalThreads) void k_box_scalars(const BoxDesc* __restrict__ boxes,
                                                               const int* __restrict__ box_ids,

@@ -11,8 +11,10 @@ void launch_cell_build(const BoxDesc* boxes, const int* ids, int n, hipStream_t 
 void launch_phase_tables(bool fp32, const BoxDesc* boxes, const Item* items, int n, hipStream_t s);
 void launch_real64(bool analytic, const BoxDesc* boxes, const Item* items, int n, int mode,
                    hipStream_t s);
-void launch_sk_partial64(const BoxDesc* boxes, const Item* items, int n, hipStream_t s);
-void launch_kforce64(const BoxDesc* boxes, const Item* items, int n, hipStream_t s);
+// hs, hsw: LDS row strides (complex elements) of the X/Y and W tiles
+void launch_sk64(const BoxDesc* boxes, const Item* items, int n, int hs, int hsw, hipStream_t s);
+// hz: register row length of the back-projection (8, 16, 24, 32, 48 or 64)
+void launch_kforce64(int hz, const BoxDesc* boxes, const Item* items, int n, hipStream_t s);
 void launch_sk_finalize(const BoxDesc* boxes, const Item* items, int n, hipStream_t s);
 void launch_force_vv(bool vv, const BoxDesc* boxes, const Item* items, int n, hipStream_t s);
 void launch_clear_foc(const BoxDesc* boxes, const Item* items, int n, hipStream_t s);
@@ -24,7 +26,10 @@ void launch_moment_correction(const BoxDesc* boxes, const int* ids, int n, hipSt
 // k_mixed.hip (FP32 pair and k-space arithmetic)
 void launch_real32(bool analytic, const BoxDesc* boxes, const Item* items, int n, int mode,
                    hipStream_t s);
-void launch_sk_partial32(const BoxDesc* boxes, const Item* items, int n, hipStream_t s);
-void launch_kforce32(const BoxDesc* boxes, const Item* items, int n, hipStream_t s);
+void launch_sk32(const BoxDesc* boxes, const Item* items, int n, int hs, int hsw, hipStream_t s);
+void launch_kforce32(int hz, const BoxDesc* boxes, const Item* items, int n, hipStream_t s);
+
+// register row lengths the back-projection is instantiated for
+int kforce_hz_for(int hmax);
 
 }  // namespace mdg

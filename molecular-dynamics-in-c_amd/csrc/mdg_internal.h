@@ -3,11 +3,11 @@
 //
 // Data layout in HBM.  Every box owns one device allocation carved into
 // SoA arrays (positions, velocities, forces and previous forces as FP64
-// x[N], y[N], z[N]; species as int32), its k-space data (k columns,
-// per-k coefficients, per-ion phase tables, partial sums) and its cell
-// list.  A BoxDesc holds the device pointers and constants of one box; a
-// launch over many boxes reads one BoxDesc per work item, so boxes of any
-// size share a launch without repacking their arrays.
+// x[N], y[N], z[N]; species as int32), its k-space data (k columns and
+// segments, per-k coefficients, per-ion phase tables, partial sums) and
+// its cell list.  A BoxDesc holds the device pointers and constants of one
+// box; a launch over many boxes reads one BoxDesc per work item, so boxes
+// of any size share a launch without repacking their arrays.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -15,14 +15,16 @@
 
 namespace mdg {
 
+constexpr int kWave = 64;           // CDNA wavefront
 constexpr int kRealThreads = 64;    // one wave per real-space work item
-constexpr int kSkThreads = 256;     // structure-factor workgroup
-constexpr int kSkIpt = 4;           // ions per thread in the S(k) pass
-constexpr int kSkChunk = kSkThreads * kSkIpt;
-constexpr int kSkSub = 8;           // k vectors per register sub-block
-constexpr int kKfThreads = 256;     // back-projection workgroup (one ion per thread)
+constexpr int kSkThreads = 256;     // structure factor: one k segment per thread, shared ion tiles
+constexpr int kSkTile = 32;         // ions staged in LDS per structure-factor tile
+constexpr int kSegLen = 8;          // k vectors per segment (one column, consecutive hz)
+constexpr int kKfThreads = 64;      // back-projection: one wave, one ion per lane
+constexpr int kKfRows = 64;         // weight rows (4 columns each) staged in LDS at a time
 constexpr int kIonThreads = 256;    // per-ion streaming kernels
 constexpr int kCellThreads = 1024;  // cell-list build, one workgroup per box
+constexpr int kCellLds = 4096;      // cells whose counters live in LDS
 constexpr int kScalThreads = 256;   // per-box scalar reduction
 
 // error bits written by kernels into BoxDesc::err
@@ -38,13 +40,27 @@ struct KCol {
   int pad[3];
 };
 
+// Up to kSegLen consecutive k of one column: hz = hz0 .. hz0+n-1, k = k0 ..
+struct KSeg {
+  int hx, hy, hz0, n;
+  int k0, pad[3];
+};
+
+// Four consecutive k columns, back-projected together.  Their weights sit
+// in rows hz = lo .. lo+nrow-1 of a zero-padded [row][4 columns] layout
+// starting at row0; a column shorter than the quad's hz span has zeros.
+struct KQuad {
+  int hx[4], hy[4];
+  int lo, nrow, row0, ncol;
+};
+
 // Work item of a launch over many boxes.
 struct Item {
   int box;   // index into the BoxDesc array of the launch
   int a;     // first ion / cell / chunk / k, by kernel
-  int b;     // partial-sum slot or column range start
-  int c;     // column count
-  int d;     // group index (back-projection partial slot)
+  int b;     // partial-sum slot, first segment or first column
+  int c;     // split index / segment count / column count
+  int d;     // back-projection partial slot
 };
 
 struct BoxDesc {
@@ -71,6 +87,9 @@ struct BoxDesc {
   const double2 *table;
   const float2 *table32;
   int table_nb;       // bins per pair type (table_division + 1)
+  // ---- real-space split: partners divided over rsplit items per home unit ----
+  int rsplit;         // 1: forces written in place (mdc's order); >1: partials
+  double *rpf;        // [rsplit][3][N] partial real-space forces (rsplit > 1)
   // ---- cell list ----
   int nc[3], ncell;
   double cellscale[3];  // nc / L
@@ -81,16 +100,20 @@ struct BoxDesc {
   double *sx, *sy, *sz;
   int *styp;
   // ---- k space ----
-  int nk, ncol, hmax;
+  int nk, ncol, nseg, nquad, hmax;
   const KCol *cols;
+  const KSeg *segs;
+  const KQuad *quads;
+  const int *khz;     // per k: hz
+  const int *kslot;   // per k: slot (row * 4 + column in quad) of its weights
   const double *kce;  // per k: vp * exp(-pi^2 m^2/alpha^2) / m^2  (energy)
   const double *kcf;  // per k: v2 * exp(...) / m^2                  (force)
-  void *tabx, *taby;  // phase tables e^{2 pi i h x/L}, h = 0..hmax: [(hmax+1)][N] complex
-  void *tabw;         // e^{2 pi i z/L}: [N] complex
+  void *tabx, *taby, *tabw;  // e^{2 pi i h r_a/L_a}, h = 0..hmax: [(hmax+1)][N] complex
   double2 *skpart;    // [nchunk][nk] partial structure factors
-  int nchunk;
-  double *kab;        // [nk][2]: cf*2Ss, cf*2Sc  (FP64)
+  int nchunk, chunk;  // ion chunks of the structure-factor pass
+  double *kab;        // [slot][4]: cf*2Ss, cf*2Sc, hz*cf*2Ss, hz*cf*2Sc  (FP64)
   float *kab32;       // same, FP32 (mixed)
+  int nslot;          // 4 * total quad rows (padding slots stay zero)
   double *kpot;       // [nk] energy terms
   double *fpart;      // [ngroup][3][N] back-projection partial forces
   int ngroup;
