@@ -139,10 +139,14 @@ int make_plan(const BoxSpec& sp, int precision, BoxPlan* P, std::string* why) {
           P->kcf.push_back(v2 * (ex / m2));
           P->khz.push_back(hz);
         }
-        for (int q = 0; q < c.len; q += kSegLen) {
+        // segments start at multiples of kSegLen in hz (aligned LDS reads);
+        // slots below lo in the first one are computed and dropped
+        for (int h0 = 0; h0 <= hi; h0 += kSegLen) {
           KSeg g{};
-          g.hx = hx; g.hy = hy; g.hz0 = lo + q; g.n = std::min(kSegLen, c.len - q);
-          g.k0 = c.k0 + q;
+          g.hx = hx; g.hy = hy; g.hz0 = h0;
+          g.n = std::min(kSegLen, hi + 1 - h0);
+          g.qlo = std::max(0, lo - h0);
+          g.k0 = c.k0 + (h0 - lo);
           P->segs.push_back(g);
         }
         P->nk += c.len;
@@ -202,7 +206,7 @@ int make_plan(const BoxSpec& sp, int precision, BoxPlan* P, std::string* why) {
 
 // ------------------------------------------------------------------ layout
 struct Layout {
-  size_t ion[12], typ, rpf, cell_of, cnt, start, sorted, s[3], styp;
+  size_t ion[12], typ, rpf, cell_of, cnt, cwave, start, sorted, s[3], styp;
   size_t cols, segs, quads, khz, kslot, kce, kcf, kab, kab32, kpot, tx, ty, tw, skp, fp;
   size_t tab, tab32, rpart, rpairs, scal, err, total;
 };
@@ -228,6 +232,7 @@ static Layout layout(const BoxPlan& P, int precision) {
   if (!P.all_pairs) {
     L.cell_of = take(off, sizeof(int) * N);
     L.cnt = take(off, sizeof(int) * P.ncell);
+    L.cwave = take(off, sizeof(int) * (size_t)kCellWaves * P.ncell);
     L.start = take(off, sizeof(int) * (P.ncell + 1));
     L.sorted = take(off, sizeof(int) * N);
     for (int d = 0; d < 3; ++d) L.s[d] = take(off, sizeof(double) * N);
@@ -341,6 +346,7 @@ int box_create(const BoxSpec& spec, int precision, Box** out, std::string* why) 
     D.ncell = P.ncell;
     D.cell_of = (int*)(base + L.cell_of);
     D.cell_count = (int*)(base + L.cnt);
+    D.cell_wave = (int*)(base + L.cwave);
     D.cell_start = (int*)(base + L.start);
     D.sorted = (int*)(base + L.sorted);
     D.sx = (double*)(base + L.s[0]);
@@ -548,7 +554,7 @@ int BoxSet::set_boxes(const std::vector<Box*>& boxes) {
   }
   const int H = hmax + 1;
   hs_ = H | 1;                      // odd strides spread lanes over LDS banks
-  hsw_ = (H + kSegLen - 1) | 1;
+  hsw_ = (H + kSegLen - 1) / kSegLen * kSegLen;  // segment reads stay 16-byte aligned
   hz_ = kforce_hz_for(hmax);
   std::vector<Item> items;
   auto app = [&](std::vector<Item>& v, int& off, int& n) {

@@ -449,10 +449,11 @@ __global__ __launch_bounds__(kSkThreads) void k_sk_seg(const BoxDesc* __restrict
   const BoxDesc& B = boxes[it.box];
   const int N = B.N, H = B.hmax + 1, tid = threadIdx.x;
   const bool sv = tid < it.c;
-  KSeg sg = {0, 0, 0, 0, 0, {0, 0, 0}};
+  KSeg sg = {0, 0, 0, 0, 0, 0, {0, 0}};
   if (sv) {
     const cptr<KSeg> sp = as_const(B.segs) + it.b + tid;
     sg.hx = sp->hx; sg.hy = sp->hy; sg.hz0 = sp->hz0; sg.n = sp->n; sg.k0 = sp->k0;
+    sg.qlo = sp->qlo;
   }
   const int ax = sg.hx < 0 ? -sg.hx : sg.hx, ay = sg.hy < 0 ? -sg.hy : sg.hy;
   const bool cxn = sg.hx < 0, cyn = sg.hy < 0;
@@ -497,7 +498,20 @@ __global__ __launch_bounds__(kSkThreads) void k_sk_seg(const BoxDesc* __restrict
       for (int q = 0; q < kSegLen; ++q) t[q] = {(R)0, (R)0};
       for (int u = 0; u < cnt; ++u) {
         const Cx<R> p = cmul(conj_if(Xt[u * hs + ax], cxn), conj_if(Yt[u * hs + ay], cyn));
-        const Cx<R>* w = Wt + u * hsw + sg.hz0;
+        // hz0 and the row stride are multiples of kSegLen: aligned wide reads
+        Cx<R> w[kSegLen];
+        if constexpr (sizeof(R) == 4) {
+          const float4* w4 = reinterpret_cast<const float4*>(Wt + u * hsw + sg.hz0);
+#pragma unroll
+          for (int q = 0; q < kSegLen / 2; ++q) {
+            const float4 v = w4[q];
+            w[2 * q] = {v.x, v.y};
+            w[2 * q + 1] = {v.z, v.w};
+          }
+        } else {
+#pragma unroll
+          for (int q = 0; q < kSegLen; ++q) w[q] = Wt[u * hsw + sg.hz0 + q];
+        }
 #pragma unroll
         for (int q = 0; q < kSegLen; ++q) cmac(t[q], p, w[q]);
       }
@@ -513,7 +527,7 @@ __global__ __launch_bounds__(kSkThreads) void k_sk_seg(const BoxDesc* __restrict
     double2* dst = B.skpart + (size_t)it.a * B.nk + sg.k0;
 #pragma unroll
     for (int q = 0; q < kSegLen; ++q)
-      if (q < sg.n) dst[q] = make_double2(acc[q].x, acc[q].y);
+      if (q >= sg.qlo && q < sg.n) dst[q] = make_double2(acc[q].x, acc[q].y);
   }
 }
 

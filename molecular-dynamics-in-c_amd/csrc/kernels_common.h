@@ -54,9 +54,14 @@ __global__ __launch_bounds__(kCellThreads) void k_cell_build(const BoxDesc* __re
   int* cnt = lds ? s_cnt : B.cell_count;
   int* start = lds ? s_start : B.cell_start;
   int err = 0;
+  // wave w owns the contiguous ion range [w * per, (w + 1) * per)
+  const int wv = tid / kWave, lane = tid % kWave;
+  const int per = (N + kCellWaves * kWave - 1) / (kCellWaves * kWave) * kWave;
+  int* wcnt = B.cell_wave + (size_t)wv * ncell;
   for (int c = tid; c < ncell; c += kCellThreads) cnt[c] = 0;
+  for (int c = lane; c < ncell; c += kWave) atomicExch(&wcnt[c], 0);
   __syncthreads();
-  for (int i = tid; i < N; i += kCellThreads) {
+  for (int i = wv * per + lane; i < min(N, (wv + 1) * per); i += kWave) {
     const double x = B.rx[i], y = B.ry[i], z = B.rz[i];
     if (!(x >= 0.0 && x <= B.L[0] && y >= 0.0 && y <= B.L[1] && z >= 0.0 && z <= B.L[2]))
       err |= kErrState;
@@ -69,13 +74,17 @@ __global__ __launch_bounds__(kCellThreads) void k_cell_build(const BoxDesc* __re
     const int c = (cz * ncy + cy) * ncx + cx;
     B.cell_of[i] = c;
     atomicAdd(&cnt[c], 1);
+    atomicAdd(&wcnt[c], 1);
   }
   __syncthreads();
-  // exclusive scan of counts -> start; counts become fill cursors
-  const int per = (ncell + kCellThreads - 1) / kCellThreads;
-  const int c0 = min(tid * per, ncell), c1 = min(c0 + per, ncell);
+  // exclusive scan of counts -> start
+  const int cper = (ncell + kCellThreads - 1) / kCellThreads;
+  const int c0 = min(tid * cper, ncell), c1 = min(c0 + cper, ncell);
   int local = 0;
-  for (int c = c0; c < c1; ++c) local += cnt[c];
+  for (int c = c0; c < c1; ++c) {
+    if (!lds) cnt[c] = atomicAdd(&cnt[c], 0);  // global counters: read them where the atomics acted
+    local += cnt[c];
+  }
   s_scan[tid] = local;
   __syncthreads();
   for (int off = 1; off < kCellThreads; off <<= 1) {
@@ -88,16 +97,22 @@ __global__ __launch_bounds__(kCellThreads) void k_cell_build(const BoxDesc* __re
   for (int c = c0; c < c1; ++c) {
     start[c] = run;
     run += cnt[c];
-    cnt[c] = 0;
   }
   if (tid == kCellThreads - 1) start[ncell] = s_scan[tid];
   __syncthreads();
-  if (tid < kWave) {
-    const int lane = tid;
+  // per-wave fill cursors: cell start plus the counts of lower waves (the
+  // per-wave counters are only ever touched by atomics, which act in L2)
+  for (int c = tid; c < ncell; c += kCellThreads) {
+    int cur = start[c];
+    for (int w = 0; w < kCellWaves; ++w) cur += atomicExch(B.cell_wave + (size_t)w * ncell + c, cur);
+  }
+  __syncthreads();
+  {
     const unsigned long long below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-    for (int base = 0; base < N; base += kWave) {
+    const int hi = min(N, (wv + 1) * per);
+    for (int base = wv * per; base < hi; base += kWave) {
       const int i = base + lane;
-      const bool valid = i < N;
+      const bool valid = i < hi;
       const int c = valid ? B.cell_of[i] : -1;
       unsigned long long todo = __ballot(valid);
       int rank = 0, first = 0, cur = 0;
@@ -113,11 +128,10 @@ __global__ __launch_bounds__(kCellThreads) void k_cell_build(const BoxDesc* __re
         }
         todo &= ~m;
       }
-      if (valid) cur = cnt[c];            // every lane reads before any lane writes
-      __builtin_amdgcn_wave_barrier();
-      if (valid && lane == first) cnt[c] = cur + __popcll(mine);
-      if (valid) B.sorted[start[c] + cur + rank] = i;
-      __builtin_amdgcn_wave_barrier();
+      // the first lane of each cell group claims the group's slots
+      if (valid && lane == first) cur = atomicAdd(&wcnt[c], __popcll(mine));
+      cur = __shfl(cur, first, kWave);
+      if (valid) B.sorted[cur + rank] = i;
     }
   }
   __syncthreads();
@@ -167,24 +181,29 @@ __global__ __launch_bounds__(kIonThreads) void k_sk_finalize(const BoxDesc* __re
   const Item it = items[blockIdx.x];
   const BoxDesc& B = boxes[it.box];
   const int k = it.a + threadIdx.x;
-  if (k >= B.nk) return;
-  double sc = 0.0, ss = 0.0;
-  for (int ch = 0; ch < B.nchunk; ++ch) {
-    const double2 p = B.skpart[(size_t)ch * B.nk + k];
-    sc += p.x;
-    ss += p.y;
+  double e[1] = {0.0};
+  if (k < B.nk) {
+    double sc = 0.0, ss = 0.0;
+    for (int ch = 0; ch < B.nchunk; ++ch) {
+      const double2 p = B.skpart[(size_t)ch * B.nk + k];
+      sc += p.x;
+      ss += p.y;
+    }
+    sc *= 2.0;
+    ss *= 2.0;
+    e[0] = B.kce[k] * (sc * sc + ss * ss);
+    const double a = B.kcf[k] * ss, b = B.kcf[k] * sc, hz = (double)B.khz[k];
+    const size_t slot = (size_t)B.kslot[k];
+    double* t = B.kab + 4 * slot;
+    t[0] = a; t[1] = b; t[2] = hz * a; t[3] = hz * b;
+    if (B.kab32) {
+      float* u = B.kab32 + 4 * slot;
+      u[0] = (float)a; u[1] = (float)b; u[2] = (float)(hz * a); u[3] = (float)(hz * b);
+    }
   }
-  sc *= 2.0;
-  ss *= 2.0;
-  B.kpot[k] = B.kce[k] * (sc * sc + ss * ss);
-  const double a = B.kcf[k] * ss, b = B.kcf[k] * sc, hz = (double)B.khz[k];
-  const size_t slot = (size_t)B.kslot[k];
-  double* t = B.kab + 4 * slot;
-  t[0] = a; t[1] = b; t[2] = hz * a; t[3] = hz * b;
-  if (B.kab32) {
-    float* u = B.kab32 + 4 * slot;
-    u[0] = (float)a; u[1] = (float)b; u[2] = (float)(hz * a); u[3] = (float)(hz * b);
-  }
+  // energy of this block of k (fixed order); box_scalars sums the blocks
+  block_sum<kIonThreads, 1>(e);
+  if (threadIdx.x == 0) B.kpot[it.a / kIonThreads] = e[0];
 }
 
 // ----------------------------------------------------------------------
@@ -308,7 +327,8 @@ __global__ __launch_bounds__(kScalThreads) void k_box_scalars(const BoxDesc* __r
     v[8] += (double)B.rpairs[s];
   }
   if (B.ewald)
-    for (int k = tid; k < B.nk; k += kScalThreads) v[4] += B.kpot[k];
+    for (int k = tid; k < (B.nk + kIonThreads - 1) / kIonThreads; k += kScalThreads)
+      v[4] += B.kpot[k];
   for (int i = tid; i < B.N; i += kScalThreads) {
     const double m = B.m[B.typ[i]];
     v[5] += m * B.vx[i] * B.vx[i];

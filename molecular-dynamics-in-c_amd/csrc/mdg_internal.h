@@ -25,6 +25,7 @@ constexpr int kKfRows = 64;         // weight rows (4 columns each) staged in LD
 constexpr int kIonThreads = 256;    // per-ion streaming kernels
 constexpr int kCellThreads = 1024;  // cell-list build, one workgroup per box
 constexpr int kCellLds = 4096;      // cells whose counters live in LDS
+constexpr int kCellWaves = kCellThreads / kWave;
 constexpr int kScalThreads = 256;   // per-box scalar reduction
 
 // error bits written by kernels into BoxDesc::err
@@ -40,10 +41,12 @@ struct KCol {
   int pad[3];
 };
 
-// Up to kSegLen consecutive k of one column: hz = hz0 .. hz0+n-1, k = k0 ..
+// kSegLen consecutive hz of one column starting at hz0 (a multiple of
+// kSegLen); slots qlo .. n-1 are k vectors k0 + q of the column (k0 may
+// precede the column's first k when qlo > 0).
 struct KSeg {
   int hx, hy, hz0, n;
-  int k0, pad[3];
+  int k0, qlo, pad[2];
 };
 
 // Four consecutive k columns, back-projected together.  Their weights sit
@@ -95,6 +98,7 @@ struct BoxDesc {
   double cellscale[3];  // nc / L
   int *cell_of;       // [N]
   int *cell_count;    // [ncell]
+  int *cell_wave;     // [kCellWaves][ncell] per-wave counts, then per-wave fill cursors
   int *cell_start;    // [ncell + 1]
   int *sorted;        // [N] ion index in cell order (ascending within a cell)
   double *sx, *sy, *sz;
