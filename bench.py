@@ -132,6 +132,43 @@ def cpu_baseline(boxes, states, budget_s):
                         f"(same cost); {secs:.1f} s of CPU time"))
 
 
+def serving(njobs, steps_scale, budget_gb):
+    """The serving workload: a seeded Poisson stream (50 jobs/s) of NaCl
+    boxes, 216..32768 ions log-uniform, 100..5000 steps scaled by
+    steps_scale, 300..2000 K, mixed precision, admitted against an HBM
+    budget with continuous batching -- and the same stream served one job
+    at a time."""
+    jobs = W.stream(seed=11, njobs=njobs, rate_hz=50.0, n_min=216, n_max=32768,
+                    steps_min=100, steps_max=5000, steps_scale=steps_scale)
+    out = {}
+    res = {}
+    for mode in ("batched", "one_at_a_time"):
+        c = mdgpu.ServeConfig()
+        c.hbm_budget = int(budget_gb * (1 << 30))
+        c.max_slots = 64
+        c.precision = mdgpu.MIXED
+        c.real_mode = mdgpu.ANALYTIC
+        c.eps, c.rc_max, c.dt_fs = 1e-6, 10.0, 2.0
+        c.one_at_a_time = 1 if mode == "one_at_a_time" else 0
+        c.realtime = 1
+        c.steps_per_tick = 8
+        r, st = mdgpu.serve(jobs, c)
+        res[mode] = r
+        out[mode] = {"ion_steps_per_s": st.ion_steps_per_s, "p50_latency_s": st.p50_latency_s,
+                     "p99_latency_s": st.p99_latency_s, "mean_latency_s": st.mean_latency_s,
+                     "wall_s": st.wall_s, "max_resident": st.max_resident, "ticks": st.ticks,
+                     "rejected": st.rejected, "peak_gb": st.peak_bytes / (1 << 30)}
+        log(f"  serving {mode}: {st.ion_steps_per_s:.4e} ion-steps/s, p50 {st.p50_latency_s:.3f} s,"
+            f" p99 {st.p99_latency_s:.3f} s")
+    same = all(a.final_checksum == b.final_checksum and a.status == b.status == mdgpu.OK
+               for a, b in zip(res["batched"], res["one_at_a_time"]))
+    out.update({"jobs": njobs, "arrival_rate_hz": 50.0, "ions": "216-32768 log-uniform",
+                "steps": f"100-5000 x {steps_scale}", "hbm_budget_gb": budget_gb,
+                "results_identical_alone_and_in_mix": same,
+                "total_ion_steps": float(sum(8 * j.nx * j.ny * j.nz * j.steps for j in jobs))})
+    return out
+
+
 def copy_probe():
     """A plain copy kernel of known size, for scaling rocprofv3's FETCH_SIZE /
     WRITE_SIZE counters (1 GiB read, 1 GiB written)."""
@@ -154,6 +191,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--copy-probe", action="store_true")
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-jobs", type=int, default=48)
+    ap.add_argument("--serve-steps-scale", type=float, default=0.05)
+    ap.add_argument("--serve-budget-gb", type=float, default=16.0)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this engine runs on one GPU; boxes would shard across GPUs with no "
@@ -217,6 +258,11 @@ def main():
                  "real_space": "mdc table, FP64", "seconds": psecs}
         log(f"  {rate:.4e} ion-steps/s")
 
+    serve = None
+    if not args.no_serve:
+        log("serving stream")
+        serve = serving(args.serve_jobs, args.serve_steps_scale, args.serve_budget_gb)
+
     cpu = None
     if not args.no_cpu:
         log("CPU baseline")
@@ -239,7 +285,7 @@ def main():
         "nve_drift_per_ion_per_1e4_steps": {"median_abs": float(np.median(np.abs(drift))),
                                             "max_abs": float(np.abs(drift).max()),
                                             "unit": "1e-18 J"},
-        "serving": None,
+        "serving": serve,
         "roofline": roof,
         "cpu_baseline": cpu,
     }

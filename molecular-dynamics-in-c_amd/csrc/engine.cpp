@@ -669,7 +669,15 @@ int BoxSet::step(int nsteps, hipStream_t s) {
   if (nsteps <= 0 || boxes_.empty()) return MDG_OK;
   int rc = sync_desc(s);
   if (rc != MDG_OK) return rc;
-  const int G = std::min(nsteps, 8);
+  // Whole groups of 8 steps replay one captured graph; a shorter remainder
+  // (or a call of fewer than 8 steps) is launched directly, so callers that
+  // step by varying amounts do not re-capture.
+  const int G = 8;
+  if (nsteps < G) {
+    for (int k = 0; k < nsteps; ++k) enqueue_step(s, true);
+    MDG_HIP(hipGetLastError());
+    return MDG_OK;
+  }
   if (!graph_exec_ || graph_steps_ != G) {
     invalidate_graph();
     MDG_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
