@@ -43,6 +43,10 @@ extern "C" {
 #define MDG_ALL_PAIRS    0
 #define MDG_CELL_LIST    1
 
+/* force terms (mdg_batch_force_terms) */
+#define MDG_TERM_REAL  1
+#define MDG_TERM_RECIP 2
+
 /* Born-Mayer-Huggins salt: pair type 0 = (0,0), 1 = (0,1), 2 = (1,1). */
 typedef struct {
   double Z[2];       /* effective charges (sqrt(kk) = 1 e)          */
@@ -118,6 +122,11 @@ int  mdg_batch_get_state(mdg_batch *b, int box, double *r, double *v, double *f,
 /* mdc's calc_foc for every box: f0 <- f, f <- real + reciprocal forces,
  * pot/vir/self as mdc writes them (no integration). */
 int  mdg_batch_forces(mdg_batch *b);
+/* One force term alone, as mdc's real_space (mdc/program/real.c:15-84) and
+ * reciprocal_space (mdc/program/rcprcl.c:11-97) each add theirs: f0 <- f,
+ * f <- the selected terms.  terms: MDG_TERM_REAL | MDG_TERM_RECIP.  The
+ * energies are left as they were. */
+int  mdg_batch_force_terms(mdg_batch *b, int terms);
 /* nsteps velocity-Verlet steps (next_rv_verlet + calc_kin + calc_press) on
  * every box; the step sequence is captured once in a hipGraph and replayed */
 int  mdg_batch_step(mdg_batch *b, int nsteps);

@@ -197,6 +197,16 @@ int mdg_batch_forces(mdg_batch* b) {
   return MDG_OK;
 }
 
+int mdg_batch_force_terms(mdg_batch* b, int terms) {
+  if (!b || terms < 1 || terms > 3) return MDG_E_ARG;
+  int rc = b->set->sync_desc(b->stream);
+  if (rc != MDG_OK) return rc;
+  b->set->enqueue_force_terms(b->stream, terms);
+  MDG_HIP(hipGetLastError());
+  MDG_HIP(hipStreamSynchronize(b->stream));
+  return MDG_OK;
+}
+
 int mdg_batch_step(mdg_batch* b, int nsteps) {
   if (!b || nsteps < 0) return MDG_E_ARG;
   return b->set->step(nsteps, b->stream);

@@ -645,6 +645,15 @@ void BoxSet::enqueue_forces_parts(hipStream_t s, bool real, int real_write_mode,
   }
 }
 
+void BoxSet::enqueue_force_terms(hipStream_t s, int terms) {
+  if (terms & 1) {
+    launch_cell_build(d_desc_, d_ids_ + off_cellid_, n_cellid_, s);
+    launch_real(s, 0);
+  }
+  if ((terms & 2) && n_sk_ > 0) launch_recip(s);
+  launch_force_vv(false, d_desc_, d_items_ + off_ion_, n_ion_, s, terms);
+}
+
 void BoxSet::enqueue_step(hipStream_t s, bool vv) {
   launch_cell_build(d_desc_, d_ids_ + off_cellid_, n_cellid_, s);
   launch_real(s, 0);

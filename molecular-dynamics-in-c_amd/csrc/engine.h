@@ -96,6 +96,9 @@ class BoxSet {
   // next_r and count the step.
   void enqueue_step(hipStream_t s, bool vv);
   void enqueue_forces_parts(hipStream_t s, bool real, int real_write_mode, bool recip);
+  // f0 <- f, f <- the selected terms (bit 0 real space, bit 1 reciprocal
+  // space); energies are not updated.
+  void enqueue_force_terms(hipStream_t s, int terms);
   void enqueue_scalars(hipStream_t s, bool count_step);
   void enqueue_clear_foc(hipStream_t s);
   void enqueue_next_v(hipStream_t s);

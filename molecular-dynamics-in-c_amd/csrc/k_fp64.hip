@@ -66,10 +66,11 @@ void launch_sk_finalize(const BoxDesc* boxes, const Item* items, int n, hipStrea
   if (n > 0) hipLaunchKernelGGL(k_sk_finalize, MDG_GRID(n, kIonThreads), boxes, items);
 }
 
-void launch_force_vv(bool vv, const BoxDesc* boxes, const Item* items, int n, hipStream_t s) {
+void launch_force_vv(bool vv, const BoxDesc* boxes, const Item* items, int n, hipStream_t s,
+                     int terms) {
   if (n <= 0) return;
-  if (vv) hipLaunchKernelGGL(k_force_vv<1>, MDG_GRID(n, kIonThreads), boxes, items);
-  else hipLaunchKernelGGL(k_force_vv<0>, MDG_GRID(n, kIonThreads), boxes, items);
+  if (vv) hipLaunchKernelGGL(k_force_vv<1>, MDG_GRID(n, kIonThreads), boxes, items, terms);
+  else hipLaunchKernelGGL(k_force_vv<0>, MDG_GRID(n, kIonThreads), boxes, items, terms);
 }
 
 void launch_clear_foc(const BoxDesc* boxes, const Item* items, int n, hipStream_t s) {

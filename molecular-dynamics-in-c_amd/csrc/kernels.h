@@ -149,7 +149,7 @@ struct HomeCtx {
 struct BoxCtx {
   double hx, hy, hz, Lx, Ly, Lz, rc2;
   int nb;
-  float rc2f, alpha, two_a3, inv_rho;
+  float rc2f, alpha, two_a3, inv_rho, inv_rho_l2e, inv_rho_l2e_lo;
 };
 
 template <int KIND>
@@ -187,16 +187,28 @@ __device__ __forceinline__ void fold(const BoxCtx& c, double& dx, double& dy, do
 }
 
 // FP32 analytic pair function: -U'(r)/r and U(r) of erfc Coulomb + BMH.
-__device__ __forceinline__ void pair32(const BoxCtx& c, const Row32& w, float r2, float& p,
+// dr is the FP64 squared distance.  r is rounded to FP32 but its residual
+// rlo is kept, and the Born-Mayer exponent (sigma - r) / rho is formed as
+// an unevaluated FP32 pair (x, xlo): the exponential multiplies r's
+// rounding error by r / rho (about 9 for alkali halides), which would
+// otherwise dominate the pair force error near contact.
+__device__ __forceinline__ void pair32(const BoxCtx& c, const Row32& w, double dr, float& p,
                                        float& f) {
   const float kLog2e = 1.4426950408889634f;
+  const float kLn2 = 0.6931471805599453f;
+  const float r2 = (float)dr;
+  const float r2lo = (float)(dr - (double)r2);
   const float rinv = __frsqrt_rn(r2);
   const float r = r2 * rinv;
+  const float rlo = 0.5f * (fmaf(-r, r, r2) + r2lo) * rinv;  // r + rlo = sqrt(dr)
   const float rinv2 = rinv * rinv;
   const float z = c.alpha * r;
   const float erfc_r = erfc_cheb(z) * rinv;
   const float g = exp2f(-z * z * kLog2e);
-  const float eb = w.cb * exp2f((w.sig - r) * c.inv_rho * kLog2e);
+  const float a = (w.sig - r) - rlo;                 // sigma - r is exact near contact
+  const float x = a * c.inv_rho_l2e;
+  const float xlo = fmaf(a, c.inv_rho_l2e, -x) + a * c.inv_rho_l2e_lo;
+  const float eb = w.cb * exp2f(x) * fmaf(kLn2, xlo, 1.0f);
   const float r6 = rinv2 * rinv2 * rinv2;
   const float r8 = r6 * rinv2;
   p = w.zz * erfc_r + eb - w.c6 * r6 - w.d8 * r8;
@@ -302,10 +314,11 @@ __device__ __forceinline__ void real_tile(const BoxDesc& B, const BoxCtx& c, con
       fold(c, dx, dy, dz);
       float p, f, fdx, fdy, fdz;
       if constexpr (KIND == kRealAnalytic32) {
+        // the FP64 cutoff test accepts exactly the FP64 path's pairs
+        const double dr = fma(dx, dx, fma(dy, dy, dz * dz));
+        if (!h.valid || jid[u] == h.i || !(dr <= c.rc2)) continue;
         fdx = (float)dx; fdy = (float)dy; fdz = (float)dz;
-        const float r2 = fmaf(fdx, fdx, fmaf(fdy, fdy, fdz * fdz));
-        if (!h.valid || jid[u] == h.i || !(r2 <= c.rc2f)) continue;
-        pair32(c, jt[u] == 0 ? h.row[0] : h.row[1], r2, p, f);
+        pair32(c, jt[u] == 0 ? h.row[0] : h.row[1], dr, p, f);
       } else {  // kRealTable32: FP64 bin as mdc, FP32 table values
         const double dr = dx * dx + dy * dy + dz * dz;
         if (!h.valid || jid[u] == h.i || !(dr <= c.rc2)) continue;
@@ -361,6 +374,9 @@ __global__ __launch_bounds__(kRealThreads) void k_real(const BoxDesc* __restrict
   c.nb = B.table_nb;
   c.rc2f = (float)B.rc2; c.alpha = (float)B.alpha; c.two_a3 = (float)(2.0 * B.a3);
   c.inv_rho = (float)(1.0 / B.rho);
+  const double irl = 1.4426950408889634 / B.rho;  // log2(e) / rho, split in two floats
+  c.inv_rho_l2e = (float)irl;
+  c.inv_rho_l2e_lo = (float)(irl - (double)c.inv_rho_l2e);
   RealAcc tot;  // pot / virial / pair count of the item
   if (B.all_pairs) {
     const int i = it.a + lane;

@@ -212,16 +212,22 @@ __global__ __launch_bounds__(kIonThreads) void k_sk_finalize(const BoxDesc* __re
 // order).  With VV also mdc's next_v then next_r (rv.c:33-66) in mdc's
 // operation order, with the one-fold strict-comparison wrap.
 // ----------------------------------------------------------------------
+// terms: bit 0 = the real-space kernel ran (its split partials or f hold
+// the real-space force), bit 1 = add the reciprocal-space partials.
 template <int VV>
 __global__ __launch_bounds__(kIonThreads) void k_force_vv(const BoxDesc* __restrict__ boxes,
-                                                          const Item* __restrict__ items) {
+                                                          const Item* __restrict__ items,
+                                                          int terms) {
   const Item it = items[blockIdx.x];
   const BoxDesc& B = boxes[it.box];
   const int N = B.N;
   const int i = it.a + threadIdx.x;
   if (i >= N) return;
   double fx, fy, fz;
-  if (B.rsplit > 1) {
+  if (!(terms & 1)) {
+    fx = 0.0; fy = 0.0; fz = 0.0;
+    B.fx0[i] = B.fx[i]; B.fy0[i] = B.fy[i]; B.fz0[i] = B.fz[i];
+  } else if (B.rsplit > 1) {
     fx = 0.0; fy = 0.0; fz = 0.0;
     for (int s = 0; s < B.rsplit; ++s) {
       const double* p = B.rpf + (size_t)s * 3 * N;
@@ -233,7 +239,7 @@ __global__ __launch_bounds__(kIonThreads) void k_force_vv(const BoxDesc* __restr
   } else {
     fx = B.fx[i]; fy = B.fy[i]; fz = B.fz[i];
   }
-  if (B.ewald) {
+  if (B.ewald && (terms & 2)) {
     for (int g = 0; g < B.ngroup; ++g) {
       const double* fp = B.fpart + (size_t)g * 3 * N;
       fx += fp[i];

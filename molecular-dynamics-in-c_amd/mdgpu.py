@@ -16,6 +16,7 @@ OK, E_NODEVICE, E_HIP, E_ARG, E_NOMEM, E_BUDGET, E_STATE, E_RANGE, E_TIMEOUT = r
 FP64, MIXED = 0, 1
 TABLE, ANALYTIC = 0, 1
 CELLS_AUTO, ALL_PAIRS, CELL_LIST = -1, 0, 1
+TERM_REAL, TERM_RECIP = 1, 2
 
 SECD_2_FS = 4.074972637944947
 KB = 1.380658e-5
@@ -160,6 +161,7 @@ def lib():
             "mdg_batch_set_state": ([vp, C.c_int, _dp, _dp, C.POINTER(C.c_int)], C.c_int),
             "mdg_batch_get_state": ([vp, C.c_int, _dp, _dp, _dp, _dp], C.c_int),
             "mdg_batch_forces": ([vp], C.c_int),
+            "mdg_batch_force_terms": ([vp, C.c_int], C.c_int),
             "mdg_batch_step": ([vp, C.c_int], C.c_int),
             "mdg_batch_scalars": ([vp, C.c_int, C.POINTER(Scalars)], C.c_int),
             "mdg_batch_sync": ([vp], C.c_int),
@@ -301,6 +303,10 @@ class Batch:
 
     def forces(self):
         check(lib().mdg_batch_forces(self.h), "forces")
+
+    def force_terms(self, terms):
+        """f0 <- f, f <- the selected terms (TERM_REAL | TERM_RECIP)."""
+        check(lib().mdg_batch_force_terms(self.h, terms), "force_terms")
 
     def step(self, n):
         check(lib().mdg_batch_step(self.h, n), "step")

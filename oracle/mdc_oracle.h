@@ -32,8 +32,28 @@
  * also kept separately so that a consistent (reciprocal/4) total can be
  * reported next to the reference's own `pot`.
  *
- * Parity status per function is recorded in oracle/README (pinned by the
- * compiled reference in oracle/_ref, closed forms, or invariants).
+ * Parity status per function (tests/test_oracle.py, tests/test_abi.py):
+ *   erfcc            compiled mdc (bit-equal), math.erfc (|err| < 1e-14)
+ *   unit_converter   compiled mdc (tables, trajectory and scalars bit-equal)
+ *   potential_nacl   compiled mdc tables bit-equal; closed-form table entries
+ *   potential_kcl    PARITY UNPINNED: mdc has no KCl; the constants are the
+ *                    published Tosi-Fumi (1964) KCl set, checked only for
+ *                    consistency of the host copy with this one
+ *   set_loc          compiled mdc (positions and species bit-equal)
+ *   mk_table         compiled mdc (bit-equal); closed form; -U'/r by
+ *                    centred differences
+ *   clear_foc, calc_foc, next_v/next_r, calc_kin, calc_press
+ *                    compiled mdc 60-step trajectory with every scalar
+ *                    bit-equal; golden fixture tests/golden/mdc_nacl216.npz
+ *   real_space       compiled mdc forces/pot/virial bit-equal; third law;
+ *                    zero force on the perfect lattice; golden fixture
+ *   real_space_analytic (pr-mdc form, not compiled here): -dE/dx = F by
+ *                    finite differences; Madelung constant of rock salt
+ *   reciprocal_space compiled mdc bit-equal; S(k) = 0 on the lattice;
+ *                    -d(E/4)/dx = F; Madelung constant
+ *   reciprocal_space3 compiled mdc (pot bit-equal)
+ *   moment_correction compiled mdc (bit-equal); zero total momentum
+ *   nve              compiled mdc trajectory; golden fixture
  */
 #ifndef MDC_ORACLE_H
 #define MDC_ORACLE_H

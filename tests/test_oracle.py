@@ -214,6 +214,33 @@ def test_oracle_matches_compiled_reference():
         assert mdref.ref().ref_erfcc(x) == mdref.lib().orc_erfcc(x)
 
 
+def test_moment_correction_removes_momentum():
+    o = cfg1_box()
+    m = o.masses()
+    v = mdref.maxwell_velocities(m, 1000.0, 21) + np.array([[0.3], [-0.2], [0.1]])
+    o.set_state(v=v)
+    o.call("moment_correction")
+    p = (o.v * m).sum(axis=1)
+    assert np.abs(p).max() < 1e-13 * (np.abs(v) * m).sum()
+    # uniform shift: each velocity loses the mass-weighted mean per ion,
+    # (sum m v / N) / m_i (mdc/program/rv.c:88-116)
+    shift = ((v * m).sum(axis=1, keepdims=True) / o.N) / m
+    assert np.allclose(o.v, v - shift, rtol=0, atol=1e-15 * np.abs(v).max())
+
+
+@pytest.mark.skipif(not mdref.ref_available(), reason="oracle/_ref not built (no reference checkout)")
+def test_moment_correction_matches_compiled_reference():
+    L = 3 * A
+    o = cfg1_box()
+    R = mdref.Reference(3, 3, 3, A, 5.6 / L, 27, L / 2, 2.0)
+    v = mdref.maxwell_velocities(o.masses(), 1500.0, 22) + 0.05
+    o.set_state(v=v)
+    R.set_state(v=v)
+    o.call("moment_correction")
+    R.call("moment_correction")
+    assert np.array_equal(R.state()["v"], o.v)
+
+
 GOLD = os.path.join(os.path.dirname(__file__), "golden", "mdc_nacl216.npz")
 
 
