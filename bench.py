@@ -71,12 +71,17 @@ def force_error(boxes, states_now):
     return mx, rms
 
 
-def plain_one_at_a_time(boxes, states, steps, warmup):
-    """The plain FP64 path (mdc's tabulated real space) run box by box."""
-    ion_steps, secs = 0.0, 0.0
-    for k, p in enumerate(boxes):
+def plain_one_at_a_time(boxes, states, steps, warmup, tune=True):
+    """The plain FP64 path (mdc's tabulated real space) run box by box, its
+    tilings tuned for a lone box on the chip."""
+    plain = []
+    for p in boxes:
         q = mdgpu.BoxParams.from_buffer_copy(p)
         q.real_mode = mdgpu.TABLE
+        plain.append(q)
+    rep = mdgpu.tune(plain, mdgpu.FP64, context_ions=1) if tune else None
+    ion_steps, secs = 0.0, 0.0
+    for k, q in enumerate(plain):
         r, v, sp = states[k]
         b = mdgpu.Batch([q], mdgpu.FP64)
         b.set_state(0, r=r, v=v, species=sp)
@@ -88,7 +93,7 @@ def plain_one_at_a_time(boxes, states, steps, warmup):
         secs += time.perf_counter() - t0
         ion_steps += b.n_ions(0) * steps
         b.close()
-    return ion_steps / secs, secs
+    return ion_steps / secs, secs, rep
 
 
 def cpu_baseline(boxes, states, budget_s):
@@ -132,7 +137,7 @@ def cpu_baseline(boxes, states, budget_s):
                         f"(same cost); {secs:.1f} s of CPU time"))
 
 
-def serving(njobs, steps_scale, budget_gb):
+def serving(njobs, steps_scale, budget_gb, tune=True):
     """The serving workload: a seeded Poisson stream (50 jobs/s) of NaCl
     boxes, 216..32768 ions log-uniform, 100..5000 steps scaled by
     steps_scale, 300..2000 K, mixed precision, admitted against an HBM
@@ -142,7 +147,8 @@ def serving(njobs, steps_scale, budget_gb):
                     steps_min=100, steps_max=5000, steps_scale=steps_scale)
     out = {}
     res = {}
-    for mode in ("batched", "one_at_a_time"):
+
+    def config(mode):
         c = mdgpu.ServeConfig()
         c.hbm_budget = int(budget_gb * (1 << 30))
         c.max_slots = 64
@@ -152,6 +158,14 @@ def serving(njobs, steps_scale, budget_gb):
         c.one_at_a_time = 1 if mode == "one_at_a_time" else 0
         c.realtime = 1
         c.steps_per_tick = 8
+        return c
+
+    if tune:
+        prm = [mdgpu.serve_job_params(j, config("batched")) for j in jobs]
+        out["autotune"] = mdgpu.tune(prm, mdgpu.MIXED)
+        log(f"  tuned {out['autotune']['classes']} job classes in {out['autotune']['seconds']:.1f} s")
+    for mode in ("batched", "one_at_a_time"):
+        c = config(mode)
         r, st = mdgpu.serve(jobs, c)
         res[mode] = r
         out[mode] = {"ion_steps_per_s": st.ion_steps_per_s, "p50_latency_s": st.p50_latency_s,
@@ -192,6 +206,7 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--copy-probe", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-tune", action="store_true", help="built-in tiling rule only")
     ap.add_argument("--serve-jobs", type=int, default=48)
     ap.add_argument("--serve-steps-scale", type=float, default=0.05)
     ap.add_argument("--serve-budget-gb", type=float, default=16.0)
@@ -213,6 +228,15 @@ def main():
 
     # ---- batched mixed precision: the headline ----
     log(f"batched mixed: {len(boxes)} boxes, {total_ions} ions")
+    tune_rep = None
+    if not args.no_tune:
+        tune_rep = mdgpu.tune(boxes, mdgpu.MIXED)
+        tl = [mdgpu.box_tiling(p, mdgpu.MIXED) for p in boxes]
+        tune_rep["boxes_with_cells"] = sum(t["cells"] for t in tl)
+        tune_rep["rsplit_hist"] = {str(r): sum(t["rsplit"] == r for t in tl)
+                                   for r in sorted({t["rsplit"] for t in tl})}
+        log(f"  tuned {tune_rep['classes']} classes ({tune_rep['runs']} runs) "
+            f"in {tune_rep['seconds']:.1f} s")
     b = make_batch(boxes, states, mdgpu.MIXED)
     b.step(args.warmup)
     b.sync()
@@ -253,15 +277,18 @@ def main():
     plain = None
     if not args.no_plain:
         log("plain FP64 one box at a time")
-        rate, psecs = plain_one_at_a_time(boxes, states, args.plain_steps, 3)
+        rate, psecs, prep = plain_one_at_a_time(boxes, states, args.plain_steps, 3,
+                                                tune=not args.no_tune)
         plain = {"value": rate, "unit": "ion-steps/s", "steps_per_box": args.plain_steps,
-                 "real_space": "mdc table, FP64", "seconds": psecs}
+                 "real_space": "mdc table, FP64", "seconds": psecs,
+                 "autotune_lone_box": prep}
         log(f"  {rate:.4e} ion-steps/s")
 
     serve = None
     if not args.no_serve:
         log("serving stream")
-        serve = serving(args.serve_jobs, args.serve_steps_scale, args.serve_budget_gb)
+        serve = serving(args.serve_jobs, args.serve_steps_scale, args.serve_budget_gb,
+                        tune=not args.no_tune)
 
     cpu = None
     if not args.no_cpu:
@@ -285,6 +312,7 @@ def main():
         "nve_drift_per_ion_per_1e4_steps": {"median_abs": float(np.median(np.abs(drift))),
                                             "max_abs": float(np.abs(drift).max()),
                                             "unit": "1e-18 J"},
+        "autotune": tune_rep,
         "serving": serve,
         "roofline": roof,
         "cpu_baseline": cpu,

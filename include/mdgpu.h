@@ -184,6 +184,46 @@ typedef struct {
 
 int  mdg_serve(const mdg_job *jobs, int njobs, const mdg_serve_config *cfg,
                mdg_job_result *results, mdg_serve_stats *stats);
+/* the box parameters mdg_serve gives a job (NaCl, a = 5.64 A, Ewald split
+ * for cfg->eps with rc = min(cfg->rc_max, L_min / 2)), e.g. to tune them */
+int  mdg_serve_job_params(const mdg_job *job, const mdg_serve_config *cfg,
+                          mdg_box_params *out);
+
+/* ------------------------------------------------------------- autotune */
+/* No counterpart in mdc (one box, one serial loop).  Tile and cell sizes
+ * of a box: real-space cell list or all pairs, cell width, partner splits;
+ * structure-factor ion chunks; back-projection k groups. */
+typedef struct {
+  int    cells;      /* 1: cell list, 0: all pairs                        */
+  double cell_w;     /* cell width / rc (>= 1)                             */
+  int    rsplit;     /* real-space partner splits per home unit            */
+  int    nchunk;     /* structure-factor ion chunks                        */
+  int    ngroup;     /* back-projection k groups                           */
+  int    tuned;      /* 1: from the tuning table, 0: built-in rule         */
+} mdg_tiling;
+
+typedef struct {
+  int    classes;    /* box classes measured                               */
+  int    runs;       /* replicated-batch runs timed                        */
+  double seconds;    /* wall time of the tuning                            */
+} mdg_tune_report;
+
+/* Measures candidate tilings for every box class among params (classes:
+ * precision, real-space mode, cell list possible, sqrt(2)-wide buckets of
+ * N and of the k-vector count).  Each class is timed in a batch of copies
+ * of its first box holding about context_ions ions (0: 131072), with
+ * per-kernel HIP events; the real-space, structure-factor and
+ * back-projection choices are made separately.  Results go to the
+ * process-wide tuning table, which boxes created afterwards consult.  A
+ * box's arithmetic depends on its tiling, so a fixed table keeps results
+ * identical alone and in any mix; save/load fix it across processes. */
+int  mdg_tune(int nbox, const mdg_box_params *params, int precision,
+              long long context_ions, mdg_tune_report *rep);
+int  mdg_tune_clear(void);
+int  mdg_tune_save(const char *path);
+int  mdg_tune_load(const char *path);
+/* the tiling a box of these parameters would be created with */
+int  mdg_box_tiling(const mdg_box_params *p, int precision, mdg_tiling *out);
 
 #ifdef __cplusplus
 }

@@ -6,6 +6,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <map>
 #include <mutex>
 
 #include "host_setup.h"
@@ -27,11 +28,54 @@ static const double kPi = 3.1415926535897932385;
 
 static int ceil_div(long a, long b) { return (int)((a + b - 1) / b); }
 
+// ------------------------------------------------------------------ tuning table
+static std::mutex g_tune_mu;
+static std::map<TuneKey, Tiling> g_tune;
+
+bool TuneKey::operator<(const TuneKey& o) const {
+  const int a[5] = {precision, real_mode, cells_ok, nbucket, kbucket};
+  const int b[5] = {o.precision, o.real_mode, o.cells_ok, o.nbucket, o.kbucket};
+  return std::lexicographical_compare(a, a + 5, b, b + 5);
+}
+
+TuneKey tune_key(const BoxSpec& spec, int precision, int nk, bool cells_ok) {
+  TuneKey k;
+  k.precision = precision;
+  k.real_mode = spec.real_mode;
+  k.cells_ok = cells_ok ? 1 : 0;
+  k.nbucket = (int)lround(2.0 * log2((double)std::max(1, spec.N)));
+  k.kbucket = nk > 0 ? (int)lround(2.0 * log2((double)nk)) : -1;
+  return k;
+}
+
+bool tune_get(const TuneKey& k, Tiling* t) {
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  auto it = g_tune.find(k);
+  if (it == g_tune.end()) return false;
+  *t = it->second;
+  return true;
+}
+
+void tune_put(const TuneKey& k, const Tiling& t) {
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  g_tune[k] = t;
+}
+
+void tune_clear() {
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  g_tune.clear();
+}
+
+std::vector<std::pair<TuneKey, Tiling>> tune_entries() {
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  return std::vector<std::pair<TuneKey, Tiling>>(g_tune.begin(), g_tune.end());
+}
+
 // ------------------------------------------------------------------ tiling
-// Work granularity per box, from the box's own size only.  A lone box
-// should still spread over the chip (a few thousand waves for the large
-// ones); partial-sum buffers stay bounded.
-static Tiling choose_tiling(const BoxPlan& P, bool can_cells) {
+// Built-in rule: work granularity per box, from the box's own size only.  A
+// lone box should still spread over the chip (a few thousand waves for the
+// large ones); partial-sum buffers stay bounded.
+Tiling default_tiling(const BoxPlan& P, bool can_cells) {
   const BoxSpec& sp = P.spec;
   Tiling t;
   const int N = sp.N;
@@ -67,6 +111,39 @@ static Tiling choose_tiling(const BoxPlan& P, bool can_cells) {
   return t;
 }
 
+// Brings a tiling from the table or the tuner within what this box allows
+// (a table entry covers a class of boxes).
+static Tiling finish_tiling(const BoxPlan& P, bool can_cells, Tiling t) {
+  const BoxSpec& sp = P.spec;
+  const int N = sp.N;
+  if (sp.cells == MDG_CELL_LIST) t.cells = true;
+  else if (sp.cells == MDG_ALL_PAIRS) t.cells = false;
+  else t.cells = t.cells && can_cells;
+  if (!t.cells) t.cell_w = 1.0;
+  t.cell_w = std::max(1.0, std::min(t.cell_w, 4.0));
+  if (t.cells) {
+    for (int d = 0; d < 3; ++d)
+      if ((int)floor(sp.L[d] / (sp.rc * t.cell_w * (1.0 + 1e-9))) < 3) t.cell_w = 1.0;
+    t.rsplit = std::max(1, std::min(t.rsplit, 27));
+  } else {
+    t.rsplit = std::max(1, std::min(t.rsplit, std::max(1, N / 32)));
+  }
+  if (sp.ordered) t.rsplit = 1;
+  if (P.ewald) {
+    int nch = std::max(1, std::min(t.nchunk, 64));
+    nch = std::max(1, std::min(nch, ceil_div(N, kSkTile)));
+    nch = std::max(1, std::min(nch, (int)((64l << 20) / (16l * std::max(1, P.nk)))));
+    t.chunk = ceil_div(ceil_div(N, nch), kSkTile) * kSkTile;
+    t.nchunk = ceil_div(N, t.chunk);
+    t.ngroup = std::max(1, std::min(std::min(t.ngroup, 64), P.ncol));
+  } else {
+    t.nchunk = 1;
+    t.chunk = 0;
+    t.ngroup = 1;
+  }
+  return t;
+}
+
 // Contiguous runs of quads with about equal numbers of weight rows.
 static std::vector<int> partition_quads(const std::vector<KQuad>& quads, int groups) {
   std::vector<int> start;
@@ -89,7 +166,6 @@ static std::vector<int> partition_quads(const std::vector<KQuad>& quads, int gro
 }
 
 int make_plan(const BoxSpec& sp, int precision, BoxPlan* P, std::string* why) {
-  (void)precision;
   *P = BoxPlan();
   P->spec = sp;
   if (sp.N <= 0 || !(sp.L[0] > 0) || !(sp.L[1] > 0) || !(sp.L[2] > 0) || !(sp.rc > 0)) {
@@ -194,7 +270,19 @@ int make_plan(const BoxSpec& sp, int precision, BoxPlan* P, std::string* why) {
     if (why) *why = "cell list needs at least 3 cells of width >= rc per axis";
     return MDG_E_RANGE;
   }
-  P->tile = choose_tiling(*P, can_cells);
+  P->cells_ok = can_cells;
+  Tiling t;
+  if (sp.tile_override) {
+    t = *sp.tile_override;
+  } else if (tune_get(tune_key(sp, precision, P->nk, can_cells), &t)) {
+    t.tuned = true;
+  } else {
+    t = default_tiling(*P, can_cells);
+  }
+  P->tile = finish_tiling(*P, can_cells, t);
+  if (P->tile.cells && P->tile.cell_w > 1.0)
+    for (int d = 0; d < 3; ++d)
+      P->nc[d] = (int)floor(sp.L[d] / (sp.rc * P->tile.cell_w * (1.0 + 1e-9)));
   P->all_pairs = !P->tile.cells;
   if (!P->all_pairs) P->ncell = P->nc[0] * P->nc[1] * P->nc[2];
   else P->nc[0] = P->nc[1] = P->nc[2] = 0;

@@ -34,6 +34,7 @@ struct BoxSpec {
   const double* pe = nullptr;
   const double* fe = nullptr;
   int table_nb = 0, table_ntypes = 3;
+  const struct Tiling* tile_override = nullptr;  // the tuner's candidates
 };
 
 // Tile sizes of one box; a function of the box alone (and its precision),
@@ -44,7 +45,23 @@ struct Tiling {
   int chunk = 0;         // ions per chunk (multiple of kSkTile)
   int ngroup = 1;        // back-projection column groups
   bool cells = false;    // cell list (else all pairs)
+  double cell_w = 1.0;   // cell width / rc (>= 1)
+  bool tuned = false;    // taken from the tuning table
 };
+
+// The tuning table: measured tilings per box class (precision, real-space
+// mode, whether a cell list fits, and sqrt(2)-wide buckets of N and of the
+// k-vector count).  Boxes created while an entry exists for their class use
+// it; the others use the built-in rule.
+struct TuneKey {
+  int precision, real_mode, cells_ok, nbucket, kbucket;
+  bool operator<(const TuneKey& o) const;
+};
+TuneKey tune_key(const BoxSpec& spec, int precision, int nk, bool cells_ok);
+bool tune_get(const TuneKey& k, Tiling* t);
+void tune_put(const TuneKey& k, const Tiling& t);
+void tune_clear();
+std::vector<std::pair<TuneKey, Tiling>> tune_entries();
 
 struct BoxPlan {
   BoxSpec spec;
@@ -57,6 +74,7 @@ struct BoxPlan {
   int nquad = 0, nslot = 0;
   std::vector<double> kce, kcf;
   bool all_pairs = true;
+  bool cells_ok = false;            // >= 3 cells of width rc per axis
   int nc[3] = {0, 0, 0}, ncell = 0;
   int nreal = 0;
   Tiling tile;
@@ -64,6 +82,8 @@ struct BoxPlan {
 };
 
 int make_plan(const BoxSpec& spec, int precision, BoxPlan* out, std::string* why);
+// The built-in tiling rule of a plan (before tuning-table lookup).
+Tiling default_tiling(const BoxPlan& P, bool cells_ok);
 
 struct Box {
   BoxPlan plan;

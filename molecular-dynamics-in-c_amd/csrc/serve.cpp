@@ -71,6 +71,13 @@ double checksum(const std::vector<double>& r) {
 
 }  // namespace
 
+extern "C" int mdg_serve_job_params(const mdg_job* job, const mdg_serve_config* cfg,
+                                    mdg_box_params* out) {
+  if (!job || !cfg || !out || !(cfg->eps > 0) || !(cfg->rc_max > 0) || !(cfg->dt_fs > 0))
+    return MDG_E_ARG;
+  return job_params(*job, *cfg, out);
+}
+
 extern "C" int mdg_serve(const mdg_job* jobs, int njobs, const mdg_serve_config* cfg,
                          mdg_job_result* results, mdg_serve_stats* stats) {
   if (!jobs || njobs <= 0 || !cfg || !results || !stats) return MDG_E_ARG;

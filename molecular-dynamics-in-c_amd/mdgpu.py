@@ -73,6 +73,18 @@ class JobResult(C.Structure):
                 ("final_checksum", C.c_double), ("status", C.c_int)]
 
 
+class Tiling(C.Structure):
+    _fields_ = [("cells", C.c_int), ("cell_w", C.c_double), ("rsplit", C.c_int),
+                ("nchunk", C.c_int), ("ngroup", C.c_int), ("tuned", C.c_int)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+class TuneReport(C.Structure):
+    _fields_ = [("classes", C.c_int), ("runs", C.c_int), ("seconds", C.c_double)]
+
+
 class ServeStats(C.Structure):
     _fields_ = [("wall_s", C.c_double), ("ion_steps", C.c_double),
                 ("ion_steps_per_s", C.c_double), ("p50_latency_s", C.c_double),
@@ -172,6 +184,14 @@ def lib():
             "mdg_error_string": ([C.c_int], C.c_char_p),
             "mdg_serve": ([C.POINTER(Job), C.c_int, C.POINTER(ServeConfig),
                            C.POINTER(JobResult), C.POINTER(ServeStats)], C.c_int),
+            "mdg_tune": ([C.c_int, C.POINTER(BoxParams), C.c_int, C.c_longlong,
+                          C.POINTER(TuneReport)], C.c_int),
+            "mdg_tune_clear": ([], C.c_int),
+            "mdg_serve_job_params": ([C.POINTER(Job), C.POINTER(ServeConfig),
+                                      C.POINTER(BoxParams)], C.c_int),
+            "mdg_tune_save": ([C.c_char_p], C.c_int),
+            "mdg_tune_load": ([C.c_char_p], C.c_int),
+            "mdg_box_tiling": ([C.POINTER(BoxParams), C.c_int, C.POINTER(Tiling)], C.c_int),
             "mdg_dropin_status": ([], C.c_int),
             "mdg_dropin_clear_status": ([], None),
             "mdg_dropin_release": ([], None),
@@ -331,6 +351,39 @@ class Batch:
         n = C.c_int()
         check(lib().mdg_batch_profile(self.h, nsteps, cap, names, _p(ms), C.byref(n)), "profile")
         return {bytes(names[k]).split(b"\0")[0].decode(): float(ms[k]) for k in range(n.value)}
+
+
+def tune(boxes, precision, context_ions=0):
+    """Autotune the tilings of every box class among `boxes` (BoxParams);
+    returns the report as a dict."""
+    arr = (BoxParams * len(boxes))(*boxes)
+    rep = TuneReport()
+    check(lib().mdg_tune(len(boxes), arr, precision, context_ions, C.byref(rep)), "tune")
+    return {"classes": rep.classes, "runs": rep.runs, "seconds": rep.seconds}
+
+
+def tune_clear():
+    check(lib().mdg_tune_clear(), "tune_clear")
+
+
+def tune_save(path):
+    check(lib().mdg_tune_save(os.fsencode(path)), "tune_save")
+
+
+def tune_load(path):
+    check(lib().mdg_tune_load(os.fsencode(path)), "tune_load")
+
+
+def box_tiling(p, precision):
+    t = Tiling()
+    check(lib().mdg_box_tiling(C.byref(p), precision, C.byref(t)), "box_tiling")
+    return t.as_dict()
+
+
+def serve_job_params(job, cfg):
+    p = BoxParams()
+    check(lib().mdg_serve_job_params(C.byref(job), C.byref(cfg), C.byref(p)), "serve_job_params")
+    return p
 
 
 def serve(jobs, cfg):

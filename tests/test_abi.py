@@ -132,3 +132,52 @@ def test_no_device_fails_loudly():
     with pytest.raises(mdgpu.MdgError) as e:
         mdgpu.Batch([mdgpu.box_params(2, 2, 2, 5.64, 0.5, 27, 5.0)])
     assert e.value.code == mdgpu.E_NODEVICE
+
+
+def _tune_key(p, precision, nk, cells_ok):
+    N = 8 * p.nx * p.ny * p.nz
+    return (precision, p.real_mode, int(cells_ok), int(np.floor(2 * np.log2(N) + 0.5)),
+            int(np.floor(2 * np.log2(nk) + 0.5)))
+
+
+def test_tuning_table_round_trip(tmp_path):
+    """A loaded table entry is what boxes of that class are created with
+    (within what the box allows), saves back unchanged, and clears."""
+    mdgpu.tune_clear()
+    p = mdgpu.box_params(6, 6, 6, 5.64, 0.35, 200, 10.0, 2.0, real_mode=mdgpu.ANALYTIC)
+    t0 = mdgpu.box_tiling(p, mdgpu.MIXED)
+    assert t0["tuned"] == 0 and t0["rsplit"] >= 1 and t0["nchunk"] >= 1 and t0["ngroup"] >= 1
+    nk = mdref.lib().orc_count_kvectors(200)
+    L = 6 * 5.64
+    cells_ok = int(np.floor(L / (10.0 * (1 + 1e-9)))) >= 3
+    key = _tune_key(p, mdgpu.MIXED, nk, cells_ok)
+    src = tmp_path / "t.txt"
+    src.write_text("# test table\n" + " ".join(map(str, key)) + "  1 1.25 9 4 8\n")
+    mdgpu.tune_load(str(src))
+    t1 = mdgpu.box_tiling(p, mdgpu.MIXED)
+    assert t1["tuned"] == 1 and t1["cells"] == 1 and t1["rsplit"] == 9
+    assert t1["nchunk"] == 4 and t1["ngroup"] == 8
+    # 33.84 A / (10 A * 1.25) < 3 cells: the width falls back to rc
+    assert t1["cell_w"] == 1.0
+    # the other precision is another class
+    assert mdgpu.box_tiling(p, mdgpu.FP64)["tuned"] == 0
+    out = tmp_path / "u.txt"
+    mdgpu.tune_save(str(out))
+    rows = [ln.split() for ln in out.read_text().splitlines() if not ln.startswith("#")]
+    assert rows == [list(map(str, key)) + ["1", "1.25", "9", "4", "8"]]
+    mdgpu.tune_clear()
+    assert mdgpu.box_tiling(p, mdgpu.MIXED)["tuned"] == 0
+    bad = tmp_path / "bad.txt"
+    bad.write_text("1 2 3\n")
+    with pytest.raises(mdgpu.MdgError):
+        mdgpu.tune_load(str(bad))
+    assert mdgpu.box_tiling(p, mdgpu.MIXED)["tuned"] == 0
+
+
+def test_tune_without_device_fails_loudly():
+    if mdgpu.device_count() > 0:
+        pytest.skip("a device is present")
+    p = mdgpu.box_params(2, 2, 2, 5.64, 0.5, 27, 5.0)
+    with pytest.raises(mdgpu.MdgError) as e:
+        mdgpu.tune([p], mdgpu.MIXED)
+    assert e.value.code == mdgpu.E_NODEVICE

@@ -120,3 +120,52 @@ def test_mixed_nve_energy_drift(gpu):
     assert abs(d32) <= 1e-5
     assert abs(d32 - d64) <= 1e-6
     assert fl32 == pytest.approx(fl64, rel=0.1)
+
+
+def test_autotuned_tilings_keep_parity(gpu):
+    """Tuned tilings change only how work is split: FP64 forces agree with
+    the built-in tiling's to 1e-12, mixed stays within 1e-5 of FP64, and a
+    tuned box computes the same bits alone and in a batch."""
+    ps = [_params(3, H.A_NACL), _params(8, H.A_NACL), _params(6, H.A_KCL, salt="KCl"),
+          _params((2, 3, 4), H.A_NACL)]
+    sts = [_state(p, 300 + k) for k, p in enumerate(ps)]
+
+    def forces(prec):
+        b = mdgpu.Batch(ps, prec)
+        for k, (r, v, sp) in enumerate(sts):
+            b.set_state(k, r=r, v=v, species=sp)
+        b.forces()
+        out = [b.state(k)["f"] for k in range(len(ps))]
+        b.close()
+        return out
+
+    mdgpu.tune_clear()
+    try:
+        base64 = forces(mdgpu.FP64)
+        rep = mdgpu.tune(ps, mdgpu.MIXED, context_ions=30000)
+        rep64 = mdgpu.tune(ps, mdgpu.FP64, context_ions=30000)
+        assert rep["classes"] == len(ps) and rep["runs"] >= len(ps)
+        assert rep64["classes"] == len(ps)
+        for p in ps:
+            assert mdgpu.box_tiling(p, mdgpu.MIXED)["tuned"] == 1
+            assert mdgpu.box_tiling(p, mdgpu.FP64)["tuned"] == 1
+        t64, t32 = forces(mdgpu.FP64), forces(mdgpu.MIXED)
+        for a, b, c in zip(t64, base64, t32):
+            assert H.rel_err(a, b) <= 1e-12
+            assert H.rel_err(c, a) <= MIXED_TOL
+        # alone == in the batch under the tuned table
+        b = mdgpu.Batch(ps, mdgpu.MIXED)
+        for k, (r, v, sp) in enumerate(sts):
+            b.set_state(k, r=r, v=v, species=sp)
+        b.step(5)
+        together = [b.state(k)["r"] for k in range(len(ps))]
+        b.close()
+        for k, p in enumerate(ps):
+            lone = mdgpu.Batch([p], mdgpu.MIXED)
+            r, v, sp = sts[k]
+            lone.set_state(0, r=r, v=v, species=sp)
+            lone.step(5)
+            assert np.array_equal(lone.state(0)["r"], together[k])
+            lone.close()
+    finally:
+        mdgpu.tune_clear()
