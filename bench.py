@@ -71,6 +71,47 @@ def force_error(boxes, states_now):
     return mx, rms
 
 
+def nve_drift(boxes, states, steps, every=100, stride=8):
+    """NVE energy drift of every `stride`-th box: the consistent total energy
+    sampled every `every` steps over `steps` steps, least-squares slope per
+    ion scaled to 1e4 steps, for the mixed and the FP64 path from the same
+    start.  Returns medians / maxima over the boxes, in mdc's energy unit
+    (1e-18 J) and relative to |E| per ion."""
+    sel = list(range(0, len(boxes), stride))
+    sub = [boxes[k] for k in sel]
+    sst = [states[k] for k in sel]
+    n = np.array([8 * p.nx * p.ny * p.nz for p in sub], dtype=float)
+    slopes, e_per_ion = {}, None
+    for prec in (mdgpu.MIXED, mdgpu.FP64):
+        b = make_batch(sub, sst, prec)
+        b.step(20)
+        t, es = [], []
+        for s in range(0, steps + 1, every):
+            b.sync()
+            t.append(s)
+            es.append([b.scalars(k)["etot"] for k in range(len(sub))])
+            if s < steps:
+                b.step(every)
+        b.close()
+        es = np.array(es)                      # [samples, boxes]
+        tt = np.array(t, dtype=float)
+        A = np.vstack([tt, np.ones_like(tt)]).T
+        coef = np.linalg.lstsq(A, es, rcond=None)[0][0]   # energy per step, per box
+        slopes[prec] = coef / n * 1e4
+        if e_per_ion is None:
+            e_per_ion = np.abs(es[0]) / n
+    d32, d64 = slopes[mdgpu.MIXED], slopes[mdgpu.FP64]
+    return {"boxes": len(sub), "steps": steps, "sample_every": every, "unit": "1e-18 J",
+            "mixed_median_abs": float(np.median(np.abs(d32))),
+            "mixed_max_abs": float(np.abs(d32).max()),
+            "fp64_median_abs": float(np.median(np.abs(d64))),
+            "fp64_max_abs": float(np.abs(d64).max()),
+            "mixed_minus_fp64_max_abs": float(np.abs(d32 - d64).max()),
+            "mixed_rel_median_abs": float(np.median(np.abs(d32) / e_per_ion)),
+            "mixed_rel_max_abs": float((np.abs(d32) / e_per_ion).max()),
+            "mixed_minus_fp64_rel_max_abs": float((np.abs(d32 - d64) / e_per_ion).max())}
+
+
 def plain_one_at_a_time(boxes, states, steps, warmup, tune=True):
     """The plain FP64 path (mdc's tabulated real space) run box by box, its
     tilings tuned for a lone box on the chip."""
@@ -183,16 +224,17 @@ def serving(njobs, steps_scale, budget_gb, tune=True):
     return out
 
 
+COPY_PROBE_BYTES = 1 << 30
+
+
 def copy_probe():
-    """A plain copy kernel of known size, for scaling rocprofv3's FETCH_SIZE /
-    WRITE_SIZE counters (1 GiB read, 1 GiB written)."""
-    import torch
-    x = torch.empty(1 << 28, dtype=torch.float32, device="cuda").fill_(1.0)
-    y = torch.empty_like(x)
-    torch.cuda.synchronize()
-    y.copy_(x)
-    torch.cuda.synchronize()
-    del x, y
+    """libmdgpu's plain copy kernel (k_copy_probe) over 1 GiB, three times:
+    the kernel of known traffic (1 GiB read, 1 GiB written per dispatch)
+    that scales rocprofv3's FETCH_SIZE / WRITE_SIZE counters, and the HBM
+    rate a streaming kernel reaches here."""
+    ms = mdgpu.copy_probe(COPY_PROBE_BYTES, reps=2)
+    return {"bytes_read": COPY_PROBE_BYTES, "bytes_written": COPY_PROBE_BYTES, "ms": ms,
+            "GBps": 2 * COPY_PROBE_BYTES / (ms * 1e-3) / 1e9}
 
 
 def main():
@@ -201,12 +243,16 @@ def main():
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--plain-steps", type=int, default=20)
+    ap.add_argument("--drift-steps", type=int, default=2000)
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
-    ap.add_argument("--copy-probe", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="built-in tiling rule only")
+    ap.add_argument("--tune-file", default=None,
+                    help="tuning table: loaded if it exists, else written after tuning")
+    ap.add_argument("--only-batched", action="store_true",
+                    help="the batched run alone (for profiler runs)")
     ap.add_argument("--serve-jobs", type=int, default=48)
     ap.add_argument("--serve-steps-scale", type=float, default=0.05)
     ap.add_argument("--serve-budget-gb", type=float, default=16.0)
@@ -218,8 +264,8 @@ def main():
     if mdgpu.device_count() < 1:
         print("bench.py: no HIP device visible", file=sys.stderr)
         sys.exit(1)
-    if args.copy_probe:
-        copy_probe()
+    probe = copy_probe()
+    log(f"copy probe: {probe['GBps']:.0f} GB/s")
 
     boxes, temps, salts = W.batch256()
     states = load_states(boxes, temps)
@@ -230,13 +276,20 @@ def main():
     log(f"batched mixed: {len(boxes)} boxes, {total_ions} ions")
     tune_rep = None
     if not args.no_tune:
-        tune_rep = mdgpu.tune(boxes, mdgpu.MIXED)
+        if args.tune_file and os.path.exists(args.tune_file):
+            mdgpu.tune_load(args.tune_file)
+            tune_rep = {"loaded": args.tune_file}
+        else:
+            tune_rep = mdgpu.tune(boxes, mdgpu.MIXED)
+            if args.tune_file:
+                mdgpu.tune_save(args.tune_file)
         tl = [mdgpu.box_tiling(p, mdgpu.MIXED) for p in boxes]
         tune_rep["boxes_with_cells"] = sum(t["cells"] for t in tl)
         tune_rep["rsplit_hist"] = {str(r): sum(t["rsplit"] == r for t in tl)
                                    for r in sorted({t["rsplit"] for t in tl})}
-        log(f"  tuned {tune_rep['classes']} classes ({tune_rep['runs']} runs) "
-            f"in {tune_rep['seconds']:.1f} s")
+        if "classes" in tune_rep:
+            log(f"  tuned {tune_rep['classes']} classes ({tune_rep['runs']} runs) "
+                f"in {tune_rep['seconds']:.1f} s")
     b = make_batch(boxes, states, mdgpu.MIXED)
     b.step(args.warmup)
     b.sync()
@@ -263,19 +316,35 @@ def main():
     dom = max(prof, key=prof.get)
     dom_ms = prof[dom]
     ach = flops.get(dom, 0) / (dom_ms * 1e-3) / 1e12 if dom in flops else None
+    # HBM bytes per launch of the dominant kernel, from the committed
+    # rocprofv3 --pmc summary of this workload (counters need their own runs)
+    traffic, traffic_src = None, None
+    tfile = os.path.join(ROOT, "profiles", "batch256_mixed_traffic.json")
+    if os.path.exists(tfile):
+        t = json.load(open(tfile))
+        if t.get("kernel") == dom:
+            traffic, traffic_src = t["traffic_bytes_per_launch"], t["source"]
     roof = {"kernel": dom, "bound": "mfma", "pipe": "VALU FP32 (no MFMA)",
             "achieved": ach, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-            "frac": (ach / PEAK_FP32_TFLOPS) if ach else None, "traffic": None,
+            "frac": (ach / PEAK_FP32_TFLOPS) if ach else None, "traffic": traffic,
+            "traffic_source": traffic_src,
             "avg_ms": dom_ms, "flop_per_launch": flops.get(dom),
             "kernels_ms": {k: round(v, 5) for k, v in prof.items()}}
 
     # ---- accuracy of the mixed path ----
-    log("force error mixed vs FP64")
-    ferr_max, ferr_rms = force_error(boxes, states_now)
+    ferr_max = ferr_rms = drift_fit = None
+    if not args.only_batched:
+        log("force error mixed vs FP64")
+        ferr_max, ferr_rms = force_error(boxes, states_now)
+        log("NVE drift, mixed and FP64")
+        drift_fit = nve_drift(boxes, states, args.drift_steps)
+        log(f"  mixed median {drift_fit['mixed_median_abs']:.3e}, fp64 median "
+            f"{drift_fit['fp64_median_abs']:.3e}, max |mixed-fp64| "
+            f"{drift_fit['mixed_minus_fp64_max_abs']:.3e} (1e-18 J / ion / 1e4 steps)")
 
     # ---- plain FP64, one box at a time ----
     plain = None
-    if not args.no_plain:
+    if not (args.no_plain or args.only_batched):
         log("plain FP64 one box at a time")
         rate, psecs, prep = plain_one_at_a_time(boxes, states, args.plain_steps, 3,
                                                 tune=not args.no_tune)
@@ -285,13 +354,13 @@ def main():
         log(f"  {rate:.4e} ion-steps/s")
 
     serve = None
-    if not args.no_serve:
+    if not (args.no_serve or args.only_batched):
         log("serving stream")
         serve = serving(args.serve_jobs, args.serve_steps_scale, args.serve_budget_gb,
                         tune=not args.no_tune)
 
     cpu = None
-    if not args.no_cpu:
+    if not (args.no_cpu or args.only_batched):
         log("CPU baseline")
         cpu = cpu_baseline(boxes, states, args.cpu_seconds)
         log(f"  {cpu['value']:.4e} ion-steps/s ({cpu['kind']})")
@@ -309,10 +378,12 @@ def main():
         "speedup_vs_plain_one_at_a_time": (value / plain["value"]) if plain else None,
         "plain_fp64_one_at_a_time": plain,
         "max_rel_force_err": ferr_max, "rms_rel_force_err": ferr_rms,
-        "nve_drift_per_ion_per_1e4_steps": {"median_abs": float(np.median(np.abs(drift))),
-                                            "max_abs": float(np.abs(drift).max()),
-                                            "unit": "1e-18 J"},
+        "nve_drift_per_ion_per_1e4_steps": drift_fit,
+        "energy_change_timed_run": {"per_ion_per_1e4_steps_median_abs":
+                                    float(np.median(np.abs(drift))),
+                                    "max_abs": float(np.abs(drift).max()), "unit": "1e-18 J"},
         "autotune": tune_rep,
+        "hbm_copy_probe": probe,
         "serving": serve,
         "roofline": roof,
         "cpu_baseline": cpu,

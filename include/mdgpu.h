@@ -144,6 +144,12 @@ int  mdg_batch_work(mdg_batch *b, int box, long long *pairs, int *nk, int *n_ion
 
 const char *mdg_error_string(int code);
 
+/* No counterpart in mdc.  A plain device copy of `bytes` bytes (read
+ * `bytes`, write `bytes`), run reps times after one warm-up: the kernel of
+ * known traffic that scales hardware byte counters, and an HBM bandwidth
+ * yardstick.  *ms_avg: average time of one copy. */
+int  mdg_copy_probe(size_t bytes, int reps, double *ms_avg);
+
 /* -------------------------------------------------------------- serving */
 typedef struct {
   double arrival_s;     /* seconds after the start of the stream          */

@@ -7,6 +7,7 @@
 
 #include "engine.h"
 #include "host_setup.h"
+#include "launch.h"
 #include "mdgpu.h"
 
 using namespace mdg;
@@ -101,6 +102,42 @@ int mdg_device_count(int* n) {
   }
   if (n) *n = c;
   return c > 0 ? MDG_OK : MDG_E_NODEVICE;
+}
+
+int mdg_copy_probe(size_t bytes, int reps, double* ms_avg) {
+  if (bytes < 16 || reps < 1 || !ms_avg) return MDG_E_ARG;
+  int ndev = 0;
+  if (mdg_device_count(&ndev) != MDG_OK) return MDG_E_NODEVICE;
+  const size_t n16 = bytes / 16;
+  void *src = nullptr, *dst = nullptr;
+  hipStream_t s = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  int rc = MDG_OK;
+  if (hipMalloc(&src, n16 * 16) != hipSuccess || hipMalloc(&dst, n16 * 16) != hipSuccess) {
+    (void)hipGetLastError();
+    rc = MDG_E_NOMEM;
+  }
+  if (rc == MDG_OK &&
+      (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess ||
+       hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess ||
+       hipMemsetAsync(src, 0, n16 * 16, s) != hipSuccess))
+    rc = MDG_E_HIP;
+  if (rc == MDG_OK) {
+    launch_copy_probe(src, dst, n16, s);  // warm-up
+    (void)hipEventRecord(e0, s);
+    for (int k = 0; k < reps; ++k) launch_copy_probe(src, dst, n16, s);
+    (void)hipEventRecord(e1, s);
+    float ms = 0.f;
+    if (hipStreamSynchronize(s) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess)
+      rc = MDG_E_HIP;
+    *ms_avg = ms / reps;
+  }
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  if (s) (void)hipStreamDestroy(s);
+  if (src) (void)hipFree(src);
+  if (dst) (void)hipFree(dst);
+  return rc;
 }
 
 int mdg_batch_destroy(mdg_batch* b) {

@@ -93,4 +93,19 @@ void launch_moment_correction(const BoxDesc* boxes, const int* ids, int n, hipSt
   if (n > 0) hipLaunchKernelGGL(k_moment_correction, MDG_GRID(n, kScalThreads), boxes, ids);
 }
 
+// Plain copy of n16 16-byte words: the reference kernel of known traffic
+// for scaling hardware byte counters, and an HBM bandwidth yardstick.
+__global__ __launch_bounds__(256) void k_copy_probe(const float4* __restrict__ src,
+                                                    float4* __restrict__ dst, size_t n16) {
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += stride)
+    dst[i] = src[i];
+}
+
+void launch_copy_probe(const void* src, void* dst, size_t n16, hipStream_t s) {
+  if (n16 == 0) return;
+  const size_t blocks = std::min<size_t>((n16 + 255) / 256, 8192);
+  hipLaunchKernelGGL(k_copy_probe, MDG_GRID(blocks, 256), (const float4*)src, (float4*)dst, n16);
+}
+
 }  // namespace mdg

@@ -16,6 +16,7 @@ void launch_sk64(const BoxDesc* boxes, const Item* items, int n, int hs, int hsw
 // hz: register row length of the back-projection (8, 16, 24, 32, 48 or 64)
 void launch_kforce64(int hz, const BoxDesc* boxes, const Item* items, int n, hipStream_t s);
 void launch_sk_finalize(const BoxDesc* boxes, const Item* items, int n, hipStream_t s);
+void launch_copy_probe(const void* src, void* dst, size_t n16, hipStream_t s);
 void launch_force_vv(bool vv, const BoxDesc* boxes, const Item* items, int n, hipStream_t s,
                      int terms = 3);
 void launch_clear_foc(const BoxDesc* boxes, const Item* items, int n, hipStream_t s);

@@ -187,6 +187,7 @@ def lib():
             "mdg_tune": ([C.c_int, C.POINTER(BoxParams), C.c_int, C.c_longlong,
                           C.POINTER(TuneReport)], C.c_int),
             "mdg_tune_clear": ([], C.c_int),
+            "mdg_copy_probe": ([C.c_size_t, C.c_int, C.POINTER(C.c_double)], C.c_int),
             "mdg_serve_job_params": ([C.POINTER(Job), C.POINTER(ServeConfig),
                                       C.POINTER(BoxParams)], C.c_int),
             "mdg_tune_save": ([C.c_char_p], C.c_int),
@@ -360,6 +361,13 @@ def tune(boxes, precision, context_ions=0):
     rep = TuneReport()
     check(lib().mdg_tune(len(boxes), arr, precision, context_ions, C.byref(rep)), "tune")
     return {"classes": rep.classes, "runs": rep.runs, "seconds": rep.seconds}
+
+
+def copy_probe(nbytes, reps=10):
+    """Average ms of a plain device copy of nbytes (read nbytes + write nbytes)."""
+    ms = C.c_double()
+    check(lib().mdg_copy_probe(nbytes, reps, C.byref(ms)), "copy_probe")
+    return ms.value
 
 
 def tune_clear():

@@ -1,0 +1,82 @@
+/*
+ * dropin_main.c -- a fixed mdc driver, linked two ways by oracle/Makefile.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Together with ref_harness.c and mdc's own
+ * sources it makes two executables in oracle/_ref:
+ *   mdc_cpu  every mdc routine from mdc/program (the reference's CPU path);
+ *   mdc_gpu  the same program with real.c, rcprcl.c and rv.c left out and
+ *            libmdgpu.so linked in their place, which is the link-time
+ *            replacement a maintainer makes to use the GPU force path.
+ * The GPU test runs both on the same input and compares what they print.
+ *
+ * The step loop is mdc's (mdc/program/main.c:26-33): next_rv_verlet,
+ * calc_kin, calc_press; without thermostat and file output.
+ *
+ * usage: mdc_{cpu,gpu} n steps seed    (n x n x n NaCl cells, a = 5.64 A,
+ *        alpha = 5.6 / L, hm = 27, cutoff L / 2, dt = 2 fs)
+ */
+#include <stdio.h>
+#include <stdlib.h>
+
+#include "md.h"
+
+/* mdc's prototypes.h declares main(void); these are the routines used */
+void next_rv_verlet(void);
+void calc_kin(void);
+void calc_press(void);
+void moment_correction(void);
+int ref_setup(int nx, int ny, int nz, double a, double alpha, int hm, double radius,
+              double dt_fs);
+
+static unsigned long long lcg_state;
+
+static double uniform(void) /* in [0, 1) */
+{
+  lcg_state = lcg_state * 6364136223846793005ull + 1442695040888963407ull;
+  return (double)(lcg_state >> 11) * (1.0 / 9007199254740992.0);
+}
+
+int main(int argc, char **argv)
+{
+  int n, steps, i;
+  double a = 5.64, L, sx = 0, sy = 0, sz = 0;
+
+  if (argc != 4) {
+    fprintf(stderr, "usage: %s n steps seed\n", argv[0]);
+    return 2;
+  }
+  n = atoi(argv[1]);
+  steps = atoi(argv[2]);
+  lcg_state = (unsigned long long)atoll(argv[3]);
+  L = n * a;
+  ref_setup(n, n, n, a, 5.6 / L, 27, L / 2.0, 2.0);
+
+  /* lattice jitter of +-0.1 A and small random velocities */
+  for (i = 0; i < sys.N; i++) {
+    sys.rx[i] += 0.2 * (uniform() - 0.5);
+    sys.ry[i] += 0.2 * (uniform() - 0.5);
+    sys.rz[i] += 0.2 * (uniform() - 0.5);
+    if (sys.rx[i] < 0.0) sys.rx[i] += L;
+    if (sys.ry[i] < 0.0) sys.ry[i] += L;
+    if (sys.rz[i] < 0.0) sys.rz[i] += L;
+    sys.vx[i] = 0.05 * (uniform() - 0.5);
+    sys.vy[i] = 0.05 * (uniform() - 0.5);
+    sys.vz[i] = 0.05 * (uniform() - 0.5);
+  }
+  moment_correction();
+
+  for (sys.step = 1; sys.step <= steps; sys.step++) {
+    next_rv_verlet();
+    calc_kin();
+    calc_press();
+    printf("%d %.17g %.17g %.17g %.17g %.17g %.17g\n", sys.step, sys.pot, sys.kin,
+           sys.virX, sys.virY, sys.virZ, sys.pres);
+  }
+  for (i = 0; i < sys.N; i++) {
+    sx += sys.rx[i];
+    sy += sys.ry[i];
+    sz += sys.rz[i];
+  }
+  printf("final %.17g %.17g %.17g %.17g %.17g\n", sx, sy, sz, sys.rx[0], sys.vx[sys.N - 1]);
+  return 0;
+}
