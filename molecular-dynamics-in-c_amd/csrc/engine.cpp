@@ -101,12 +101,21 @@ Tiling default_tiling(const BoxPlan& P, bool can_cells) {
     nch = std::max(1, std::min(nch, (int)((64l << 20) / (16l * std::max(1, P.nk)))));
     t.chunk = ceil_div(ceil_div(N, nch), kSkTile) * kSkTile;
     t.nchunk = ceil_div(N, t.chunk);
-    // back-projection: ~2048 waves for a lone box, >= 64 k per group
-    const int nib = ceil_div(N, kKfThreads);
-    int g = ceil_div(2048, nib);
-    g = std::min(g, std::max(1, P.nk / 64));
-    g = std::min(g, 64);
-    t.ngroup = std::max(1, std::min(g, P.ncol));
+    if (!P.kftiles.empty()) {
+      // MFMA back-projection (32 ions per wave): ~1024 waves for a lone
+      // box, >= 8 tiles per group
+      const int nib = ceil_div(N, 32);
+      int g = ceil_div(1024, nib);
+      g = std::min(g, std::max(1, (int)P.kftiles.size() / 8));
+      t.ngroup = std::max(1, std::min(g, 64));
+    } else {
+      // back-projection: ~2048 waves for a lone box, >= 64 k per group
+      const int nib = ceil_div(N, kKfThreads);
+      int g = ceil_div(2048, nib);
+      g = std::min(g, std::max(1, P.nk / 64));
+      g = std::min(g, 64);
+      t.ngroup = std::max(1, std::min(g, P.ncol));
+    }
   }
   return t;
 }
@@ -144,18 +153,18 @@ static Tiling finish_tiling(const BoxPlan& P, bool can_cells, Tiling t) {
   return t;
 }
 
-// Contiguous runs of quads with about equal numbers of weight rows.
-static std::vector<int> partition_quads(const std::vector<KQuad>& quads, int groups) {
+// Contiguous runs of units with about equal total weight.
+static std::vector<int> partition_runs(const std::vector<long>& weight, int groups) {
   std::vector<int> start;
-  const int nq = (int)quads.size();
+  const int nq = (int)weight.size();
   long rows = 0;
-  for (const KQuad& q : quads) rows += q.nrow;
+  for (long w : weight) rows += w;
   groups = std::max(1, std::min(groups, nq));
   start.push_back(0);
   long acc = 0;
   int g = 1;
   for (int c = 0; c < nq && g < groups; ++c) {
-    acc += quads[c].nrow;
+    acc += weight[c];
     if (acc * groups >= (long)g * rows) {
       if (c + 1 < nq) start.push_back(c + 1);
       ++g;
@@ -163,6 +172,20 @@ static std::vector<int> partition_quads(const std::vector<KQuad>& quads, int gro
   }
   start.push_back(nq);
   return start;
+}
+
+// Quads weighted by their weight rows (FP64 back-projection).
+static std::vector<int> partition_quads(const std::vector<KQuad>& quads, int groups) {
+  std::vector<long> w;
+  for (const KQuad& q : quads) w.push_back(q.nrow);
+  return partition_runs(w, groups);
+}
+
+// MFMA tiles weighted by their K blocks (mixed back-projection).
+static std::vector<int> partition_tiles(const std::vector<KfTile>& tiles, int groups) {
+  std::vector<long> w;
+  for (const KfTile& t : tiles) w.push_back(std::max(1, t.kb_hi - t.kb_lo));
+  return partition_runs(w, groups);
 }
 
 int make_plan(const BoxSpec& sp, int precision, BoxPlan* P, std::string* why) {
@@ -258,6 +281,23 @@ int make_plan(const BoxSpec& sp, int precision, BoxPlan* P, std::string* why) {
     }
     P->nquad = (int)P->quads.size();
     P->nslot = 4 * row;
+    if (precision == MDG_MIXED) {
+      // MFMA tiles: 8 consecutive columns and the K blocks their hz span
+      P->kbk = (P->hmax + 1 + 7) / 8;
+      for (int c0 = 0; c0 < P->ncol; c0 += kKfTileCols) {
+        KfTile t{};
+        t.c0 = c0;
+        t.nc = std::min(kKfTileCols, P->ncol - c0);
+        int lo = 1 << 30, hi = 0;
+        for (int c = c0; c < c0 + t.nc; ++c) {
+          lo = std::min(lo, P->cols[c].hz0);
+          hi = std::max(hi, P->cols[c].hz0 + P->cols[c].len);
+        }
+        t.kb_lo = lo / 8;
+        t.kb_hi = (hi + 7) / 8;
+        P->kftiles.push_back(t);
+      }
+    }
   }
   // cell grid: cells wider than rc (with a relative margin, so that two ions
   // within rc never bin two cells apart after rounding)
@@ -288,7 +328,9 @@ int make_plan(const BoxSpec& sp, int precision, BoxPlan* P, std::string* why) {
   else P->nc[0] = P->nc[1] = P->nc[2] = 0;
   const int homes = P->all_pairs ? ceil_div(sp.N, kRealThreads) : P->ncell;
   P->nreal = homes * P->tile.rsplit;
-  if (P->ewald) P->kf_group_start = partition_quads(P->quads, P->tile.ngroup);
+  if (P->ewald)
+    P->kf_group_start = precision == MDG_MIXED ? partition_tiles(P->kftiles, P->tile.ngroup)
+                                               : partition_quads(P->quads, P->tile.ngroup);
   return MDG_OK;
 }
 
@@ -296,6 +338,7 @@ int make_plan(const BoxSpec& sp, int precision, BoxPlan* P, std::string* why) {
 struct Layout {
   size_t ion[12], typ, rpf, cell_of, cnt, cwave, start, sorted, s[3], styp;
   size_t cols, segs, quads, khz, kslot, kce, kcf, kab, kab32, kpot, tx, ty, tw, skp, fp;
+  size_t wb, kfa, colw, kft;
   size_t tab, tab32, rpart, rpairs, scal, err, total;
 };
 
@@ -336,7 +379,13 @@ static Layout layout(const BoxPlan& P, int precision) {
     L.kce = take(off, sizeof(double) * P.nk);
     L.kcf = take(off, sizeof(double) * P.nk);
     L.kab = take(off, sizeof(double) * 4 * (size_t)P.nslot);
-    if (mixed) L.kab32 = take(off, sizeof(float) * 4 * (size_t)P.nslot);
+    if (mixed) {
+      const size_t nt = P.kftiles.size();
+      L.wb = take(off, 32 * (size_t)(2 * P.kbk) * N);
+      L.kfa = take(off, 16 * (size_t)kWave * 2 * P.kbk * nt);
+      L.colw = take(off, sizeof(float4) * (size_t)P.ncol);
+      L.kft = take(off, sizeof(KfTile) * nt);
+    }
     L.kpot = take(off, sizeof(double) * P.nk);
     L.tx = take(off, cplx * H * N);
     L.ty = take(off, cplx * H * N);
@@ -464,7 +513,16 @@ int box_create(const BoxSpec& spec, int precision, Box** out, std::string* why) 
     D.kce = (const double*)(base + L.kce);
     D.kcf = (const double*)(base + L.kcf);
     D.kab = (double*)(base + L.kab);
-    D.kab32 = mixed ? (float*)(base + L.kab32) : nullptr;
+    D.kab32 = nullptr;
+    if (mixed) {
+      D.wb = base + L.wb;
+      D.kfa = base + L.kfa;
+      D.colw = (float4*)(base + L.colw);
+      D.kft = (const KfTile*)(base + L.kft);
+      D.kbk = P.kbk;
+      D.ntile = (int)P.kftiles.size();
+      up(L.kft, P.kftiles.data(), sizeof(KfTile) * P.kftiles.size());
+    }
     D.kpot = (double*)(base + L.kpot);
     D.tabx = base + L.tx;
     D.taby = base + L.ty;
@@ -606,9 +664,15 @@ int BoxSet::set_boxes(const std::vector<Box*>& boxes) {
   // big boxes' items first, so the longest tiles start earliest
   std::stable_sort(order.begin(), order.end(),
                    [&](int a, int b) { return boxes_[a]->desc.N > boxes_[b]->desc.N; });
-  std::vector<Item> real, tab, sk, fin, kf, ion;
+  std::vector<Item> real, tab, sk, fin, kf, ion, pack;
   std::vector<int> cellid, allid;
   int hmax = 0;
+  const bool mixed = precision_ == MDG_MIXED;
+  // the widest box fixes the back-projection's K blocks, hence its ion blocks
+  int hmax_all = 0;
+  for (const Box* B : boxes_)
+    if (B->plan.ewald) hmax_all = std::max(hmax_all, B->plan.hmax);
+  const int kf_ions = kforce_mfma_ions(kforce_hz_for(hmax_all) / 8);
   for (int bi : order) {
     const Box* B = boxes_[bi];
     const BoxPlan& P = B->plan;
@@ -634,10 +698,13 @@ int BoxSet::set_boxes(const std::vector<Box*>& boxes) {
           sk.push_back({bi, ch, s0, std::min(kSkThreads, P.nseg - s0), 0});
       for (int k0 = 0; k0 < P.nk; k0 += kIonThreads) fin.push_back({bi, k0, 0, 0, 0});
       const int gk = (int)P.kf_group_start.size() - 1;
-      for (int i0 = 0; i0 < N; i0 += kKfThreads)
+      const int ion_step = mixed ? kf_ions : kKfThreads;
+      for (int i0 = 0; i0 < N; i0 += ion_step)
         for (int g = 0; g < gk; ++g)
           kf.push_back({bi, i0, P.kf_group_start[g],
                         P.kf_group_start[g + 1] - P.kf_group_start[g], g});
+      if (mixed)
+        for (int t = 0; t < (int)P.kftiles.size(); ++t) pack.push_back({bi, t, 0, 0, 0});
     }
   }
   const int H = hmax + 1;
@@ -656,6 +723,7 @@ int BoxSet::set_boxes(const std::vector<Box*>& boxes) {
   app(fin, off_fin_, n_fin_);
   app(kf, off_kf_, n_kf_);
   app(ion, off_ion_, n_ion_);
+  app(pack, off_pack_, n_pack_);
   std::vector<int> ids;
   off_cellid_ = 0;
   n_cellid_ = (int)cellid.size();
@@ -715,11 +783,17 @@ void BoxSet::launch_real(hipStream_t s, int mode) {
 void BoxSet::launch_recip(hipStream_t s) {
   const bool mixed = precision_ == MDG_MIXED;
   launch_phase_tables(mixed, d_desc_, d_items_ + off_tab_, n_tab_, s);
-  if (mixed) launch_sk32(d_desc_, d_items_ + off_sk_, n_sk_, hs_, hsw_, s);
-  else launch_sk64(d_desc_, d_items_ + off_sk_, n_sk_, hs_, hsw_, s);
-  launch_sk_finalize(d_desc_, d_items_ + off_fin_, n_fin_, s);
-  if (mixed) launch_kforce32(hz_, d_desc_, d_items_ + off_kf_, n_kf_, s);
-  else launch_kforce64(hz_, d_desc_, d_items_ + off_kf_, n_kf_, s);
+  if (mixed) {
+    launch_wb_tables(d_desc_, d_items_ + off_tab_, n_tab_, s);
+    launch_sk32(d_desc_, d_items_ + off_sk_, n_sk_, hs_, hsw_, s);
+    launch_sk_finalize(d_desc_, d_items_ + off_fin_, n_fin_, s);
+    launch_kf_pack(d_desc_, d_items_ + off_pack_, n_pack_, s);
+    launch_kforce_mfma(hz_ / 8, d_desc_, d_items_ + off_kf_, n_kf_, s);
+  } else {
+    launch_sk64(d_desc_, d_items_ + off_sk_, n_sk_, hs_, hsw_, s);
+    launch_sk_finalize(d_desc_, d_items_ + off_fin_, n_fin_, s);
+    launch_kforce64(hz_, d_desc_, d_items_ + off_kf_, n_kf_, s);
+  }
 }
 
 void BoxSet::enqueue_forces_parts(hipStream_t s, bool real, int real_write_mode, bool recip) {
@@ -798,11 +872,13 @@ int BoxSet::profile(int nsteps, hipStream_t s, std::vector<std::string>* names,
   int rc = sync_desc(s);
   if (rc != MDG_OK) return rc;
   const bool mixed = precision_ == MDG_MIXED;
-  const char* kn[8] = {"cell_build", "phase_tables", "real_space", "sk_partial",
-                       "sk_finalize", "kforce",      "force_vv",   "box_scalars"};
-  std::vector<hipEvent_t> ev(2 * 8 * std::max(1, nsteps));
+  constexpr int K = 10;
+  const char* kn[K] = {"cell_build", "phase_tables", "real_space", "sk_partial",
+                       "sk_finalize", "kforce",      "force_vv",   "box_scalars",
+                       "wb_tables",  "kf_pack"};
+  std::vector<hipEvent_t> ev(2 * K * std::max(1, nsteps));
   for (auto& e : ev) MDG_HIP(hipEventCreate(&e));
-  std::vector<int> used(8, 0), which;
+  std::vector<int> used(K, 0), which;
   int e = 0;
   auto rec = [&](int k, auto&& fn) {
     (void)hipEventRecord(ev[e], s);
@@ -817,13 +893,15 @@ int BoxSet::profile(int nsteps, hipStream_t s, std::vector<std::string>* names,
     rec(2, [&] { launch_real(s, 0); });
     if (n_sk_) {
       rec(1, [&] { launch_phase_tables(mixed, d_desc_, d_items_ + off_tab_, n_tab_, s); });
+      if (mixed) rec(8, [&] { launch_wb_tables(d_desc_, d_items_ + off_tab_, n_tab_, s); });
       rec(3, [&] {
         if (mixed) launch_sk32(d_desc_, d_items_ + off_sk_, n_sk_, hs_, hsw_, s);
         else launch_sk64(d_desc_, d_items_ + off_sk_, n_sk_, hs_, hsw_, s);
       });
       rec(4, [&] { launch_sk_finalize(d_desc_, d_items_ + off_fin_, n_fin_, s); });
+      if (mixed) rec(9, [&] { launch_kf_pack(d_desc_, d_items_ + off_pack_, n_pack_, s); });
       rec(5, [&] {
-        if (mixed) launch_kforce32(hz_, d_desc_, d_items_ + off_kf_, n_kf_, s);
+        if (mixed) launch_kforce_mfma(hz_ / 8, d_desc_, d_items_ + off_kf_, n_kf_, s);
         else launch_kforce64(hz_, d_desc_, d_items_ + off_kf_, n_kf_, s);
       });
     }
@@ -831,7 +909,7 @@ int BoxSet::profile(int nsteps, hipStream_t s, std::vector<std::string>* names,
     rec(7, [&] { launch_box_scalars(d_desc_, d_ids_ + off_allid_, n_allid_, 1, s); });
   }
   MDG_HIP(hipStreamSynchronize(s));
-  std::vector<double> tot(8, 0.0);
+  std::vector<double> tot(K, 0.0);
   for (size_t q = 0; q < which.size(); ++q) {
     float t = 0.f;
     MDG_HIP(hipEventElapsedTime(&t, ev[2 * q], ev[2 * q + 1]));
@@ -840,7 +918,7 @@ int BoxSet::profile(int nsteps, hipStream_t s, std::vector<std::string>* names,
   for (auto& x : ev) (void)hipEventDestroy(x);
   names->clear();
   ms->clear();
-  for (int k = 0; k < 8; ++k)
+  for (int k = 0; k < K; ++k)
     if (used[k]) {
       names->push_back(kn[k]);
       ms->push_back(tot[k] / nsteps);

@@ -78,7 +78,11 @@ struct BoxPlan {
   int nc[3] = {0, 0, 0}, ncell = 0;
   int nreal = 0;
   Tiling tile;
-  std::vector<int> kf_group_start;  // first quad of each back-projection group (+ end)
+  // back-projection groups: first quad (FP64) or first MFMA tile (mixed) of
+  // each group, plus the end
+  std::vector<int> kf_group_start;
+  std::vector<KfTile> kftiles;      // mixed: MFMA tiles of 8 columns
+  int kbk = 0;                      // mixed: K blocks (8 hz each) covering 0..hmax
 };
 
 int make_plan(const BoxSpec& spec, int precision, BoxPlan* out, std::string* why);
@@ -141,6 +145,7 @@ class BoxSet {
   int off_real_ = 0, n_real_ = 0, off_tab_ = 0, n_tab_ = 0, off_sk_ = 0, n_sk_ = 0;
   int off_fin_ = 0, n_fin_ = 0, off_kf_ = 0, n_kf_ = 0, off_ion_ = 0, n_ion_ = 0;
   int off_cellid_ = 0, n_cellid_ = 0, off_allid_ = 0, n_allid_ = 0;
+  int off_pack_ = 0, n_pack_ = 0;
   int hs_ = 1, hsw_ = 8, hz_ = 8;  // LDS strides / register row of the k-space kernels
   hipGraphExec_t graph_exec_ = nullptr;
   hipGraph_t graph_ = nullptr;

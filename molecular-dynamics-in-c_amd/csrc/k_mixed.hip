@@ -1,7 +1,9 @@
-// k_mixed.hip -- FP32 pair and k-space kernels of the mixed-precision path.
-// Positions, structure-factor sums beyond one LDS tile, per-ion force sums
-// across tiles and columns, energies and virials stay FP64.
+// k_mixed.hip -- FP32 pair and k-space kernels of the mixed-precision path,
+// and its back-projection on the matrix cores (FP16 hi/lo operands, FP32
+// accumulation).  Positions, structure-factor sums beyond one LDS tile,
+// per-ion force sums across tiles, energies and virials stay FP64.
 #include "kernels.h"
+#include "kernels_mfma.h"
 #include "launch.h"
 
 namespace mdg {
@@ -24,16 +26,30 @@ void launch_sk32(const BoxDesc* boxes, const Item* items, int n, int hs, int hsw
   hipLaunchKernelGGL(k_sk_seg<float>, MDG_GRID_LDS(n, kSkThreads, lds), boxes, items, hs, hsw);
 }
 
-void launch_kforce32(int hz, const BoxDesc* boxes, const Item* items, int n, hipStream_t s) {
-  if (n <= 0) return;
-  switch (hz) {
-    case 8: hipLaunchKernelGGL((k_kforce<float, 8>), MDG_GRID(n, kKfThreads), boxes, items); break;
-    case 16: hipLaunchKernelGGL((k_kforce<float, 16>), MDG_GRID(n, kKfThreads), boxes, items); break;
-    case 24: hipLaunchKernelGGL((k_kforce<float, 24>), MDG_GRID(n, kKfThreads), boxes, items); break;
-    case 32: hipLaunchKernelGGL((k_kforce<float, 32>), MDG_GRID(n, kKfThreads), boxes, items); break;
-    case 48: hipLaunchKernelGGL((k_kforce<float, 48>), MDG_GRID(n, kKfThreads), boxes, items); break;
-    default: hipLaunchKernelGGL((k_kforce<float, 64>), MDG_GRID(n, kKfThreads), boxes, items); break;
-  }
+void launch_wb_tables(const BoxDesc* boxes, const Item* items, int n, hipStream_t s) {
+  if (n > 0) hipLaunchKernelGGL(k_wb_tables, MDG_GRID(n, kIonThreads), boxes, items);
 }
+
+void launch_kf_pack(const BoxDesc* boxes, const Item* items, int n, hipStream_t s) {
+  if (n > 0) hipLaunchKernelGGL(k_kf_pack, MDG_GRID(n, kWave), boxes, items);
+}
+
+void launch_kforce_mfma(int kb, const BoxDesc* boxes, const Item* items, int n, hipStream_t s) {
+  if (n <= 0) return;
+  const int t = kKfMfmaWaves * kWave;
+#define MDG_KF(K) \
+  hipLaunchKernelGGL((k_kforce_mfma<K, kf_blocks_for(K)>), MDG_GRID(n, t), boxes, items)
+  switch (kb) {
+    case 1: MDG_KF(1); break;
+    case 2: MDG_KF(2); break;
+    case 3: MDG_KF(3); break;
+    case 4: MDG_KF(4); break;
+    case 6: MDG_KF(6); break;
+    default: MDG_KF(8); break;
+  }
+#undef MDG_KF
+}
+
+int kforce_mfma_ions(int kb) { return 32 * kf_blocks_for(kb) * kKfMfmaWaves; }
 
 }  // namespace mdg

@@ -1,0 +1,284 @@
+// kernels_mfma.h -- reciprocal-space back-projection on the matrix cores
+// (mixed path).
+//
+// Per k column (hx, hy) and ion i the back-projection needs
+//   U_i = sum_hz T_k W_i[hz],   V_i = sum_hz hz T_k W_i[hz]
+// (T_k = cf_k (2Ss + i 2Sc), W_i[hz] = e^{2 pi i hz z_i / Lz}), then
+// G = Re(X_i[hx] Y_i[hy] U_i) and Gz likewise from V_i (the sum of
+// mdc/program/rcprcl.c:84-92 in separable form).  Over 8 columns and 32
+// ions these are one real matrix product
+//   C[(column, Re U | Im U | Re V | Im V)][ion] = sum_(hz, re|im) A B,
+// done with v_mfma_f32_32x32x16_f16.  FP16 keeps 11 bits, so every operand
+// is carried as hi + lo halves (hi = fp16(x), lo = fp16(x - hi)) and each
+// product as hi*hi + hi*lo + lo*hi in FP32 accumulators: about 22 bits per
+// product, the FP32 arithmetic of the VALU path.  The weights of a column
+// are scaled by powers of two (largest |value| in [2^10, 2^11)) so that
+// they sit in FP16's range; the scale is undone in the per-column epilogue
+// weights.
+//
+// Operand layouts of v_mfma_f32_32x32x16_f16 (lane l, element e):
+//   A[row = l % 32][k = 8 (l / 32) + e],   B[k = 8 (l / 32) + e][col = l % 32],
+//   C[row = 8 (v / 4) + 4 (l / 32) + v % 4][col = l % 32] for result v.
+// Here k = 2 hz' + (re|im), hz' = 8 kb + k / 2 within K block kb; a lane's
+// 8 B elements are W[4 (l/32) .. 4 (l/32) + 3] (re, im) of its ion; its 16
+// results are columns 2 j + l/32 (j = v / 4) with the four sums v % 4.
+#pragma once
+
+#include "kernels.h"
+
+namespace mdg {
+
+typedef _Float16 h8_t __attribute__((ext_vector_type(8)));
+typedef float f16_t __attribute__((ext_vector_type(16)));
+typedef unsigned int u4_t __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ void split_f16(float x, _Float16& hi, _Float16& lo) {
+  hi = (_Float16)x;
+  lo = (_Float16)(x - (float)hi);
+}
+
+__device__ __forceinline__ u4_t pack_h8(const _Float16 (&v)[8]) {
+  h8_t h;
+#pragma unroll
+  for (int e = 0; e < 8; ++e) h[e] = v[e];
+  return __builtin_bit_cast(u4_t, h);
+}
+
+// ----------------------------------------------------------------------
+// B operands: W[hz] of every ion as FP16 hi/lo records, [hz/4][N] x 32 B
+// (8 hi halves: re, im of hz = 4q .. 4q+3; then the 8 lo halves).
+// ----------------------------------------------------------------------
+__global__ __launch_bounds__(kIonThreads) void k_wb_tables(const BoxDesc* __restrict__ boxes,
+                                                           const Item* __restrict__ items) {
+  const Item it = items[blockIdx.x];
+  const BoxDesc& B = boxes[it.box];
+  const int N = B.N;
+  const int i = it.a + threadIdx.x;
+  if (i >= N) return;
+  const double sz = B.rz[i] * B.invL[2];
+  u4_t* wb = reinterpret_cast<u4_t*>(B.wb);
+  for (int q4 = 0; q4 < 2 * B.kbk; ++q4) {
+    _Float16 hv[8], lv[8];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int hz = 4 * q4 + e;
+      float c = 0.f, s = 0.f;
+      if (hz <= B.hmax) {
+        double sn, cs;
+        sincospi(2.0 * (double)hz * sz, &sn, &cs);
+        c = (float)cs;
+        s = (float)sn;
+      }
+      split_f16(c, hv[2 * e], lv[2 * e]);
+      split_f16(s, hv[2 * e + 1], lv[2 * e + 1]);
+    }
+    u4_t* dst = wb + ((size_t)q4 * N + i) * 2;
+    dst[0] = pack_h8(hv);
+    dst[1] = pack_h8(lv);
+  }
+}
+
+// ----------------------------------------------------------------------
+// A operands of one tile (8 columns), one wave: lane l packs row l % 32 of
+// every K block; rows are (column 2 (r/8) + (r/4) % 2, sum r % 4) so that
+// the result layout gives each lane all four sums of its columns.  Also
+// the per-column epilogue weights hx/sU, hy/sU, 1/sV.
+// ----------------------------------------------------------------------
+__global__ __launch_bounds__(kWave) void k_kf_pack(const BoxDesc* __restrict__ boxes,
+                                                   const Item* __restrict__ items) {
+  const Item it = items[blockIdx.x];
+  const BoxDesc& B = boxes[it.box];
+  const int t = it.a, l = threadIdx.x, r = l & 31, g = l >> 5;
+  const KfTile tl = B.kft[t];
+  const int ct = 2 * (r >> 3) + ((r >> 2) & 1), q = r & 3;
+  const bool cv = ct < tl.nc;
+  KCol col = {0, 0, 0, 0, 0, {0, 0, 0}};
+  if (cv) col = B.cols[tl.c0 + ct];
+  // column scales: largest |T| and |hz T| to [2^10, 2^11)
+  double mu = 0.0, mv = 0.0;
+  for (int j = 0; j < col.len; ++j) {
+    const double* w = B.kab + 4 * (size_t)B.kslot[col.k0 + j];
+    mu = fmax(mu, fmax(fabs(w[0]), fabs(w[1])));
+    mv = fmax(mv, fmax(fabs(w[2]), fabs(w[3])));
+  }
+  int eu = 0, ev = 0;
+  (void)frexp(mu, &eu);
+  (void)frexp(mv, &ev);
+  const double su = mu > 0.0 ? ldexp(1.0, 11 - eu) : 1.0;
+  const double sv = mv > 0.0 ? ldexp(1.0, 11 - ev) : 1.0;
+  const bool isv = q >= 2;
+  const double sc = isv ? sv : su;
+  u4_t* kfa = reinterpret_cast<u4_t*>(B.kfa);
+  for (int kb = 0; kb < B.kbk; ++kb) {
+    _Float16 hv[8], lv[8];
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      const int hz = 8 * kb + 4 * g + (e >> 1), p = e & 1;
+      float v = 0.f;
+      if (cv && hz >= col.hz0 && hz < col.hz0 + col.len) {
+        const double* w = B.kab + 4 * (size_t)B.kslot[col.k0 + hz - col.hz0] + (isv ? 2 : 0);
+        const double tr = w[0] * sc, ti = w[1] * sc;
+        // Re rows take (Tr, -Ti) against (Re W, Im W); Im rows (Ti, Tr)
+        v = (float)((q & 1) == 0 ? (p == 0 ? tr : -ti) : (p == 0 ? ti : tr));
+      }
+      split_f16(v, hv[e], lv[e]);
+    }
+    u4_t* dst = kfa + ((size_t)t * B.kbk + kb) * 2 * kWave + l;
+    dst[0] = pack_h8(hv);
+    dst[kWave] = pack_h8(lv);
+  }
+  if (cv && q == 0 && g == 0) {
+    const int bits = (col.hx & 0xffff) | (col.hy << 16);
+    B.colw[tl.c0 + ct] = make_float4((float)(col.hx / su), (float)(col.hy / su),
+                                     (float)(1.0 / sv), __int_as_float(bits));
+  }
+}
+
+// ----------------------------------------------------------------------
+// Back-projection.  Each wave owns NB blocks of 32 ions, two lanes per ion
+// (one per half of every tile's columns), and a run of tiles; the ions' W
+// operands stay in registers.  Per tile the column data come through the
+// scalar cache and every vector load of the tile (the packed weights of
+// its K blocks, X[hx] and Y[hy] of the lane's columns) is issued before
+// the MFMAs, so the loads of a tile overlap one another and the matrix
+// work.  Per K block and ion block three MFMAs, then G = Re(Q U) and
+// Gz = Re(Q V) for the lane's 4 columns with Q = X[hx] Y[hy].  Tile sums in
+// FP32, sums over tiles in FP64.
+// ----------------------------------------------------------------------
+template <int KB, int NB>
+__global__ __launch_bounds__(kKfMfmaWaves* kWave) void k_kforce_mfma(
+    const BoxDesc* __restrict__ boxes, const Item* __restrict__ items) {
+  const Item it = items[blockIdx.x];
+  const BoxDesc& B = boxes[it.box];
+  const int N = B.N, kbk = B.kbk;
+  const int w = threadIdx.x / kWave, l = threadIdx.x % kWave, n = l & 31, g = l >> 5;
+  const int base = it.a + 32 * NB * w;
+  if (base >= N) return;  // whole wave past the box's ions (no barriers here)
+  const u4_t* wb = reinterpret_cast<const u4_t*>(B.wb);
+  h8_t bh[NB][KB], bl[NB][KB];
+  int ii[NB];
+#pragma unroll
+  for (int q = 0; q < NB; ++q) {
+    const int i = base + 32 * q + n;
+    ii[q] = i < N ? i : 0;
+#pragma unroll
+    for (int kb = 0; kb < KB; ++kb) {
+      u4_t x = {0u, 0u, 0u, 0u}, y = {0u, 0u, 0u, 0u};
+      if (i < N && kb < kbk) {
+        const u4_t* p = wb + ((size_t)(2 * kb + g) * N + i) * 2;
+        x = p[0];
+        y = p[1];
+      }
+      bh[q][kb] = __builtin_bit_cast(h8_t, x);
+      bl[q][kb] = __builtin_bit_cast(h8_t, y);
+    }
+  }
+  const u4_t* kfa = reinterpret_cast<const u4_t*>(B.kfa);
+  const cptr<KfTile> tiles = as_const(B.kft);
+  const cptr<float> colw = as_const(reinterpret_cast<const float*>(B.colw));
+  const float2* X = reinterpret_cast<const float2*>(B.tabx);
+  const float2* Y = reinterpret_cast<const float2*>(B.taby);
+  double F[NB][3];
+#pragma unroll
+  for (int q = 0; q < NB; ++q) F[q][0] = F[q][1] = F[q][2] = 0.0;
+  const int tend = it.b + it.c;
+  for (int t = it.b; t < tend; ++t) {
+    const int c0 = tiles[t].c0, nc = tiles[t].nc, klo = tiles[t].kb_lo, khi = tiles[t].kb_hi;
+    // the lane's 4 columns: scalar loads of both halves' data, then select
+    float4 cw[4];
+    int hx[4], hy[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int ca = min(c0 + 2 * j, c0 + nc - 1), cb = min(c0 + 2 * j + 1, c0 + nc - 1);
+      const float4 wa = make_float4(colw[4 * ca], colw[4 * ca + 1], colw[4 * ca + 2], colw[4 * ca + 3]);
+      const float4 wv = make_float4(colw[4 * cb], colw[4 * cb + 1], colw[4 * cb + 2], colw[4 * cb + 3]);
+      cw[j] = g ? wv : wa;
+      const int bits = __float_as_int(cw[j].w);
+      hx[j] = (int)(short)(bits & 0xffff);
+      hy[j] = bits >> 16;
+    }
+    // vector loads of the tile, all issued before the matrix work
+    u4_t ar[KB][2];
+#pragma unroll
+    for (int kb = 0; kb < KB; ++kb) {
+      ar[kb][0] = u4_t{0u, 0u, 0u, 0u};
+      ar[kb][1] = u4_t{0u, 0u, 0u, 0u};
+      if (kb >= klo && kb < khi) {
+        const u4_t* ap = kfa + ((size_t)t * kbk + kb) * 2 * kWave + l;
+        ar[kb][0] = ap[0];
+        ar[kb][1] = ap[kWave];
+      }
+    }
+    float2 xv[NB][4], yv[NB][4];
+#pragma unroll
+    for (int q = 0; q < NB; ++q)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        xv[q][j] = X[(size_t)(hx[j] < 0 ? -hx[j] : hx[j]) * N + ii[q]];
+        yv[q][j] = Y[(size_t)(hy[j] < 0 ? -hy[j] : hy[j]) * N + ii[q]];
+      }
+    f16_t acc[NB];
+#pragma unroll
+    for (int q = 0; q < NB; ++q)
+#pragma unroll
+      for (int v = 0; v < 16; ++v) acc[q][v] = 0.f;
+#pragma unroll
+    for (int kb = 0; kb < KB; ++kb) {
+      if (kb >= klo && kb < khi) {
+        const h8_t ah = __builtin_bit_cast(h8_t, ar[kb][0]);
+        const h8_t al = __builtin_bit_cast(h8_t, ar[kb][1]);
+#pragma unroll
+        for (int q = 0; q < NB; ++q)
+          acc[q] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, bh[q][kb], acc[q], 0, 0, 0);
+#pragma unroll
+        for (int q = 0; q < NB; ++q)
+          acc[q] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, bl[q][kb], acc[q], 0, 0, 0);
+#pragma unroll
+        for (int q = 0; q < NB; ++q)
+          acc[q] = __builtin_amdgcn_mfma_f32_32x32x16_f16(al, bh[q][kb], acc[q], 0, 0, 0);
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < NB; ++q) {
+      float fx = 0.f, fy = 0.f, fz = 0.f;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        if (2 * j + g < nc) {
+          float2 x = xv[q][j], y = yv[q][j];
+          if (hx[j] < 0) x.y = -x.y;
+          if (hy[j] < 0) y.y = -y.y;
+          const float qr = x.x * y.x - x.y * y.y, qi = x.x * y.y + x.y * y.x;
+          const float G = qr * acc[q][4 * j] - qi * acc[q][4 * j + 1];
+          const float Gz = qr * acc[q][4 * j + 2] - qi * acc[q][4 * j + 3];
+          fx += cw[j].x * G;
+          fy += cw[j].y * G;
+          fz += cw[j].z * Gz;
+        }
+      }
+      F[q][0] += (double)fx;
+      F[q][1] += (double)fy;
+      F[q][2] += (double)fz;
+    }
+  }
+  // the two lanes of an ion hold the two halves of every tile's columns
+#pragma unroll
+  for (int q = 0; q < NB; ++q) {
+    const double fx = F[q][0] + __shfl_xor(F[q][0], 32);
+    const double fy = F[q][1] + __shfl_xor(F[q][1], 32);
+    const double fz = F[q][2] + __shfl_xor(F[q][2], 32);
+    const int i = base + 32 * q + n;
+    if (g == 0 && i < N) {
+      const double zq = B.Z[B.typ[i]];
+      double* fp = B.fpart + (size_t)it.d * 3 * N;
+      fp[i] = -zq * fx * B.invL[0];
+      fp[N + i] = -zq * fy * B.invL[1];
+      fp[2 * N + i] = -zq * fz * B.invL[2];
+    }
+  }
+}
+
+// ion blocks per wave for a K-block count: one (measured faster than two,
+// which halves occupancy)
+constexpr int kf_blocks_for(int kb) { return kb > 0 ? 1 : 1; }
+
+}  // namespace mdg

@@ -29,7 +29,13 @@ void launch_moment_correction(const BoxDesc* boxes, const int* ids, int n, hipSt
 void launch_real32(bool analytic, const BoxDesc* boxes, const Item* items, int n, int mode,
                    hipStream_t s);
 void launch_sk32(const BoxDesc* boxes, const Item* items, int n, int hs, int hsw, hipStream_t s);
-void launch_kforce32(int hz, const BoxDesc* boxes, const Item* items, int n, hipStream_t s);
+// MFMA back-projection: FP16 hi/lo W operands, packed weight tiles, then
+// the tiles against 32-ion blocks; kb = K blocks (8 hz) of the widest box
+void launch_wb_tables(const BoxDesc* boxes, const Item* items, int n, hipStream_t s);
+void launch_kf_pack(const BoxDesc* boxes, const Item* items, int n, hipStream_t s);
+void launch_kforce_mfma(int kb, const BoxDesc* boxes, const Item* items, int n, hipStream_t s);
+// ions per back-projection workgroup for kb K blocks
+int kforce_mfma_ions(int kb);
 
 // register row lengths the back-projection is instantiated for
 int kforce_hz_for(int hmax);

@@ -57,6 +57,15 @@ struct KQuad {
   int lo, nrow, row0, ncol;
 };
 
+// Eight consecutive k columns back-projected by one MFMA tile (mixed
+// path): columns c0 .. c0+nc-1, their weights nonzero only in K blocks
+// (8 hz each) kb_lo .. kb_hi-1.
+struct KfTile {
+  int c0, nc, kb_lo, kb_hi;
+};
+constexpr int kKfTileCols = 8;
+constexpr int kKfMfmaWaves = 4;     // waves per back-projection workgroup
+
 // Work item of a launch over many boxes.
 struct Item {
   int box;   // index into the BoxDesc array of the launch
@@ -121,6 +130,17 @@ struct BoxDesc {
   double *kpot;       // [nk] energy terms
   double *fpart;      // [ngroup][3][N] back-projection partial forces
   int ngroup;
+  // ---- back-projection on MFMA (mixed path) ----
+  // W[hz] of each ion as FP16 hi + lo pairs in the B-operand order of
+  // v_mfma_f32_32x32x16_f16: [hz/4][N] records of 8 hi then 8 lo halves
+  // (re, im of 4 consecutive hz), hz < 8 * kbk, zero beyond hmax.
+  void *wb;
+  // Packed A operands per tile and K block: [ntile][kbk][hi, lo][64 lanes]
+  // 16-byte lane records; rows are (column, {Re U, Im U, Re V, Im V}).
+  void *kfa;
+  float4 *colw;       // per column: hx/sU, hy/sU, 1/sV, (hx, hy) as packed int16
+  const KfTile *kft;  // [ntile]
+  int kbk, ntile;
   // ---- reductions ----
   double *rpart;      // [nreal][4]: pot, vir x, y, z per real-space item
   unsigned long long *rpairs;  // [nreal] accepted unordered pairs per item
