@@ -139,6 +139,7 @@ struct HomeCtx {
   int i, ti;
   bool valid;
   double xi, yi, zi;
+  float xf, yf, zf;       // screening coordinates (see real_tile)
   const double2* t64[4];  // table rows against partner species 0..3
   const float2* t32[4];
   double Zi;
@@ -150,6 +151,8 @@ struct BoxCtx {
   double hx, hy, hz, Lx, Ly, Lz, rc2;
   int nb;
   float rc2f, alpha, two_a3, inv_rho, inv_rho_l2e, inv_rho_l2e_lo;
+  float hxf, hyf, hzf, Lxf, Lyf, Lzf;
+  float rc2m;  // screening cutoff: rc^2 plus a margin above FP32 rounding
 };
 
 template <int KIND>
@@ -232,51 +235,49 @@ __device__ __forceinline__ void pair64(const BoxDesc& B, int t, double ZZ, doubl
   f = fe + fr;
 }
 
-// Partners u in [0, cnt) of one LDS tile, in order.
-template <int KIND>
+// One LDS tile of partners u in [0, cnt), in two phases.
+//  1. Screen: every lane tests all partners in FP32 (screening coordinates
+//     jf / h.xf, a cutoff with a margin larger than their rounding) and
+//     keeps a 64-bit mask of candidates.  Partners are read as broadcasts.
+//  2. Evaluate: each lane pops its own candidates in increasing u (so a
+//     lane's force is still summed in partner order) and applies the exact
+//     FP64 fold and cutoff of the unscreened code; lanes work on different
+//     partners at once instead of idling while one lane's pair runs.
+// The accepted pairs and the per-ion summation order are those of a plain
+// loop over all partners.  FOLD32: screening coordinates are absolute and
+// need the minimum-image fold (all-pairs path); cell-list partners come
+// pre-shifted to the home cell's image, relative to its corner.
+template <int KIND, bool FOLD32>
 __device__ __forceinline__ void real_tile(const BoxDesc& B, const BoxCtx& c, const HomeCtx& h,
                                           const double* jx, const double* jy, const double* jz,
-                                          const int* jt, const int* jid, int cnt, RealAcc& a,
-                                          int& err) {
-  if constexpr (KIND == kRealTable64) {
-    // 8 partners at a time: the table gathers are issued together, the
-    // accumulation stays in partner order
-    int u = 0;
-    for (; u + 8 <= cnt; u += 8) {
-      double dx[8], dy[8], dz[8];
-      bool ok[8];
-      double2 pf[8];
-#pragma unroll
-      for (int q = 0; q < 8; ++q) {
-        dx[q] = h.xi - jx[u + q]; dy[q] = h.yi - jy[u + q]; dz[q] = h.zi - jz[u + q];
-        fold(c, dx[q], dy[q], dz[q]);
-        const double dr = dx[q] * dx[q] + dy[q] * dy[q] + dz[q] * dz[q];  // unfused, as mdc
-        ok[q] = h.valid && jid[u + q] != h.i && dr <= c.rc2;
-        int bin = (int)((sqrt(dr) - 0.5) * 1000.0);  // truncating, real.c:62
-        if (ok[q] && (bin < 0 || bin >= c.nb)) { err |= kErrRange; ok[q] = false; }
-        bin = ok[q] ? bin : 0;
-        const int s = jt[u + q];
-        const double2* row = s == 0 ? h.t64[0] : (s == 1 ? h.t64[1] : (s == 2 ? h.t64[2] : h.t64[3]));
-        pf[q] = row[bin];
+                                          const float4* jf, const int* jt, const int* jid,
+                                          int cnt, RealAcc& a, int& err) {
+  unsigned long long m = 0ull;
+  if (h.valid) {
+    for (int u = 0; u < cnt; ++u) {
+      const float4 p = jf[u];
+      float dx = h.xf - p.x, dy = h.yf - p.y, dz = h.zf - p.z;
+      if constexpr (FOLD32) {
+        dx = dx > c.hxf ? dx - c.Lxf : (dx < -c.hxf ? dx + c.Lxf : dx);
+        dy = dy > c.hyf ? dy - c.Lyf : (dy < -c.hyf ? dy + c.Lyf : dy);
+        dz = dz > c.hzf ? dz - c.Lzf : (dz < -c.hzf ? dz + c.Lzf : dz);
       }
-#pragma unroll
-      for (int q = 0; q < 8; ++q) {
-        if (!ok[q]) continue;
-        a.fx += pf[q].y * dx[q]; a.fy += pf[q].y * dy[q]; a.fz += pf[q].y * dz[q];
-        if (jid[u + q] > h.i) {
-          a.pot += pf[q].x;
-          a.vx += pf[q].y * dx[q] * dx[q]; a.vy += pf[q].y * dy[q] * dy[q];
-          a.vz += pf[q].y * dz[q] * dz[q];
-          ++a.np;
-        }
-      }
+      const float dr = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+      if (dr <= c.rc2m) m |= 1ull << u;
     }
-    for (; u < cnt; ++u) {
-      double dx = h.xi - jx[u], dy = h.yi - jy[u], dz = h.zi - jz[u];
-      fold(c, dx, dy, dz);
-      const double dr = dx * dx + dy * dy + dz * dz;
-      if (!h.valid || jid[u] == h.i || !(dr <= c.rc2)) continue;
-      const int bin = (int)((sqrt(dr) - 0.5) * 1000.0);
+  }
+  float ffx = 0.f, ffy = 0.f, ffz = 0.f, fpot = 0.f, fvx = 0.f, fvy = 0.f, fvz = 0.f;
+  while (__ballot(m != 0ull)) {
+    if (m == 0ull) continue;
+    const int u = __builtin_ctzll(m);
+    m &= m - 1ull;
+    if (jid[u] == h.i) continue;
+    double dx = h.xi - jx[u], dy = h.yi - jy[u], dz = h.zi - jz[u];
+    fold(c, dx, dy, dz);
+    if constexpr (KIND == kRealTable64) {
+      const double dr = dx * dx + dy * dy + dz * dz;  // unfused, as mdc
+      if (!(dr <= c.rc2)) continue;
+      const int bin = (int)((sqrt(dr) - 0.5) * 1000.0);  // truncating, real.c:62
       if (bin < 0 || bin >= c.nb) { err |= kErrRange; continue; }
       const int s = jt[u];
       const double2* row = s == 0 ? h.t64[0] : (s == 1 ? h.t64[1] : (s == 2 ? h.t64[2] : h.t64[3]));
@@ -287,13 +288,9 @@ __device__ __forceinline__ void real_tile(const BoxDesc& B, const BoxCtx& c, con
         a.vx += pf.y * dx * dx; a.vy += pf.y * dy * dy; a.vz += pf.y * dz * dz;
         ++a.np;
       }
-    }
-  } else if constexpr (KIND == kRealAnalytic64) {
-    for (int u = 0; u < cnt; ++u) {
-      double dx = h.xi - jx[u], dy = h.yi - jy[u], dz = h.zi - jz[u];
-      fold(c, dx, dy, dz);
+    } else if constexpr (KIND == kRealAnalytic64) {
       const double dr = dx * dx + dy * dy + dz * dz;
-      if (!h.valid || jid[u] == h.i || !(dr <= c.rc2)) continue;
+      if (!(dr <= c.rc2)) continue;
       const int s = jt[u];
       const int t = s == 0 ? h.ta[0] : (s == 1 ? h.ta[1] : (s == 2 ? h.ta[2] : h.ta[3]));
       double p, f;
@@ -304,32 +301,26 @@ __device__ __forceinline__ void real_tile(const BoxDesc& B, const BoxCtx& c, con
         a.vx += f * dx * dx; a.vy += f * dy * dy; a.vz += f * dz * dz;
         ++a.np;
       }
-    }
-  } else {
-    // FP32 pair arithmetic; the fold stays FP64 (positions are FP64).
-    // Tile sums (at most 64 pairs) in FP32, running sums in FP64.
-    float ffx = 0.f, ffy = 0.f, ffz = 0.f, fpot = 0.f, fvx = 0.f, fvy = 0.f, fvz = 0.f;
-    for (int u = 0; u < cnt; ++u) {
-      double dx = h.xi - jx[u], dy = h.yi - jy[u], dz = h.zi - jz[u];
-      fold(c, dx, dy, dz);
-      float p, f, fdx, fdy, fdz;
+    } else {
+      // FP32 pair arithmetic; the fold and the cutoff test stay FP64.
+      // Tile sums (at most 64 pairs) in FP32, running sums in FP64.
+      float p, f;
       if constexpr (KIND == kRealAnalytic32) {
         // the FP64 cutoff test accepts exactly the FP64 path's pairs
         const double dr = fma(dx, dx, fma(dy, dy, dz * dz));
-        if (!h.valid || jid[u] == h.i || !(dr <= c.rc2)) continue;
-        fdx = (float)dx; fdy = (float)dy; fdz = (float)dz;
+        if (!(dr <= c.rc2)) continue;
         pair32(c, jt[u] == 0 ? h.row[0] : h.row[1], dr, p, f);
       } else {  // kRealTable32: FP64 bin as mdc, FP32 table values
         const double dr = dx * dx + dy * dy + dz * dz;
-        if (!h.valid || jid[u] == h.i || !(dr <= c.rc2)) continue;
+        if (!(dr <= c.rc2)) continue;
         const int bin = (int)((sqrt(dr) - 0.5) * 1000.0);
         if (bin < 0 || bin >= c.nb) { err |= kErrRange; continue; }
         const int s = jt[u];
         const float2* row = s == 0 ? h.t32[0] : (s == 1 ? h.t32[1] : (s == 2 ? h.t32[2] : h.t32[3]));
         const float2 pf = row[bin];
         p = pf.x; f = pf.y;
-        fdx = (float)dx; fdy = (float)dy; fdz = (float)dz;
       }
+      const float fdx = (float)dx, fdy = (float)dy, fdz = (float)dz;
       ffx = fmaf(f, fdx, ffx); ffy = fmaf(f, fdy, ffy); ffz = fmaf(f, fdz, ffz);
       if (jid[u] > h.i) {
         fpot += p;
@@ -337,6 +328,8 @@ __device__ __forceinline__ void real_tile(const BoxDesc& B, const BoxCtx& c, con
         ++a.np;
       }
     }
+  }
+  if constexpr (KIND == kRealAnalytic32 || KIND == kRealTable32) {
     a.fx += ffx; a.fy += ffy; a.fz += ffz; a.pot += fpot;
     a.vx += fvx; a.vy += fvy; a.vz += fvz;
   }
@@ -362,6 +355,7 @@ __global__ __launch_bounds__(kRealThreads) void k_real(const BoxDesc* __restrict
                                                        const Item* __restrict__ items, int mode) {
   __shared__ double jx[kRealThreads], jy[kRealThreads], jz[kRealThreads];
   __shared__ int jt[kRealThreads], jid[kRealThreads];
+  __shared__ float4 jf[kRealThreads];
   const Item it = items[blockIdx.x];
   const BoxDesc& B = boxes[it.box];
   const int lane = threadIdx.x;
@@ -377,6 +371,11 @@ __global__ __launch_bounds__(kRealThreads) void k_real(const BoxDesc* __restrict
   const double irl = 1.4426950408889634 / B.rho;  // log2(e) / rho, split in two floats
   c.inv_rho_l2e = (float)irl;
   c.inv_rho_l2e_lo = (float)(irl - (double)c.inv_rho_l2e);
+  c.hxf = (float)c.hx; c.hyf = (float)c.hy; c.hzf = (float)c.hz;
+  c.Lxf = (float)c.Lx; c.Lyf = (float)c.Ly; c.Lzf = (float)c.Lz;
+  // FP32 screening coordinates are within ~1e-5 A: a relative margin of
+  // 1e-4 on rc^2 (plus 1e-3 A^2) keeps every pair the exact test accepts
+  c.rc2m = (float)(B.rc2 * 1.0001 + 1e-3);
   RealAcc tot;  // pot / virial / pair count of the item
   if (B.all_pairs) {
     const int i = it.a + lane;
@@ -384,6 +383,7 @@ __global__ __launch_bounds__(kRealThreads) void k_real(const BoxDesc* __restrict
     const int ii = valid ? i : 0;
     HomeCtx h;
     make_home<KIND>(B, i, B.typ[ii], valid, B.rx[ii], B.ry[ii], B.rz[ii], h);
+    h.xf = (float)h.xi; h.yf = (float)h.yi; h.zf = (float)h.zi;
     const int js = (int)((long)split * N / nsplit), je = (int)((long)(split + 1) * N / nsplit);
     RealAcc a;
     for (int j0 = js; j0 < je; j0 += kRealThreads) {
@@ -391,9 +391,11 @@ __global__ __launch_bounds__(kRealThreads) void k_real(const BoxDesc* __restrict
       if (j < je) {
         jx[lane] = B.rx[j]; jy[lane] = B.ry[j]; jz[lane] = B.rz[j];
         jt[lane] = B.typ[j]; jid[lane] = j;
+        jf[lane] = make_float4((float)B.rx[j], (float)B.ry[j], (float)B.rz[j], 0.f);
       }
       __syncthreads();
-      real_tile<KIND>(B, c, h, jx, jy, jz, jt, jid, min(kRealThreads, je - j0), a, err);
+      real_tile<KIND, true>(B, c, h, jx, jy, jz, jf, jt, jid, min(kRealThreads, je - j0), a,
+                            err);
       __syncthreads();
     }
     real_store(B, h, split, mode, a);
@@ -410,6 +412,9 @@ __global__ __launch_bounds__(kRealThreads) void k_real(const BoxDesc* __restrict
       const int hq = valid ? hp : hs;
       HomeCtx h;
       make_home<KIND>(B, B.sorted[hq], B.styp[hq], valid, B.sx[hq], B.sy[hq], B.sz[hq], h);
+      // screening coordinates relative to the home cell's corner
+      const double ox0 = cx * (B.L[0] / ncx), oy0 = cy * (B.L[1] / ncy), oz0 = cz * (B.L[2] / ncz);
+      h.xf = (float)(h.xi - ox0); h.yf = (float)(h.yi - oy0); h.zf = (float)(h.zi - oz0);
       RealAcc a;
       for (int n = n0; n < n1; ++n) {
         const int oz = n / 9 - 1, oy = (n / 3) % 3 - 1, ox = n % 3 - 1;
@@ -417,14 +422,21 @@ __global__ __launch_bounds__(kRealThreads) void k_real(const BoxDesc* __restrict
                   nxc = (cx + ox + ncx) % ncx;
         const int nb = (nzc * ncy + nyc) * ncx + nxc;
         const int ns = B.cell_start[nb], ne = B.cell_start[nb + 1];
+        // the neighbour's image next to the home cell, relative to its corner
+        const double shx = (cx + ox < 0 ? -B.L[0] : (cx + ox >= ncx ? B.L[0] : 0.0)) - ox0;
+        const double shy = (cy + oy < 0 ? -B.L[1] : (cy + oy >= ncy ? B.L[1] : 0.0)) - oy0;
+        const double shz = (cz + oz < 0 ? -B.L[2] : (cz + oz >= ncz ? B.L[2] : 0.0)) - oz0;
         for (int j0 = ns; j0 < ne; j0 += kRealThreads) {
           const int jp = j0 + lane;
           if (jp < ne) {
             jx[lane] = B.sx[jp]; jy[lane] = B.sy[jp]; jz[lane] = B.sz[jp];
             jt[lane] = B.styp[jp]; jid[lane] = B.sorted[jp];
+            jf[lane] = make_float4((float)(B.sx[jp] + shx), (float)(B.sy[jp] + shy),
+                                   (float)(B.sz[jp] + shz), 0.f);
           }
           __syncthreads();
-          real_tile<KIND>(B, c, h, jx, jy, jz, jt, jid, min(kRealThreads, ne - j0), a, err);
+          real_tile<KIND, false>(B, c, h, jx, jy, jz, jf, jt, jid, min(kRealThreads, ne - j0), a,
+                                 err);
           __syncthreads();
         }
       }
