@@ -142,6 +142,9 @@ static Tiling finish_tiling(const BoxPlan& P, bool can_cells, Tiling t) {
     int nch = std::max(1, std::min(t.nchunk, 64));
     nch = std::max(1, std::min(nch, ceil_div(N, kSkTile)));
     nch = std::max(1, std::min(nch, (int)((64l << 20) / (16l * std::max(1, P.nk)))));
+    // mixed path: its structure-factor partials are FP32 sums over one
+    // chunk, so chunks hold at most kSkmChunk ions
+    if (!P.kftiles.empty()) nch = std::max(nch, ceil_div(N, kSkmChunk));
     t.chunk = ceil_div(ceil_div(N, nch), kSkTile) * kSkTile;
     t.nchunk = ceil_div(N, t.chunk);
     t.ngroup = std::max(1, std::min(std::min(t.ngroup, 64), P.ncol));
@@ -338,7 +341,7 @@ int make_plan(const BoxSpec& sp, int precision, BoxPlan* P, std::string* why) {
 struct Layout {
   size_t ion[12], typ, rpf, cell_of, cnt, cwave, start, sorted, s[3], styp;
   size_t cols, segs, quads, khz, kslot, kce, kcf, kab, kab32, kpot, tx, ty, tw, skp, fp;
-  size_t wb, kfa, colw, kft;
+  size_t wb, kfa, colw, kft, sb;
   size_t tab, tab32, rpart, rpairs, scal, err, total;
 };
 
@@ -385,6 +388,8 @@ static Layout layout(const BoxPlan& P, int precision) {
       L.kfa = take(off, 16 * (size_t)kWave * 2 * P.kbk * nt);
       L.colw = take(off, sizeof(float4) * (size_t)P.ncol);
       L.kft = take(off, sizeof(KfTile) * nt);
+      const size_t npad = (size_t)(N + 7) / 8 * 8, hp = 16 * (size_t)((P.hmax + 16) / 16);
+      L.sb = take(off, 16 * npad * hp);
     }
     L.kpot = take(off, sizeof(double) * P.nk);
     L.tx = take(off, cplx * H * N);
@@ -521,6 +526,9 @@ int box_create(const BoxSpec& spec, int precision, Box** out, std::string* why) 
       D.kft = (const KfTile*)(base + L.kft);
       D.kbk = P.kbk;
       D.ntile = (int)P.kftiles.size();
+      D.sb = base + L.sb;
+      D.sknt = (P.hmax + 16) / 16;
+      D.npad = (N + 7) / 8 * 8;
       up(L.kft, P.kftiles.data(), sizeof(KfTile) * P.kftiles.size());
     }
     D.kpot = (double*)(base + L.kpot);
@@ -664,7 +672,8 @@ int BoxSet::set_boxes(const std::vector<Box*>& boxes) {
   // big boxes' items first, so the longest tiles start earliest
   std::stable_sort(order.begin(), order.end(),
                    [&](int a, int b) { return boxes_[a]->desc.N > boxes_[b]->desc.N; });
-  std::vector<Item> real, tab, sk, fin, kf, ion, pack;
+  std::vector<Item> real, tab, sk, fin, kf, ion, pack, sbt;
+  skn_ = 1;
   std::vector<int> cellid, allid;
   int hmax = 0;
   const bool mixed = precision_ == MDG_MIXED;
@@ -693,9 +702,22 @@ int BoxSet::set_boxes(const std::vector<Box*>& boxes) {
     if (P.ewald) {
       hmax = std::max(hmax, P.hmax);
       for (int i0 = 0; i0 < N; i0 += kIonThreads) tab.push_back({bi, i0, 0, 0, 0});
-      for (int ch = 0; ch < P.tile.nchunk; ++ch)
-        for (int s0 = 0; s0 < P.nseg; s0 += kSkThreads)
-          sk.push_back({bi, ch, s0, std::min(kSkThreads, P.nseg - s0), 0});
+      if (mixed) {
+        // MFMA structure factor: 4 waves x 32 columns per item
+        // (two hz tiles of 16 per item)
+        const int cw = 32 * kKfMfmaWaves;
+        const int nt = (P.hmax + 16) / 16, npad = (N + 7) / 8 * 8;
+        for (int ch = 0; ch < P.tile.nchunk; ++ch)
+          for (int c0 = 0; c0 < P.ncol; c0 += cw)
+            for (int t0 = 0; t0 < nt; t0 += 2) sk.push_back({bi, ch, c0, t0, 0});
+        const int nrec = npad / 4 * 16 * nt;
+        for (int e0 = 0; e0 < nrec; e0 += kIonThreads) sbt.push_back({bi, e0, 0, 0, 0});
+        skn_ = std::max(skn_, P.hmax + 1);
+      } else {
+        for (int ch = 0; ch < P.tile.nchunk; ++ch)
+          for (int s0 = 0; s0 < P.nseg; s0 += kSkThreads)
+            sk.push_back({bi, ch, s0, std::min(kSkThreads, P.nseg - s0), 0});
+      }
       for (int k0 = 0; k0 < P.nk; k0 += kIonThreads) fin.push_back({bi, k0, 0, 0, 0});
       const int gk = (int)P.kf_group_start.size() - 1;
       const int ion_step = mixed ? kf_ions : kKfThreads;
@@ -724,6 +746,7 @@ int BoxSet::set_boxes(const std::vector<Box*>& boxes) {
   app(kf, off_kf_, n_kf_);
   app(ion, off_ion_, n_ion_);
   app(pack, off_pack_, n_pack_);
+  app(sbt, off_sbt_, n_sbt_);
   std::vector<int> ids;
   off_cellid_ = 0;
   n_cellid_ = (int)cellid.size();
@@ -785,7 +808,8 @@ void BoxSet::launch_recip(hipStream_t s) {
   launch_phase_tables(mixed, d_desc_, d_items_ + off_tab_, n_tab_, s);
   if (mixed) {
     launch_wb_tables(d_desc_, d_items_ + off_tab_, n_tab_, s);
-    launch_sk32(d_desc_, d_items_ + off_sk_, n_sk_, hs_, hsw_, s);
+    launch_sb_tables(d_desc_, d_items_ + off_sbt_, n_sbt_, s);
+    launch_sk_mfma(skn_, d_desc_, d_items_ + off_sk_, n_sk_, s);
     launch_sk_finalize(d_desc_, d_items_ + off_fin_, n_fin_, s);
     launch_kf_pack(d_desc_, d_items_ + off_pack_, n_pack_, s);
     launch_kforce_mfma(hz_ / 8, d_desc_, d_items_ + off_kf_, n_kf_, s);
@@ -872,10 +896,10 @@ int BoxSet::profile(int nsteps, hipStream_t s, std::vector<std::string>* names,
   int rc = sync_desc(s);
   if (rc != MDG_OK) return rc;
   const bool mixed = precision_ == MDG_MIXED;
-  constexpr int K = 10;
+  constexpr int K = 11;
   const char* kn[K] = {"cell_build", "phase_tables", "real_space", "sk_partial",
                        "sk_finalize", "kforce",      "force_vv",   "box_scalars",
-                       "wb_tables",  "kf_pack"};
+                       "wb_tables",  "kf_pack",      "sb_tables"};
   std::vector<hipEvent_t> ev(2 * K * std::max(1, nsteps));
   for (auto& e : ev) MDG_HIP(hipEventCreate(&e));
   std::vector<int> used(K, 0), which;
@@ -894,8 +918,9 @@ int BoxSet::profile(int nsteps, hipStream_t s, std::vector<std::string>* names,
     if (n_sk_) {
       rec(1, [&] { launch_phase_tables(mixed, d_desc_, d_items_ + off_tab_, n_tab_, s); });
       if (mixed) rec(8, [&] { launch_wb_tables(d_desc_, d_items_ + off_tab_, n_tab_, s); });
+      if (mixed) rec(10, [&] { launch_sb_tables(d_desc_, d_items_ + off_sbt_, n_sbt_, s); });
       rec(3, [&] {
-        if (mixed) launch_sk32(d_desc_, d_items_ + off_sk_, n_sk_, hs_, hsw_, s);
+        if (mixed) launch_sk_mfma(skn_, d_desc_, d_items_ + off_sk_, n_sk_, s);
         else launch_sk64(d_desc_, d_items_ + off_sk_, n_sk_, hs_, hsw_, s);
       });
       rec(4, [&] { launch_sk_finalize(d_desc_, d_items_ + off_fin_, n_fin_, s); });

@@ -145,7 +145,8 @@ class BoxSet {
   int off_real_ = 0, n_real_ = 0, off_tab_ = 0, n_tab_ = 0, off_sk_ = 0, n_sk_ = 0;
   int off_fin_ = 0, n_fin_ = 0, off_kf_ = 0, n_kf_ = 0, off_ion_ = 0, n_ion_ = 0;
   int off_cellid_ = 0, n_cellid_ = 0, off_allid_ = 0, n_allid_ = 0;
-  int off_pack_ = 0, n_pack_ = 0;
+  int off_pack_ = 0, n_pack_ = 0, off_sbt_ = 0, n_sbt_ = 0;
+  int skn_ = 1;  // hmax + 1 of the widest box (MFMA structure factor LDS tiles)
   int hs_ = 1, hsw_ = 8, hz_ = 8;  // LDS strides / register row of the k-space kernels
   hipGraphExec_t graph_exec_ = nullptr;
   hipGraph_t graph_ = nullptr;

@@ -20,12 +20,6 @@ void launch_real32(bool analytic, const BoxDesc* boxes, const Item* items, int n
     hipLaunchKernelGGL(k_real<kRealTable32>, MDG_GRID(n, kRealThreads), boxes, items, mode);
 }
 
-void launch_sk32(const BoxDesc* boxes, const Item* items, int n, int hs, int hsw, hipStream_t s) {
-  if (n <= 0) return;
-  const size_t lds = (size_t)kSkTile * (2 * hs + hsw) * sizeof(float2);
-  hipLaunchKernelGGL(k_sk_seg<float>, MDG_GRID_LDS(n, kSkThreads, lds), boxes, items, hs, hsw);
-}
-
 void launch_wb_tables(const BoxDesc* boxes, const Item* items, int n, hipStream_t s) {
   if (n > 0) hipLaunchKernelGGL(k_wb_tables, MDG_GRID(n, kIonThreads), boxes, items);
 }
@@ -51,5 +45,21 @@ void launch_kforce_mfma(int kb, const BoxDesc* boxes, const Item* items, int n, 
 }
 
 int kforce_mfma_ions(int kb) { return 32 * kf_blocks_for(kb) * kKfMfmaWaves; }
+
+void launch_sb_tables(const BoxDesc* boxes, const Item* items, int n, hipStream_t s) {
+  if (n > 0) hipLaunchKernelGGL(k_sb_tables, MDG_GRID(n, kIonThreads), boxes, items);
+}
+
+void launch_sk_mfma(int hl, const BoxDesc* boxes, const Item* items, int n, hipStream_t s) {
+  if (n <= 0) return;
+  const int lds = sk_mfma_lds_bytes(hl);
+  static int attr_set = 0;  // largest dynamic LDS size granted so far
+  if (lds > 65536 && lds > attr_set) {
+    (void)hipFuncSetAttribute((const void*)k_sk_mfma, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              lds);
+    attr_set = lds;
+  }
+  hipLaunchKernelGGL(k_sk_mfma, MDG_GRID_LDS(n, kSkmThreads, lds), boxes, items, hl);
+}
 
 }  // namespace mdg

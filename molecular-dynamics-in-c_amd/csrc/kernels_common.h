@@ -193,7 +193,8 @@ __global__ __launch_bounds__(kIonThreads) void k_sk_finalize(const BoxDesc* __re
     ss *= 2.0;
     e[0] = B.kce[k] * (sc * sc + ss * ss);
     const double a = B.kcf[k] * ss, b = B.kcf[k] * sc, hz = (double)B.khz[k];
-    const size_t slot = (size_t)B.kslot[k];
+    // FP64 back-projection: quad slot layout; MFMA packing (mixed): k order
+    const size_t slot = B.kfa ? (size_t)k : (size_t)B.kslot[k];
     double* t = B.kab + 4 * slot;
     t[0] = a; t[1] = b; t[2] = hz * a; t[3] = hz * b;
     if (B.kab32) {

@@ -1,5 +1,6 @@
-// kernels_mfma.h -- reciprocal-space back-projection on the matrix cores
-// (mixed path).
+// kernels_mfma.h -- the reciprocal-space passes of the mixed path on the
+// matrix cores: the back-projection (this header comment) and the
+// structure factor (its own section below).
 //
 // Per k column (hx, hy) and ion i the back-projection needs
 //   U_i = sum_hz T_k W_i[hz],   V_i = sum_hz hz T_k W_i[hz]
@@ -94,12 +95,26 @@ __global__ __launch_bounds__(kWave) void k_kf_pack(const BoxDesc* __restrict__ b
   const bool cv = ct < tl.nc;
   KCol col = {0, 0, 0, 0, 0, {0, 0, 0}};
   if (cv) col = B.cols[tl.c0 + ct];
-  // column scales: largest |T| and |hz T| to [2^10, 2^11)
+  // column scales: largest |T| and |hz T| to [2^10, 2^11).  Eight lanes
+  // scan each of the tile's columns (weights in k order), then combine.
   double mu = 0.0, mv = 0.0;
-  for (int j = 0; j < col.len; ++j) {
-    const double* w = B.kab + 4 * (size_t)B.kslot[col.k0 + j];
-    mu = fmax(mu, fmax(fabs(w[0]), fabs(w[1])));
-    mv = fmax(mv, fmax(fabs(w[2]), fabs(w[3])));
+  {
+    const int m = l >> 3, sub = l & 7;
+    if (m < tl.nc) {
+      const KCol cm = B.cols[tl.c0 + m];
+      for (int j = sub; j < cm.len; j += 8) {
+        const double* w = B.kab + 4 * (size_t)(cm.k0 + j);
+        mu = fmax(mu, fmax(fabs(w[0]), fabs(w[1])));
+        mv = fmax(mv, fmax(fabs(w[2]), fabs(w[3])));
+      }
+    }
+#pragma unroll
+    for (int off = 1; off < 8; off <<= 1) {
+      mu = fmax(mu, __shfl_xor(mu, off));
+      mv = fmax(mv, __shfl_xor(mv, off));
+    }
+    mu = __shfl(mu, 8 * ct);  // this lane's column
+    mv = __shfl(mv, 8 * ct);
   }
   int eu = 0, ev = 0;
   (void)frexp(mu, &eu);
@@ -116,7 +131,7 @@ __global__ __launch_bounds__(kWave) void k_kf_pack(const BoxDesc* __restrict__ b
       const int hz = 8 * kb + 4 * g + (e >> 1), p = e & 1;
       float v = 0.f;
       if (cv && hz >= col.hz0 && hz < col.hz0 + col.len) {
-        const double* w = B.kab + 4 * (size_t)B.kslot[col.k0 + hz - col.hz0] + (isv ? 2 : 0);
+        const double* w = B.kab + 4 * (size_t)(col.k0 + hz - col.hz0) + (isv ? 2 : 0);
         const double tr = w[0] * sc, ti = w[1] * sc;
         // Re rows take (Tr, -Ti) against (Re W, Im W); Im rows (Ti, Tr)
         v = (float)((q & 1) == 0 ? (p == 0 ? tr : -ti) : (p == 0 ? ti : tr));
@@ -280,5 +295,184 @@ __global__ __launch_bounds__(kKfMfmaWaves* kWave) void k_kforce_mfma(
 // ion blocks per wave for a K-block count: one (measured faster than two,
 // which halves occupancy)
 constexpr int kf_blocks_for(int kb) { return kb > 0 ? 1 : 1; }
+
+}  // namespace mdg
+
+namespace mdg {
+
+// ======================================================================
+// Structure factor on the matrix cores (mixed path).
+//   S(k) = sum_i Z_i X_i[hx] Y_i[hy] W_i[hz]  (mdc/program/rcprcl.c:59-78)
+// For 32 columns (hx, hy) and 16 hz this is one real matrix product over
+// the ions:  C[column][(hz, Re|Im)] = sum_(ion, re|im) A B  with
+//   A[column][(ion, p)] = Q = X[hx] Y[hy]  (p: Re Q, Im Q),
+//   B[(ion, p)][(hz, out)] from Z W[hz]: Re S takes (ZWr, -ZWi), Im S
+//   takes (ZWi, ZWr).
+// Q is formed per lane (its column, 4 ions per K step) and split into FP16
+// hi/lo; the B records are precomputed.  Three MFMAs per K step and hz
+// tile (hi*hi + hi*lo + lo*hi), FP32 accumulation over 64 ions, then FP64.
+// ======================================================================
+
+// B records, one thread per (ion group of 4, hz): both outputs, hi and lo.
+__global__ __launch_bounds__(kIonThreads) void k_sb_tables(const BoxDesc* __restrict__ boxes,
+                                                           const Item* __restrict__ items) {
+  const Item it = items[blockIdx.x];
+  const BoxDesc& B = boxes[it.box];
+  const int N = B.N, HP = 16 * B.sknt;
+  const int idx = it.a + threadIdx.x;
+  const int grp = idx / HP, hz = idx - grp * HP;
+  if (grp >= B.npad / 4) return;
+  const float2* W = reinterpret_cast<const float2*>(B.tabw);
+  _Float16 h0[8], l0[8], h1[8], l1[8];
+#pragma unroll
+  for (int e = 0; e < 4; ++e) {
+    const int i = 4 * grp + e;
+    float wr = 0.f, wi = 0.f;
+    if (i < N && hz <= B.hmax) {
+      const float2 w = W[(size_t)hz * N + i];
+      const float z = (float)B.Z[B.typ[i]];
+      wr = z * w.x;
+      wi = z * w.y;
+    }
+    split_f16(wr, h0[2 * e], l0[2 * e]);       // Re S: (ZWr, -ZWi)
+    split_f16(-wi, h0[2 * e + 1], l0[2 * e + 1]);
+    split_f16(wi, h1[2 * e], l1[2 * e]);       // Im S: (ZWi, ZWr)
+    split_f16(wr, h1[2 * e + 1], l1[2 * e + 1]);
+  }
+  u4_t* dst = reinterpret_cast<u4_t*>(B.sb) + ((size_t)grp * HP + hz) * 4;
+  dst[0] = pack_h8(h0);
+  dst[1] = pack_h8(l0);
+  dst[2] = pack_h8(h1);
+  dst[3] = pack_h8(l1);
+}
+
+// One workgroup: 4 waves x 32 columns, one ion chunk, two hz tiles of 16
+// (further hz tiles are further items).  Ion tiles of 32 are staged in LDS
+// for the four waves: X and Y rows of every h (rows padded by two complex
+// so that lanes reading different rows use different banks) and the
+// tile's Z W records of the item's hz tiles.
+constexpr int kSkmTile = 32;
+constexpr int kSkmRow = kSkmTile + 2;  // complex per LDS row
+constexpr int kSkmThreads = kKfMfmaWaves * kWave;
+
+// LDS bytes of k_sk_mfma for a launch whose largest box has H = hmax + 1
+__host__ __device__ inline int sk_mfma_lds_bytes(int H) {
+  return 2 * H * kSkmRow * 8 + 8 * 32 * 4 * 16;
+}
+
+__global__ __launch_bounds__(kSkmThreads) void k_sk_mfma(const BoxDesc* __restrict__ boxes,
+                                                         const Item* __restrict__ items,
+                                                         int hl) {
+  extern __shared__ u4_t smem4[];
+  const Item it = items[blockIdx.x];
+  const BoxDesc& B = boxes[it.box];
+  const int N = B.N, ncol = B.ncol, HP = 16 * B.sknt, H = B.hmax + 1;
+  const int tid = threadIdx.x, w = tid / kWave, l = tid % kWave, r = l & 31, g = l >> 5;
+  const int t0 = it.c;                 // first hz tile of the item
+  const int nt = min(2, B.sknt - t0);  // its hz tiles with k vectors (1 or 2)
+  constexpr int kRow4 = kSkmRow / 2;   // LDS row in 16-byte units
+  typedef __attribute__((address_space(3))) u4_t lds_u4;  // LDS stores, not flat
+  lds_u4* xs = (lds_u4*)smem4;         // [hl][kRow4]
+  lds_u4* ys = xs + hl * kRow4;        // [hl][kRow4]
+  lds_u4* ss = ys + hl * kRow4;        // [8 groups][32 hz][4]
+  const int ex = H * (kSkmTile / 2);   // 16-byte elements per table and tile
+  constexpr int es = 8 * 32 * 4;
+  const u4_t* gx = reinterpret_cast<const u4_t*>(B.tabx);
+  const u4_t* gy = reinterpret_cast<const u4_t*>(B.taby);
+  const u4_t* gs = reinterpret_cast<const u4_t*>(B.sb);
+  const int jb = it.a * B.chunk, je = min(N, jb + B.chunk);
+  const int ngrp = B.npad / 4;
+  // this lane's column
+  const int c0 = it.b + 32 * w;
+  const int c = c0 + r;
+  const bool cv = c < ncol;
+  int ax = 0, ay = 0;
+  float sx = 1.f, sy = 1.f;
+  if (cv) {
+    const int hx = B.cols[c].hx, hy = B.cols[c].hy;
+    ax = hx < 0 ? -hx : hx;
+    ay = hy < 0 ? -hy : hy;
+    sx = hx < 0 ? -1.f : 1.f;
+    sy = hy < 0 ? -1.f : 1.f;
+  }
+  typedef float f2_t __attribute__((ext_vector_type(2)));
+  typedef __attribute__((address_space(3))) const f2_t lds_f2;
+  lds_f2* xr = (lds_f2*)(xs + ax * kRow4);
+  lds_f2* yr = (lds_f2*)(ys + ay * kRow4);
+  // FP32 sums over the chunk (at most kSkmChunk ions); chunks add in FP64
+  f16_t acc[2];
+#pragma unroll
+  for (int t = 0; t < 2; ++t)
+#pragma unroll
+    for (int v = 0; v < 16; ++v) acc[t][v] = 0.f;
+  for (int j0 = jb; j0 < je; j0 += kSkmTile) {
+    __syncthreads();  // the previous tile's reads are done
+    // X and Y rows of the tile
+    for (int e = tid; e < 2 * ex; e += kSkmThreads) {
+      const int tb = e >= ex, q = e - tb * ex, h = q >> 4, p = q & 15;
+      const int i = j0 + 2 * p;
+      u4_t v = {0u, 0u, 0u, 0u};
+      if (i + 2 <= N) v = (tb ? gy : gx)[((size_t)h * N + i) >> 1];
+      if (tb) ys[h * kRow4 + p] = v;
+      else xs[h * kRow4 + p] = v;
+    }
+    // records of the tile's 8 ion groups, both hz tiles (zero past the box's hz)
+    for (int q = tid; q < es; q += kSkmThreads) {
+      const int grp = q >> 7, rem = q & 127, hz = 16 * t0 + (rem >> 2);
+      const int gg = (j0 >> 2) + grp;
+      u4_t v = {0u, 0u, 0u, 0u};
+      if (gg < ngrp && hz < HP) v = gs[((size_t)gg * HP + 16 * t0) * 4 + rem];
+      ss[q] = v;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int s = 0; s < kSkmTile / 8; ++s) {
+      const int u = 8 * s + 4 * g;  // tile-local first ion of this lane's K slice
+      h8_t ah, al;
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const f2_t x = xr[u + e], y = yr[u + e];
+        const float xi = sx * x.y, yi = sy * y.y;
+        float qr = x.x * y.x - xi * yi, qi = x.x * yi + xi * y.x;
+        if (!cv) qr = qi = 0.f;
+        _Float16 h0, l0, h1, l1;
+        split_f16(qr, h0, l0);
+        split_f16(qi, h1, l1);
+        ah[2 * e] = h0; al[2 * e] = l0;
+        ah[2 * e + 1] = h1; al[2 * e + 1] = l1;
+      }
+      const int grp = 2 * s + g;  // tile-local ion group of this K slice
+#pragma unroll
+      for (int t = 0; t < 2; ++t) {  // a second tile past the box's hz adds zeros
+        const lds_u4* rec = ss + (grp * 32 + 16 * t + (r >> 1)) * 4 + 2 * (r & 1);
+        const h8_t bh = __builtin_bit_cast(h8_t, rec[0]);
+        const h8_t bl = __builtin_bit_cast(h8_t, rec[1]);
+        acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, bh, acc[t], 0, 0, 0);
+        acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, bl, acc[t], 0, 0, 0);
+        acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_f16(al, bh, acc[t], 0, 0, 0);
+      }
+    }
+  }
+  if (c0 >= ncol) return;
+  // result (row = column 8 (v/4) + 4 g + v%4, col = (hz r/2, out r%2)):
+  // lanes r and r^1 hold Re S and Im S of the same (column, hz)
+  double2* dst = B.skpart + (size_t)it.a * B.nk;
+#pragma unroll
+  for (int t = 0; t < 2; ++t) {
+    if (t >= nt) continue;
+    const int hz = 16 * (t0 + t) + (r >> 1);
+#pragma unroll
+    for (int v = 0; v < 16; ++v) {
+      const double mine = (double)acc[t][v];
+      const double other = __shfl_xor(mine, 1);
+      const int cc = c0 + 8 * (v >> 2) + 4 * g + (v & 3);
+      if ((r & 1) == 0 && cc < ncol) {
+        const KCol col = B.cols[cc];
+        if (hz >= col.hz0 && hz < col.hz0 + col.len)
+          dst[col.k0 + hz - col.hz0] = make_double2(mine, other);
+      }
+    }
+  }
+}
 
 }  // namespace mdg

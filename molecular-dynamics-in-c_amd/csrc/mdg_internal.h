@@ -65,6 +65,7 @@ struct KfTile {
 };
 constexpr int kKfTileCols = 8;
 constexpr int kKfMfmaWaves = 4;     // waves per back-projection workgroup
+constexpr int kSkmChunk = 512;      // mixed structure factor: ions per FP32 partial sum
 
 // Work item of a launch over many boxes.
 struct Item {
@@ -141,6 +142,14 @@ struct BoxDesc {
   float4 *colw;       // per column: hx/sU, hy/sU, 1/sV, (hx, hy) as packed int16
   const KfTile *kft;  // [ntile]
   int kbk, ntile;
+  // ---- structure factor on MFMA (mixed path) ----
+  // Z*W[hz] as B operands of v_mfma_f32_32x32x16_f16: per group of 4 ions,
+  // per hz < 16 * sknt, per output (Re S, Im S) a 32-byte record of 8 hi
+  // then 8 lo halves, (p = re, im of Q) x 4 ions: for Re S (ZWr, -ZWi),
+  // for Im S (ZWi, ZWr).  Ions padded to a multiple of 8 with zeros.
+  void *sb;
+  int sknt;           // hz tiles of 16
+  int npad;           // N rounded up to a multiple of 8
   // ---- reductions ----
   double *rpart;      // [nreal][4]: pot, vir x, y, z per real-space item
   unsigned long long *rpairs;  // [nreal] accepted unordered pairs per item
