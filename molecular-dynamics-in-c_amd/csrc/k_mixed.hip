@@ -15,9 +15,9 @@ void launch_real32(bool analytic, const BoxDesc* boxes, const Item* items, int n
                    hipStream_t s) {
   if (n <= 0) return;
   if (analytic)
-    hipLaunchKernelGGL(k_real<kRealAnalytic32>, MDG_GRID(n, kRealThreads), boxes, items, mode);
+    hipLaunchKernelGGL(k_real_win<kRealAnalytic32>, MDG_GRID(n, kRealThreads), boxes, items, mode);
   else
-    hipLaunchKernelGGL(k_real<kRealTable32>, MDG_GRID(n, kRealThreads), boxes, items, mode);
+    hipLaunchKernelGGL(k_real_win<kRealTable32>, MDG_GRID(n, kRealThreads), boxes, items, mode);
 }
 
 void launch_wb_tables(const BoxDesc* boxes, const Item* items, int n, hipStream_t s) {

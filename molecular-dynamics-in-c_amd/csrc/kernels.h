@@ -219,6 +219,48 @@ __device__ __forceinline__ void pair32(const BoxCtx& c, const Row32& w, double d
       (eb * c.inv_rho * rinv - 6.0f * w.c6 * r8 - 8.0f * w.d8 * r8 * rinv2);
 }
 
+// pair32 with the hardware reciprocal and base-2 exponential (v_rcp_f32,
+// v_exp_f32, about 1 ulp) in place of the correctly rounded forms.  The
+// library forms only add a rescaling for results in the denormal range;
+// here the exponents are about -20 (Gaussian, (alpha rc)^2 log2 e at an
+// Ewald accuracy of 1e-6) and -32 (Born-Mayer at 10 A), and a result
+// below 2^-126 would be flushed to a zero that is negligible anyway.
+__device__ __forceinline__ void pair32_fast(const BoxCtx& c, const Row32& w, double dr, float& p,
+                                            float& f) {
+  const float kLog2e = 1.4426950408889634f;
+  const float kLn2 = 0.6931471805599453f;
+  const float r2 = (float)dr;
+  const float r2lo = (float)(dr - (double)r2);
+  const float rinv = __builtin_amdgcn_rsqf(r2);
+  const float r = r2 * rinv;
+  const float rlo = 0.5f * (fmaf(-r, r, r2) + r2lo) * rinv;  // r + rlo = sqrt(dr)
+  const float rinv2 = rinv * rinv;
+  const float z = c.alpha * r;
+  // erfc(z) = t exp(-z^2 + P(t)), t = 1/(1 + z/2): Numerical Recipes' Chebyshev fit
+  const float t = __builtin_amdgcn_rcpf(fmaf(0.5f, z, 1.0f));
+  float q = fmaf(t, 0.17087277f, -0.82215223f);
+  q = fmaf(t, q, 1.48851587f);
+  q = fmaf(t, q, -1.13520398f);
+  q = fmaf(t, q, 0.27886807f);
+  q = fmaf(t, q, -0.18628806f);
+  q = fmaf(t, q, 0.09678418f);
+  q = fmaf(t, q, 0.37409196f);
+  q = fmaf(t, q, 1.00002368f);
+  q = fmaf(t, q, -1.26551223f);
+  const float mz2 = -z * z;
+  const float erfc_r = t * __builtin_amdgcn_exp2f((mz2 + q) * kLog2e) * rinv;
+  const float g = __builtin_amdgcn_exp2f(mz2 * kLog2e);
+  const float a = (w.sig - r) - rlo;
+  const float x = a * c.inv_rho_l2e;
+  const float xlo = fmaf(a, c.inv_rho_l2e, -x) + a * c.inv_rho_l2e_lo;
+  const float eb = w.cb * __builtin_amdgcn_exp2f(x) * fmaf(kLn2, xlo, 1.0f);
+  const float r6 = rinv2 * rinv2 * rinv2;
+  const float r8 = r6 * rinv2;
+  p = w.zz * erfc_r + eb - w.c6 * r6 - w.d8 * r8;
+  f = w.zz * (c.two_a3 * g + erfc_r) * rinv2 +
+      (eb * c.inv_rho * rinv - 6.0f * w.c6 * r8 - 8.0f * w.d8 * r8 * rinv2);
+}
+
 // FP64 analytic pair function in the operation order of mdc's mk_table
 // (mdc/program/control.c:219-240), erfc per pair as pr-mdc's ewald1 does
 // (pr-mdc/program/ewald.c:37-43).
@@ -443,6 +485,246 @@ __global__ __launch_bounds__(kRealThreads) void k_real(const BoxDesc* __restrict
       real_store(B, h, split, mode, a);
       tot.pot += a.pot; tot.vx += a.vx; tot.vy += a.vy; tot.vz += a.vz; tot.np += a.np;
     }
+  }
+  const double pot = wave_sum(tot.pot), vx = wave_sum(tot.vx), vy = wave_sum(tot.vy),
+               vz = wave_sum(tot.vz);
+  const unsigned long long np = wave_sum_u64(tot.np);
+  if (err) atomicOr(B.err, err);
+  if (lane == 0) {
+    double* rp = B.rpart + 4 * (size_t)it.b;
+    rp[0] = pot; rp[1] = vx; rp[2] = vy; rp[3] = vz;
+    B.rpairs[it.b] = np;
+  }
+}
+
+// ======================================================================
+// Real space, mixed precision: screened windows of partners.
+//
+// The same pairs, pair function and per-ion FP64 sums as k_real's mixed
+// kinds, organised for issue rate:
+//  * Partners are staged kRealWin at a time (a window), so one lane's pop
+//    loop spans four tiles and the lanes' candidate counts even out.
+//  * Screening runs on 16-bit fixed-point coordinates q = x * 65536 / L
+//    (mod 2^16).  The packed 16-bit difference of two such coordinates
+//    wraps to the minimum image by itself, and two v_dot2_i32_i16 give
+//    the squared distance against a threshold: five instructions per
+//    candidate pair, no fold.  With u = min(L) / 65536 each axis' wrapped
+//    difference is at most |dx| / u + 1, so every pair within rc passes
+//    d^2 <= (rc / u + sqrt 3)^2 (a superset: the exact test follows).
+//  * A lane's candidates are bits of 32-bit words kept in LDS; the lane
+//    pops them in increasing partner order and applies the FP64
+//    minimum-image fold and the FP64 cutoff test, then the FP32 pair
+//    function.  Forces, energy and virial accumulate in FP64.
+// The fold is x - L rint(x / L), which equals mdc's strict-comparison
+// fold (mdc/program/real.c:43-49) except for a separation of exactly L/2
+// on an axis, where both images are at the same distance.
+// ======================================================================
+constexpr int kRealWin = 256;                 // partners per window
+constexpr int kRealWinWords = kRealWin / 32;  // candidate words per lane
+
+struct alignas(16) WinRec {
+  double x, y, z;
+  int id, t;
+};
+
+// 16-bit fixed-point coordinates of one ion: (qx | qy << 16, qz)
+__device__ __forceinline__ uint2 quantize(double x, double y, double z, double sx, double sy,
+                                          double sz) {
+  const unsigned qx = (unsigned)(int)rint(x * sx) & 0xffffu;
+  const unsigned qy = (unsigned)(int)rint(y * sy) & 0xffffu;
+  const unsigned qz = (unsigned)(int)rint(z * sz) & 0xffffu;
+  return make_uint2(qx | (qy << 16), qz);
+}
+
+typedef short mdg_s2 __attribute__((ext_vector_type(2)));
+
+// 32 candidates u0 .. u0+31 of one lane: bit 31-k is set when partner
+// u0+k passes the screen.  thr1 = threshold + 1.
+__device__ __forceinline__ unsigned screen32(uint2 hq, const uint2* pq, int u0, int thr1) {
+  unsigned m = 0u;
+#pragma unroll
+  for (int k = 0; k < 32; ++k) {
+    const uint2 p = pq[u0 + k];
+    const mdg_s2 dxy = __builtin_bit_cast(mdg_s2, hq.x) - __builtin_bit_cast(mdg_s2, p.x);
+    const mdg_s2 dz = __builtin_bit_cast(mdg_s2, hq.y) - __builtin_bit_cast(mdg_s2, p.y);
+    int a = __builtin_amdgcn_sdot2(dxy, dxy, -thr1, true);
+    a = __builtin_amdgcn_sdot2(dz, dz, a, true);   // saturates: far pairs stay positive
+    m = __builtin_amdgcn_alignbit(m, (unsigned)a, 31);  // m << 1 | sign(a)
+  }
+  return m;
+}
+
+template <int KIND>
+__global__ __launch_bounds__(kRealThreads) void k_real_win(const BoxDesc* __restrict__ boxes,
+                                                           const Item* __restrict__ items,
+                                                           int mode) {
+  static_assert(KIND == kRealAnalytic32 || KIND == kRealTable32, "mixed kinds only");
+  __shared__ WinRec pr[kRealWin];
+  __shared__ uint2 pq[kRealWin];
+  __shared__ unsigned pm[kRealWinWords][kRealThreads];
+  __shared__ int ncum[28], nst[27];
+  const Item it = items[blockIdx.x];
+  const BoxDesc& B = boxes[it.box];
+  const int lane = threadIdx.x;
+  const int N = B.N, split = it.c, nsplit = B.rsplit;
+  int err = 0;
+  BoxCtx c;
+  c.Lx = B.L[0]; c.Ly = B.L[1]; c.Lz = B.L[2];
+  c.rc2 = B.rc2;
+  c.nb = B.table_nb;
+  c.alpha = (float)B.alpha; c.two_a3 = (float)(2.0 * B.a3);
+  c.inv_rho = (float)(1.0 / B.rho);
+  const double irl = 1.4426950408889634 / B.rho;  // log2(e) / rho, split in two floats
+  c.inv_rho_l2e = (float)irl;
+  c.inv_rho_l2e_lo = (float)(irl - (double)c.inv_rho_l2e);
+  const double ilx = B.invL[0], ily = B.invL[1], ilz = B.invL[2];
+  const double sx = 65536.0 * ilx, sy = 65536.0 * ily, sz = 65536.0 * ilz;
+  int thr1;
+  bool screen_all;
+  {
+    const double lmin = fmin(c.Lx, fmin(c.Ly, c.Lz));
+    const double s = sqrt(B.rc2) * (65536.0 / lmin) + 1.7320508075688772 + 0.01;
+    // cutoffs beyond ~0.7 L exceed the 31-bit range: every partner is a candidate
+    screen_all = s * s >= 2.0e9;
+    thr1 = screen_all ? 0 : (int)(s * s) + 1;
+  }
+  RealAcc tot;
+  // partner source: all pairs -> index range; cells -> run of neighbour cells
+  int hs, he, pbeg, pend;
+  int cx = 0, cy = 0, cz = 0;
+  if (B.all_pairs) {
+    hs = it.a; he = min(N, it.a + kRealThreads);
+    pbeg = (int)((long)split * N / nsplit); pend = (int)((long)(split + 1) * N / nsplit);
+  } else {
+    const int cell = it.a;
+    const int ncx = B.nc[0], ncy = B.nc[1], ncz = B.nc[2];
+    cx = cell % ncx; cy = (cell / ncx) % ncy; cz = cell / (ncx * ncy);
+    hs = B.cell_start[cell]; he = B.cell_start[cell + 1];
+    const int n0 = split * 27 / nsplit, n1 = (split + 1) * 27 / nsplit;
+    const int n = n0 + lane;
+    int cnt = 0, st = 0;
+    if (n < n1) {
+      const int oz = n / 9 - 1, oy = (n / 3) % 3 - 1, ox = n % 3 - 1;
+      const int nb = (((cz + oz + ncz) % ncz) * ncy + (cy + oy + ncy) % ncy) * ncx +
+                     (cx + ox + ncx) % ncx;
+      st = B.cell_start[nb]; cnt = B.cell_start[nb + 1] - st;
+    }
+    int inc = cnt;  // inclusive prefix over the run
+#pragma unroll
+    for (int off = 1; off < 32; off <<= 1) {
+      const int v = __shfl_up(inc, off, 64);
+      if (lane >= off) inc += v;
+    }
+    if (lane < 27) { nst[lane] = st; ncum[lane + 1] = inc; }
+    if (lane == 0) ncum[0] = 0;
+    pbeg = 0; pend = __shfl(inc, 26, 64);
+    __syncthreads();
+  }
+  for (int h0 = hs; h0 < he; h0 += kRealThreads) {
+    const int hp = h0 + lane;
+    const bool valid = hp < he;
+    const int hq = valid ? hp : hs;
+    HomeCtx h;
+    if (B.all_pairs)
+      make_home<KIND>(B, hq, B.typ[hq], valid, B.rx[hq], B.ry[hq], B.rz[hq], h);
+    else
+      make_home<KIND>(B, B.sorted[hq], B.styp[hq], valid, B.sx[hq], B.sy[hq], B.sz[hq], h);
+    const uint2 hqz = quantize(h.xi, h.yi, h.zi, sx, sy, sz);
+    if constexpr (KIND == kRealAnalytic32) {
+      // pin the FP32 pair constants in registers; otherwise the compiler
+      // re-derives them from the FP64 box constants inside the pair loop
+      for (int s = 0; s < 2; ++s)
+        __asm__ volatile("" : "+v"(h.row[s].zz), "+v"(h.row[s].cb), "+v"(h.row[s].sig),
+                         "+v"(h.row[s].c6), "+v"(h.row[s].d8));
+    }
+    RealAcc a;
+    for (int w0 = pbeg; w0 < pend; w0 += kRealWin) {
+      const int cnt = min(kRealWin, pend - w0);
+      // stage the window
+      for (int s = lane; s < cnt; s += kRealThreads) {
+        int j;
+        double x, y, z;
+        int t;
+        if (B.all_pairs) {
+          j = w0 + s;
+          x = B.rx[j]; y = B.ry[j]; z = B.rz[j]; t = B.typ[j];
+        } else {
+          const int g = w0 + s;
+          int lo = 0, hi = 26;  // last run entry with ncum[n] <= g
+          while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (ncum[mid] <= g) lo = mid; else hi = mid - 1;
+          }
+          const int jp = nst[lo] + (g - ncum[lo]);
+          x = B.sx[jp]; y = B.sy[jp]; z = B.sz[jp]; t = B.styp[jp]; j = B.sorted[jp];
+        }
+        WinRec r;
+        r.x = x; r.y = y; r.z = z; r.id = j; r.t = t;
+        pr[s] = r;
+        pq[s] = quantize(x, y, z, sx, sy, sz);
+      }
+      __syncthreads();
+      // screen: one candidate word per 32 partners
+      const int nw = (cnt + 31) >> 5;
+      unsigned nz = 0u;
+      for (int w = 0; w < nw; ++w) {
+        unsigned m = screen32(hqz, pq, w * 32, thr1);
+        if (screen_all) m = 0xffffffffu;
+        const int nv = min(32, cnt - w * 32);
+        if (nv < 32) m &= ~(0xffffffffu >> nv);
+        if (!valid) m = 0u;
+        pm[w][lane] = m;
+        nz |= (m != 0u ? 1u : 0u) << w;
+      }
+      // Evaluate: each lane pops its own candidates in increasing order.
+      // The body is branch-free; a lane out of candidates evaluates a
+      // discarded pair.  Energy and virial are summed over ordered pairs
+      // and halved once per item.
+      int w = nz ? __builtin_ctz(nz) : 0;
+      unsigned cur = nz ? pm[w][lane] : 0u;
+      if (__ballot(cur != 0u)) do {
+        const bool live = cur != 0u;
+        const int b = __builtin_clz(cur | 1u);
+        cur &= ~(0x80000000u >> b);
+        const WinRec P = pr[w * 32 + b];
+        if (live && cur == 0u) {
+          nz &= ~(1u << w);
+          if (nz) { w = __builtin_ctz(nz); cur = pm[w][lane]; }
+        }
+        double dx = h.xi - P.x, dy = h.yi - P.y, dz = h.zi - P.z;
+        dx = fma(-c.Lx, rint(dx * ilx), dx);
+        dy = fma(-c.Ly, rint(dy * ily), dy);
+        dz = fma(-c.Lz, rint(dz * ilz), dz);
+        bool ok = live && P.id != h.i;
+        float p, f;
+        if constexpr (KIND == kRealAnalytic32) {
+          const double dr = fma(dx, dx, fma(dy, dy, dz * dz));
+          ok = ok && dr <= c.rc2;
+          // a discarded pair may give inf or NaN here; it is selected away
+          pair32_fast(c, P.t == 0 ? h.row[0] : h.row[1], dr, p, f);
+        } else {
+          const double dr = dx * dx + dy * dy + dz * dz;
+          ok = ok && dr <= c.rc2;
+          int bin = (int)((sqrt(dr) - 0.5) * 1000.0);
+          if (ok && (bin < 0 || bin >= c.nb)) { err |= kErrRange; ok = false; }
+          bin = ok ? bin : 0;
+          const int s = P.t;
+          const float2* row =
+              s == 0 ? h.t32[0] : (s == 1 ? h.t32[1] : (s == 2 ? h.t32[2] : h.t32[3]));
+          const float2 pf = row[bin];
+          p = pf.x; f = pf.y;
+        }
+        const double fd = ok ? (double)f : 0.0;
+        a.fx = fma(fd, dx, a.fx); a.fy = fma(fd, dy, a.fy); a.fz = fma(fd, dz, a.fz);
+        a.pot += ok ? (double)p : 0.0;
+        a.vx = fma(fd * dx, dx, a.vx); a.vy = fma(fd * dy, dy, a.vy); a.vz = fma(fd * dz, dz, a.vz);
+        a.np += (ok && P.id > h.i) ? 1u : 0u;
+      } while (__ballot(cur != 0u));
+      __syncthreads();
+    }
+    real_store(B, h, split, mode, a);
+    tot.pot += 0.5 * a.pot; tot.vx += 0.5 * a.vx; tot.vy += 0.5 * a.vy; tot.vz += 0.5 * a.vz;
+    tot.np += a.np;
   }
   const double pot = wave_sum(tot.pot), vx = wave_sum(tot.vx), vy = wave_sum(tot.vy),
                vz = wave_sum(tot.vz);

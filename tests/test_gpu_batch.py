@@ -108,6 +108,39 @@ def nve_drift(p, prec, r, v, sp, steps=1000, every=10):
     return np.polyfit(t, es, 1)[0] * 1e4, np.ptp(es)
 
 
+@pytest.mark.parametrize("n,salt,a", [(8, "NaCl", H.A_NACL), ((6, 7, 8), "KCl", H.A_KCL),
+                                      ((2, 3, 4), "NaCl", H.A_NACL)])
+def test_mixed_screen_keeps_every_pair(gpu, n, salt, a):
+    """The mixed real space screens candidates on 16-bit periodic
+    coordinates before its exact FP64 cutoff test: it must accept exactly
+    the FP64 path's pairs, with all pairs and with cell lists, cubic or
+    not (rc = L_min / 2 in the 2x3x4 box)."""
+    modes = [mdgpu.ALL_PAIRS]
+    p0 = _params(n, a, salt=salt)
+    nx, ny, nz = n if isinstance(n, tuple) else (n, n, n)
+    if min(np.array([nx, ny, nz]) * a) >= 3 * p0.rc:
+        modes.append(mdgpu.CELL_LIST)
+    r, v, sp = _state(p0, 11, amp=0.5)
+    b = mdgpu.Batch([p0], mdgpu.FP64)
+    b.set_state(0, r=r, v=v, species=sp)
+    b.forces()
+    f64, pairs64 = b.state(0)["f"], b.work(0)["pairs"]
+    b.close()
+    got = []
+    for cells in modes:
+        p = _params(n, a, salt=salt, cells=cells)
+        b = mdgpu.Batch([p], mdgpu.MIXED)
+        b.set_state(0, r=r, v=v, species=sp)
+        b.forces()
+        f32, pairs32 = b.state(0)["f"], b.work(0)["pairs"]
+        b.close()
+        assert pairs32 == pairs64, (cells, pairs32, pairs64)
+        assert H.rel_err(f32, f64) <= MIXED_TOL
+        got.append(f32)
+    if len(got) == 2:  # same pairs, other summation order
+        assert H.rel_err(got[1], got[0]) <= 1e-6
+
+
 def test_mixed_nve_energy_drift(gpu):
     """NVE (analytic real space, dt = 1 fs, 1000 K): the mixed path's energy
     drift is <= 1e-5 per ion per 1e4 steps and equals the FP64 path's to
