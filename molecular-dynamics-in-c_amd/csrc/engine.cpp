@@ -386,7 +386,7 @@ static Layout layout(const BoxPlan& P, int precision) {
       const size_t nt = P.kftiles.size();
       L.wb = take(off, 32 * (size_t)(2 * P.kbk) * N);
       L.kfa = take(off, 16 * (size_t)kWave * 2 * P.kbk * nt);
-      L.colw = take(off, sizeof(float4) * (size_t)P.ncol);
+      L.colw = take(off, sizeof(float4) * kKfTileCols * nt);
       L.kft = take(off, sizeof(KfTile) * nt);
       const size_t npad = (size_t)(N + 7) / 8 * 8, hp = 16 * (size_t)((P.hmax + 16) / 16);
       L.sb = take(off, 16 * npad * hp);

@@ -32,7 +32,7 @@ void launch_kforce_mfma(int kb, const BoxDesc* boxes, const Item* items, int n, 
   if (n <= 0) return;
   const int t = kKfMfmaWaves * kWave;
 #define MDG_KF(K) \
-  hipLaunchKernelGGL((k_kforce_mfma<K, kf_blocks_for(K)>), MDG_GRID(n, t), boxes, items)
+  hipLaunchKernelGGL((k_kforce_mfma<K>), MDG_GRID(n, t), boxes, items)
   switch (kb) {
     case 1: MDG_KF(1); break;
     case 2: MDG_KF(2); break;
@@ -44,7 +44,7 @@ void launch_kforce_mfma(int kb, const BoxDesc* boxes, const Item* items, int n, 
 #undef MDG_KF
 }
 
-int kforce_mfma_ions(int kb) { return 32 * kf_blocks_for(kb) * kKfMfmaWaves; }
+int kforce_mfma_ions(int) { return 32 * kKfMfmaWaves; }
 
 void launch_sb_tables(const BoxDesc* boxes, const Item* items, int n, hipStream_t s) {
   if (n > 0) hipLaunchKernelGGL(k_sb_tables, MDG_GRID(n, kIonThreads), boxes, items);

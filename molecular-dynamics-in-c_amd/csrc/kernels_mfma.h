@@ -32,6 +32,21 @@ namespace mdg {
 typedef _Float16 h8_t __attribute__((ext_vector_type(8)));
 typedef float f16_t __attribute__((ext_vector_type(16)));
 typedef unsigned int u4_t __attribute__((ext_vector_type(4)));
+typedef float f4v_t __attribute__((ext_vector_type(4)));
+typedef float f2v_t __attribute__((ext_vector_type(2)));
+
+// Byte pointer in the global address space: a load through it takes a
+// uniform 64-bit base and a 32-bit per-lane byte offset.
+typedef const __attribute__((address_space(1))) unsigned char* gbytes;
+template <typename T>
+__device__ __forceinline__ T gload(gbytes base, unsigned off) {
+  return *reinterpret_cast<const __attribute__((address_space(1))) T*>(base + off);
+}
+
+// x with its sign flipped where s is negative
+__device__ __forceinline__ float flip_sign(float x, float s) {
+  return __uint_as_float(__float_as_uint(x) ^ (__float_as_uint(s) & 0x80000000u));
+}
 
 __device__ __forceinline__ void split_f16(float x, _Float16& hi, _Float16& lo) {
   hi = (_Float16)x;
@@ -142,159 +157,167 @@ __global__ __launch_bounds__(kWave) void k_kf_pack(const BoxDesc* __restrict__ b
     dst[0] = pack_h8(hv);
     dst[kWave] = pack_h8(lv);
   }
-  if (cv && q == 0 && g == 0) {
-    const int bits = (col.hx & 0xffff) | (col.hy << 16);
-    B.colw[tl.c0 + ct] = make_float4((float)(col.hx / su), (float)(col.hy / su),
-                                     (float)(1.0 / sv), __int_as_float(bits));
+  if (q == 0 && g == 0) {  // one lane per tile column ct, padding included
+    const int bits = abs(col.hx) | (abs(col.hy) << 8);
+    B.colw[(size_t)t * kKfTileCols + (ct & 1) * 4 + (ct >> 1)] =
+        cv ? make_float4((float)(col.hx / su), (float)(col.hy / su), (float)(1.0 / sv),
+                         __int_as_float(bits))
+           : make_float4(0.f, 0.f, 0.f, 0.f);
   }
 }
 
 // ----------------------------------------------------------------------
-// Back-projection.  Each wave owns NB blocks of 32 ions, two lanes per ion
-// (one per half of every tile's columns), and a run of tiles; the ions' W
-// operands stay in registers.  Per tile the column data come through the
-// scalar cache and every vector load of the tile (the packed weights of
-// its K blocks, X[hx] and Y[hy] of the lane's columns) is issued before
-// the MFMAs, so the loads of a tile overlap one another and the matrix
-// work.  Per K block and ion block three MFMAs, then G = Re(Q U) and
-// Gz = Re(Q V) for the lane's 4 columns with Q = X[hx] Y[hy].  Tile sums in
-// FP32, sums over tiles in FP64.
+// Back-projection.  A workgroup of kKfMfmaWaves waves owns 32 ions per
+// wave (two lanes per ion, one per half of every tile's columns) and a run
+// of tiles; the ions' W operands stay in registers.  The tiles' packed A
+// operands and column records are the same for every wave, so each tile
+// is copied once per workgroup into LDS, double-buffered: the copy of tile
+// t+1 is in flight while tile t is computed, and the vector-memory pipe
+// carries a tile once per workgroup instead of once per wave.  Per tile
+// and K block three MFMAs, then G = Re(Q U) and Gz = Re(Q V) for the
+// lane's 4 columns with Q = X[hx] Y[hy] gathered from the phase tables.
+// Tile sums in FP32, sums over tiles in FP64.
 // ----------------------------------------------------------------------
-template <int KB, int NB>
+// 16-byte units of one LDS tile: KB blocks of (hi, lo) x 64 lane records,
+// then the tile's column records
+template <int KB>
+constexpr int kf_tile_units() { return KB * 2 * kWave + kKfTileCols; }
+
+template <int KB>
 __global__ __launch_bounds__(kKfMfmaWaves* kWave) void k_kforce_mfma(
     const BoxDesc* __restrict__ boxes, const Item* __restrict__ items) {
+  constexpr int U = kf_tile_units<KB>();
+  constexpr int NT = kKfMfmaWaves * kWave;
+  constexpr int P = (U + NT - 1) / NT;  // staging loads per thread and tile
+  __shared__ u4_t lds[2][U];
   const Item it = items[blockIdx.x];
   const BoxDesc& B = boxes[it.box];
   const int N = B.N, kbk = B.kbk;
-  const int w = threadIdx.x / kWave, l = threadIdx.x % kWave, n = l & 31, g = l >> 5;
-  const int base = it.a + 32 * NB * w;
-  if (base >= N) return;  // whole wave past the box's ions (no barriers here)
+  const int tid = threadIdx.x;
+  const int w = tid / kWave, l = tid % kWave, n = l & 31, g = l >> 5;
+  const bool act = it.a + 32 * w < N;  // the wave has ions (uniform)
+  const int i = it.a + 32 * w + n;
+  const int ii = i < N ? i : 0;
   const u4_t* wb = reinterpret_cast<const u4_t*>(B.wb);
-  h8_t bh[NB][KB], bl[NB][KB];
-  int ii[NB];
+  h8_t bh[KB], bl[KB];
 #pragma unroll
-  for (int q = 0; q < NB; ++q) {
-    const int i = base + 32 * q + n;
-    ii[q] = i < N ? i : 0;
-#pragma unroll
-    for (int kb = 0; kb < KB; ++kb) {
-      u4_t x = {0u, 0u, 0u, 0u}, y = {0u, 0u, 0u, 0u};
-      if (i < N && kb < kbk) {
-        const u4_t* p = wb + ((size_t)(2 * kb + g) * N + i) * 2;
-        x = p[0];
-        y = p[1];
-      }
-      bh[q][kb] = __builtin_bit_cast(h8_t, x);
-      bl[q][kb] = __builtin_bit_cast(h8_t, y);
+  for (int kb = 0; kb < KB; ++kb) {
+    u4_t x = {0u, 0u, 0u, 0u}, y = {0u, 0u, 0u, 0u};
+    if (i < N && kb < kbk) {
+      const u4_t* p = wb + ((size_t)(2 * kb + g) * N + i) * 2;
+      x = p[0];
+      y = p[1];
     }
+    bh[kb] = __builtin_bit_cast(h8_t, x);
+    bl[kb] = __builtin_bit_cast(h8_t, y);
   }
-  const u4_t* kfa = reinterpret_cast<const u4_t*>(B.kfa);
+  // Global-address-space views: uniform bases, 32-bit per-lane byte offsets
+  const gbytes kfa = (gbytes)B.kfa;
+  const gbytes X = (gbytes)B.tabx;
+  const gbytes Y = (gbytes)B.taby;
+  const gbytes colw = (gbytes)B.colw;
   const cptr<KfTile> tiles = as_const(B.kft);
-  const cptr<float> colw = as_const(reinterpret_cast<const float*>(B.colw));
-  const float2* X = reinterpret_cast<const float2*>(B.tabx);
-  const float2* Y = reinterpret_cast<const float2*>(B.taby);
-  double F[NB][3];
+  const unsigned n8 = 8u * (unsigned)N, i8 = 8u * (unsigned)ii;
+  // Staging of tile t: unit u < na is unit kb_lo * 128 + u of its packed
+  // blocks (contiguous in kfa), the next 8 are its column records.  Every
+  // thread issues exactly P loads (units past the tile reload a column
+  // record and are not stored), so the wait counts stay static.
+  auto fetch = [&](int t, u4_t(&pf)[P]) {
+    const int klo = tiles[t].kb_lo, na = (tiles[t].kb_hi - klo) * 2 * kWave;
 #pragma unroll
-  for (int q = 0; q < NB; ++q) F[q][0] = F[q][1] = F[q][2] = 0.0;
+    for (int m = 0; m < P; ++m) {
+      const int u = tid + NT * m;
+      pf[m] = u < na ? gload<u4_t>(kfa, 16u * (unsigned)((t * kbk + klo) * 2 * kWave + u))
+                     : gload<u4_t>(colw, 16u * (unsigned)(t * kKfTileCols +
+                                                          min(u - na, kKfTileCols - 1)));
+    }
+  };
+  auto store = [&](int t, int buf, const u4_t(&pf)[P]) {
+    const int klo = tiles[t].kb_lo, na = (tiles[t].kb_hi - klo) * 2 * kWave;
+#pragma unroll
+    for (int m = 0; m < P; ++m) {
+      const int u = tid + NT * m;
+      if (u < na)
+        lds[buf][klo * 2 * kWave + u] = pf[m];
+      else if (u < na + kKfTileCols)
+        lds[buf][KB * 2 * kWave + (u - na)] = pf[m];
+    }
+  };
+  double F0 = 0.0, F1 = 0.0, F2 = 0.0;
   const int tend = it.b + it.c;
+  {
+    u4_t pf[P];
+    fetch(it.b, pf);
+    store(it.b, 0, pf);
+  }
+  __syncthreads();
   for (int t = it.b; t < tend; ++t) {
-    const int c0 = tiles[t].c0, nc = tiles[t].nc, klo = tiles[t].kb_lo, khi = tiles[t].kb_hi;
-    // the lane's 4 columns: scalar loads of both halves' data, then select
-    float4 cw[4];
-    int hx[4], hy[4];
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const int ca = min(c0 + 2 * j, c0 + nc - 1), cb = min(c0 + 2 * j + 1, c0 + nc - 1);
-      const float4 wa = make_float4(colw[4 * ca], colw[4 * ca + 1], colw[4 * ca + 2], colw[4 * ca + 3]);
-      const float4 wv = make_float4(colw[4 * cb], colw[4 * cb + 1], colw[4 * cb + 2], colw[4 * cb + 3]);
-      cw[j] = g ? wv : wa;
-      const int bits = __float_as_int(cw[j].w);
-      hx[j] = (int)(short)(bits & 0xffff);
-      hy[j] = bits >> 16;
-    }
-    // vector loads of the tile, all issued before the matrix work
-    u4_t ar[KB][2];
-#pragma unroll
-    for (int kb = 0; kb < KB; ++kb) {
-      ar[kb][0] = u4_t{0u, 0u, 0u, 0u};
-      ar[kb][1] = u4_t{0u, 0u, 0u, 0u};
-      if (kb >= klo && kb < khi) {
-        const u4_t* ap = kfa + ((size_t)t * kbk + kb) * 2 * kWave + l;
-        ar[kb][0] = ap[0];
-        ar[kb][1] = ap[kWave];
-      }
-    }
-    float2 xv[NB][4], yv[NB][4];
-#pragma unroll
-    for (int q = 0; q < NB; ++q)
+    const int buf = (t - it.b) & 1;
+    const bool more = t + 1 < tend;
+    const u4_t* sa = lds[buf];
+    f4v_t cw[4];
+    f2v_t xv[4], yv[4];
+    if (act) {
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
-        xv[q][j] = X[(size_t)(hx[j] < 0 ? -hx[j] : hx[j]) * N + ii[q]];
-        yv[q][j] = Y[(size_t)(hy[j] < 0 ? -hy[j] : hy[j]) * N + ii[q]];
-      }
-    f16_t acc[NB];
-#pragma unroll
-    for (int q = 0; q < NB; ++q)
-#pragma unroll
-      for (int v = 0; v < 16; ++v) acc[q][v] = 0.f;
-#pragma unroll
-    for (int kb = 0; kb < KB; ++kb) {
-      if (kb >= klo && kb < khi) {
-        const h8_t ah = __builtin_bit_cast(h8_t, ar[kb][0]);
-        const h8_t al = __builtin_bit_cast(h8_t, ar[kb][1]);
-#pragma unroll
-        for (int q = 0; q < NB; ++q)
-          acc[q] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, bh[q][kb], acc[q], 0, 0, 0);
-#pragma unroll
-        for (int q = 0; q < NB; ++q)
-          acc[q] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, bl[q][kb], acc[q], 0, 0, 0);
-#pragma unroll
-        for (int q = 0; q < NB; ++q)
-          acc[q] = __builtin_amdgcn_mfma_f32_32x32x16_f16(al, bh[q][kb], acc[q], 0, 0, 0);
+        cw[j] = __builtin_bit_cast(f4v_t, sa[KB * 2 * kWave + 4 * g + j]);
+        const unsigned bits = __float_as_uint(cw[j].w);
+        xv[j] = gload<f2v_t>(X, (bits & 0xffu) * n8 + i8);
+        yv[j] = gload<f2v_t>(Y, ((bits >> 8) & 0xffu) * n8 + i8);
       }
     }
+    // the next tile's copy goes out after this tile's gathers, and always
+    // (the last tile re-reads itself) so that the wait counts stay static
+    __asm__ volatile("" ::: "memory");
+    u4_t pf[P];
+    fetch(more ? t + 1 : t, pf);
+    if (act) {
+      const int klo = tiles[t].kb_lo, khi = tiles[t].kb_hi;
+      f16_t acc;
 #pragma unroll
-    for (int q = 0; q < NB; ++q) {
+      for (int v = 0; v < 16; ++v) acc[v] = 0.f;
+#pragma unroll
+      for (int kb = 0; kb < KB; ++kb) {
+        if (kb >= klo && kb < khi) {
+          const h8_t ah = __builtin_bit_cast(h8_t, sa[kb * 2 * kWave + l]);
+          const h8_t al = __builtin_bit_cast(h8_t, sa[kb * 2 * kWave + kWave + l]);
+          acc = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, bh[kb], acc, 0, 0, 0);
+          acc = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, bl[kb], acc, 0, 0, 0);
+          acc = __builtin_amdgcn_mfma_f32_32x32x16_f16(al, bh[kb], acc, 0, 0, 0);
+        }
+      }
       float fx = 0.f, fy = 0.f, fz = 0.f;
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
-        if (2 * j + g < nc) {
-          float2 x = xv[q][j], y = yv[q][j];
-          if (hx[j] < 0) x.y = -x.y;
-          if (hy[j] < 0) y.y = -y.y;
-          const float qr = x.x * y.x - x.y * y.y, qi = x.x * y.y + x.y * y.x;
-          const float G = qr * acc[q][4 * j] - qi * acc[q][4 * j + 1];
-          const float Gz = qr * acc[q][4 * j + 2] - qi * acc[q][4 * j + 3];
-          fx += cw[j].x * G;
-          fy += cw[j].y * G;
-          fz += cw[j].z * Gz;
-        }
+        // X[-h] = conj X[h]: the sign of hx (hy) is that of cw.x (cw.y)
+        const float xr = xv[j].x, xi = flip_sign(xv[j].y, cw[j].x);
+        const float yr = yv[j].x, yi = flip_sign(yv[j].y, cw[j].y);
+        const float qr = xr * yr - xi * yi, qi = xr * yi + xi * yr;
+        const float G = qr * acc[4 * j] - qi * acc[4 * j + 1];
+        const float Gz = qr * acc[4 * j + 2] - qi * acc[4 * j + 3];
+        fx += cw[j].x * G;
+        fy += cw[j].y * G;
+        fz += cw[j].z * Gz;
       }
-      F[q][0] += (double)fx;
-      F[q][1] += (double)fy;
-      F[q][2] += (double)fz;
+      F0 += (double)fx;
+      F1 += (double)fy;
+      F2 += (double)fz;
     }
+    if (more) store(t + 1, buf ^ 1, pf);
+    __syncthreads();
   }
   // the two lanes of an ion hold the two halves of every tile's columns
-#pragma unroll
-  for (int q = 0; q < NB; ++q) {
-    const double fx = F[q][0] + __shfl_xor(F[q][0], 32);
-    const double fy = F[q][1] + __shfl_xor(F[q][1], 32);
-    const double fz = F[q][2] + __shfl_xor(F[q][2], 32);
-    const int i = base + 32 * q + n;
-    if (g == 0 && i < N) {
-      const double zq = B.Z[B.typ[i]];
-      double* fp = B.fpart + (size_t)it.d * 3 * N;
-      fp[i] = -zq * fx * B.invL[0];
-      fp[N + i] = -zq * fy * B.invL[1];
-      fp[2 * N + i] = -zq * fz * B.invL[2];
-    }
+  const double fx = F0 + __shfl_xor(F0, 32);
+  const double fy = F1 + __shfl_xor(F1, 32);
+  const double fz = F2 + __shfl_xor(F2, 32);
+  if (act && g == 0 && i < N) {
+    const double zq = B.Z[B.typ[i]];
+    double* fp = B.fpart + (size_t)it.d * 3 * N;
+    fp[i] = -zq * fx * B.invL[0];
+    fp[N + i] = -zq * fy * B.invL[1];
+    fp[2 * N + i] = -zq * fz * B.invL[2];
   }
 }
-
-// ion blocks per wave for a K-block count: one (measured faster than two,
-// which halves occupancy)
-constexpr int kf_blocks_for(int kb) { return kb > 0 ? 1 : 1; }
 
 }  // namespace mdg
 

@@ -139,7 +139,11 @@ struct BoxDesc {
   // Packed A operands per tile and K block: [ntile][kbk][hi, lo][64 lanes]
   // 16-byte lane records; rows are (column, {Re U, Im U, Re V, Im V}).
   void *kfa;
-  float4 *colw;       // per column: hx/sU, hy/sU, 1/sV, (hx, hy) as packed int16
+  // Epilogue records per tile, in lane order [ntile][half g][j]: column
+  // 2 j + g as (hx/sU, hy/sU, 1/sV, |hx| | |hy| << 8); all zero for the
+  // padding columns of a short tile.  The signs of hx and hy are those of
+  // the first two fields.
+  float4 *colw;
   const KfTile *kft;  // [ntile]
   int kbk, ntile;
   // ---- structure factor on MFMA (mixed path) ----
