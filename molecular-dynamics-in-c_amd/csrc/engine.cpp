@@ -663,6 +663,61 @@ void BoxSet::invalidate_graph() {
   graph_steps_ = 0;
 }
 
+// The workgroups of one launch go to the chip's XCDs in turn (workgroup w
+// to XCD w % kXcds), and each XCD has its own L2.  This reorders a launch's
+// items (a permutation: every item's work and output slot stay the same)
+// so that the items of one box sit at positions of one residue mod kXcds,
+// i.e. run on one XCD and share its L2.  Boxes are dealt, largest first, to
+// the XCD with the fewest items; a box with more than 1/16 of all items is
+// spread over every XCD instead, so that a lone large box still fills the
+// chip.  Positions an XCD has no items left for take the longest queue's.
+static void xcd_order(std::vector<Item>& v) {
+  constexpr int kXcds = 8;
+  if (v.size() < 4 * kXcds) return;
+  std::vector<int> first;  // boxes in order of first appearance
+  std::map<int, std::vector<Item>> by;
+  for (const Item& it : v) {
+    std::vector<Item>& q = by[it.box];
+    if (q.empty()) first.push_back(it.box);
+    q.push_back(it);
+  }
+  if (first.size() < 2) return;
+  std::vector<std::vector<Item>> qx(kXcds);
+  size_t next = 0;  // round-robin cursor for spread boxes
+  std::vector<int> grouped;
+  for (int b : first) {
+    const std::vector<Item>& q = by[b];
+    if (q.size() * 16 > v.size()) {
+      for (const Item& it : q) qx[next++ % kXcds].push_back(it);
+    } else {
+      grouped.push_back(b);
+    }
+  }
+  std::stable_sort(grouped.begin(), grouped.end(),
+                   [&](int a, int b) { return by[a].size() > by[b].size(); });
+  for (int b : grouped) {
+    int x = 0;
+    for (int y = 1; y < kXcds; ++y)
+      if (qx[y].size() < qx[x].size()) x = y;
+    qx[x].insert(qx[x].end(), by[b].begin(), by[b].end());
+  }
+  std::vector<Item> out;
+  out.reserve(v.size());
+  std::vector<size_t> pos(kXcds, 0);
+  while (out.size() < v.size()) {
+    for (int x = 0; x < kXcds && out.size() < v.size(); ++x) {
+      int y = x;
+      if (pos[x] == qx[x].size()) {  // borrow from the longest remaining queue
+        size_t best = 0;
+        for (int z = 0; z < kXcds; ++z)
+          if (qx[z].size() - pos[z] > best) { best = qx[z].size() - pos[z]; y = z; }
+      }
+      out.push_back(qx[y][pos[y]++]);
+    }
+  }
+  v.swap(out);
+}
+
 int BoxSet::set_boxes(const std::vector<Box*>& boxes) {
   invalidate_graph();
   boxes_ = boxes;
@@ -735,6 +790,7 @@ int BoxSet::set_boxes(const std::vector<Box*>& boxes) {
   hz_ = kforce_hz_for(hmax);
   std::vector<Item> items;
   auto app = [&](std::vector<Item>& v, int& off, int& n) {
+    xcd_order(v);
     off = (int)items.size();
     n = (int)v.size();
     items.insert(items.end(), v.begin(), v.end());
