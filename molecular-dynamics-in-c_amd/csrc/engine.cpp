@@ -650,9 +650,39 @@ BoxSet::BoxSet(int precision, int real_mode) : precision_(precision), real_mode_
 
 BoxSet::~BoxSet() {
   invalidate_graph();
+  if (side_) {
+    (void)hipStreamSynchronize(side_);
+    (void)hipStreamDestroy(side_);
+  }
+  if (ev_fork_) (void)hipEventDestroy(ev_fork_);
+  if (ev_join_) (void)hipEventDestroy(ev_join_);
   if (d_desc_) (void)hipFree(d_desc_);
   if (d_items_) (void)hipFree(d_items_);
   if (d_ids_) (void)hipFree(d_ids_);
+}
+
+// The side stream of the real-space branch and its fork/join events,
+// created on first use (default priority).  False if they cannot be made;
+// the step then runs on one stream.
+bool BoxSet::ensure_side_stream() {
+  if (side_) return true;
+  if (side_failed_) return false;
+  hipStream_t st = nullptr;
+  hipEvent_t a = nullptr, b = nullptr;
+  if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&a, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&b, hipEventDisableTiming) != hipSuccess) {
+    if (st) (void)hipStreamDestroy(st);
+    if (a) (void)hipEventDestroy(a);
+    if (b) (void)hipEventDestroy(b);
+    (void)hipGetLastError();
+    side_failed_ = true;
+    return false;
+  }
+  side_ = st;
+  ev_fork_ = a;
+  ev_join_ = b;
+  return true;
 }
 
 void BoxSet::invalidate_graph() {
@@ -896,10 +926,26 @@ void BoxSet::enqueue_force_terms(hipStream_t s, int terms) {
   launch_force_vv(false, d_desc_, d_items_ + off_ion_, n_ion_, s, terms);
 }
 
+// The real-space branch (cell lists, pair forces) and the reciprocal chain
+// read the same positions and write disjoint buffers (f/f0 or the split
+// partials; tables, S(k), the back-projection partials), so the real-space
+// branch runs on a side stream between a fork and a join event.  Both are
+// latency-bound and share the chip better than they run back to back;
+// captured into the step graph, the fork and join become graph edges.
 void BoxSet::enqueue_step(hipStream_t s, bool vv) {
-  launch_cell_build(d_desc_, d_ids_ + off_cellid_, n_cellid_, s);
-  launch_real(s, 0);
-  if (n_sk_ > 0) launch_recip(s);
+  if (n_sk_ > 0 && ensure_side_stream()) {
+    (void)hipEventRecord(ev_fork_, s);
+    (void)hipStreamWaitEvent(side_, ev_fork_, 0);
+    launch_cell_build(d_desc_, d_ids_ + off_cellid_, n_cellid_, side_);
+    launch_real(side_, 0);
+    (void)hipEventRecord(ev_join_, side_);
+    launch_recip(s);
+    (void)hipStreamWaitEvent(s, ev_join_, 0);
+  } else {
+    launch_cell_build(d_desc_, d_ids_ + off_cellid_, n_cellid_, s);
+    launch_real(s, 0);
+    if (n_sk_ > 0) launch_recip(s);
+  }
   launch_force_vv(vv, d_desc_, d_items_ + off_ion_, n_ion_, s);
   launch_box_scalars(d_desc_, d_ids_ + off_allid_, n_allid_, vv ? 1 : 0, s);
 }

@@ -151,6 +151,11 @@ class BoxSet {
   hipGraphExec_t graph_exec_ = nullptr;
   hipGraph_t graph_ = nullptr;
   int graph_steps_ = 0;
+  // real-space branch of a step (see enqueue_step)
+  hipStream_t side_ = nullptr;
+  hipEvent_t ev_fork_ = nullptr, ev_join_ = nullptr;
+  bool side_failed_ = false;
+  bool ensure_side_stream();
   void launch_real(hipStream_t s, int mode);
   void launch_recip(hipStream_t s);
 };
