@@ -650,38 +650,42 @@ BoxSet::BoxSet(int precision, int real_mode) : precision_(precision), real_mode_
 
 BoxSet::~BoxSet() {
   invalidate_graph();
-  if (side_) {
-    (void)hipStreamSynchronize(side_);
-    (void)hipStreamDestroy(side_);
+  for (hipStream_t st : side_) {
+    (void)hipStreamSynchronize(st);
+    (void)hipStreamDestroy(st);
   }
+  for (hipEvent_t ev : ev_join_) (void)hipEventDestroy(ev);
   if (ev_fork_) (void)hipEventDestroy(ev_fork_);
-  if (ev_join_) (void)hipEventDestroy(ev_join_);
   if (d_desc_) (void)hipFree(d_desc_);
   if (d_items_) (void)hipFree(d_items_);
   if (d_ids_) (void)hipFree(d_ids_);
 }
 
-// The side stream of the real-space branch and its fork/join events,
+// Side streams of a step's concurrent branches and their fork/join events,
 // created on first use (default priority).  False if they cannot be made;
 // the step then runs on one stream.
-bool BoxSet::ensure_side_stream() {
-  if (side_) return true;
+bool BoxSet::ensure_side_streams(int count) {
+  if ((int)side_.size() >= count) return true;
   if (side_failed_) return false;
-  hipStream_t st = nullptr;
-  hipEvent_t a = nullptr, b = nullptr;
-  if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess ||
-      hipEventCreateWithFlags(&a, hipEventDisableTiming) != hipSuccess ||
-      hipEventCreateWithFlags(&b, hipEventDisableTiming) != hipSuccess) {
-    if (st) (void)hipStreamDestroy(st);
-    if (a) (void)hipEventDestroy(a);
-    if (b) (void)hipEventDestroy(b);
+  if (!ev_fork_ && hipEventCreateWithFlags(&ev_fork_, hipEventDisableTiming) != hipSuccess) {
+    ev_fork_ = nullptr;
     (void)hipGetLastError();
     side_failed_ = true;
     return false;
   }
-  side_ = st;
-  ev_fork_ = a;
-  ev_join_ = b;
+  while ((int)side_.size() < count) {
+    hipStream_t st = nullptr;
+    hipEvent_t ev = nullptr;
+    if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess) {
+      if (st) (void)hipStreamDestroy(st);
+      (void)hipGetLastError();
+      side_failed_ = true;
+      return false;
+    }
+    side_.push_back(st);
+    ev_join_.push_back(ev);
+  }
   return true;
 }
 
@@ -757,9 +761,33 @@ int BoxSet::set_boxes(const std::vector<Box*>& boxes) {
   // big boxes' items first, so the longest tiles start earliest
   std::stable_sort(order.begin(), order.end(),
                    [&](int a, int b) { return boxes_[a]->desc.N > boxes_[b]->desc.N; });
-  std::vector<Item> real, tab, sk, fin, kf, ion, pack, sbt;
+  // Box groups whose step chains run concurrently: with enough boxes, two
+  // groups of about equal work (ions x (k vectors + partners)), dealt
+  // largest first to the lighter group.
+  const int G = nb >= 2 * kStepGroups * 4 ? kStepGroups : 1;
+  std::vector<int> group_of(nb, 0);
+  {
+    // group 0 starts its reciprocal chain at once, the others after their
+    // real space; kGroup0Share is group 0's share of the work (an equal
+    // split measured best on the 256-box batch)
+    std::vector<double> load(G, 0.0), share(G, G > 1 ? (1.0 - kGroup0Share) / (G - 1) : 1.0);
+    share[0] = G > 1 ? kGroup0Share : 1.0;
+    for (int bi : order) {
+      const BoxPlan& P = boxes_[bi]->plan;
+      const double N = (double)boxes_[bi]->desc.N;
+      const double w = N * ((P.ewald ? (double)P.nk : 0.0) + 2.0 * std::min(N, 200.0));
+      int g = 0;
+      for (int h = 1; h < G; ++h)
+        if (load[h] / share[h] < load[g] / share[g]) g = h;
+      group_of[bi] = g;
+      load[g] += w;
+    }
+  }
+  std::vector<std::vector<Item>> L[kLists];
+  std::vector<std::vector<int>> I[kIdLists];
+  for (auto& v : L) v.assign(G, {});
+  for (auto& v : I) v.assign(G, {});
   skn_ = 1;
-  std::vector<int> cellid, allid;
   int hmax = 0;
   const bool mixed = precision_ == MDG_MIXED;
   // the widest box fixes the back-projection's K blocks, hence its ion blocks
@@ -772,21 +800,23 @@ int BoxSet::set_boxes(const std::vector<Box*>& boxes) {
     const BoxPlan& P = B->plan;
     const int N = B->desc.N;
     const int rs = P.tile.rsplit;
-    allid.push_back(bi);
+    const int g = group_of[bi];
+    std::vector<Item>& real = L[LReal][g];
+    I[IAll][g].push_back(bi);
     if (P.all_pairs) {
       int slot = 0;
       for (int i0 = 0; i0 < N; i0 += kRealThreads)
         for (int s = 0; s < rs; ++s) real.push_back({bi, i0, slot++, s, 0});
     } else {
-      cellid.push_back(bi);
+      I[ICell][g].push_back(bi);
       int slot = 0;
       for (int c = 0; c < P.ncell; ++c)
         for (int s = 0; s < rs; ++s) real.push_back({bi, c, slot++, s, 0});
     }
-    for (int i0 = 0; i0 < N; i0 += kIonThreads) ion.push_back({bi, i0, 0, 0, 0});
+    for (int i0 = 0; i0 < N; i0 += kIonThreads) L[LIon][g].push_back({bi, i0, 0, 0, 0});
     if (P.ewald) {
       hmax = std::max(hmax, P.hmax);
-      for (int i0 = 0; i0 < N; i0 += kIonThreads) tab.push_back({bi, i0, 0, 0, 0});
+      for (int i0 = 0; i0 < N; i0 += kIonThreads) L[LTab][g].push_back({bi, i0, 0, 0, 0});
       if (mixed) {
         // MFMA structure factor: 4 waves x 32 columns per item
         // (two hz tiles of 16 per item)
@@ -794,52 +824,55 @@ int BoxSet::set_boxes(const std::vector<Box*>& boxes) {
         const int nt = (P.hmax + 16) / 16, npad = (N + 7) / 8 * 8;
         for (int ch = 0; ch < P.tile.nchunk; ++ch)
           for (int c0 = 0; c0 < P.ncol; c0 += cw)
-            for (int t0 = 0; t0 < nt; t0 += 2) sk.push_back({bi, ch, c0, t0, 0});
+            for (int t0 = 0; t0 < nt; t0 += 2) L[LSk][g].push_back({bi, ch, c0, t0, 0});
         const int nrec = npad / 4 * 16 * nt;
-        for (int e0 = 0; e0 < nrec; e0 += kIonThreads) sbt.push_back({bi, e0, 0, 0, 0});
+        for (int e0 = 0; e0 < nrec; e0 += kIonThreads) L[LSbt][g].push_back({bi, e0, 0, 0, 0});
         skn_ = std::max(skn_, P.hmax + 1);
       } else {
         for (int ch = 0; ch < P.tile.nchunk; ++ch)
           for (int s0 = 0; s0 < P.nseg; s0 += kSkThreads)
-            sk.push_back({bi, ch, s0, std::min(kSkThreads, P.nseg - s0), 0});
+            L[LSk][g].push_back({bi, ch, s0, std::min(kSkThreads, P.nseg - s0), 0});
       }
-      for (int k0 = 0; k0 < P.nk; k0 += kIonThreads) fin.push_back({bi, k0, 0, 0, 0});
+      for (int k0 = 0; k0 < P.nk; k0 += kIonThreads) L[LFin][g].push_back({bi, k0, 0, 0, 0});
       const int gk = (int)P.kf_group_start.size() - 1;
       const int ion_step = mixed ? kf_ions : kKfThreads;
       for (int i0 = 0; i0 < N; i0 += ion_step)
-        for (int g = 0; g < gk; ++g)
-          kf.push_back({bi, i0, P.kf_group_start[g],
-                        P.kf_group_start[g + 1] - P.kf_group_start[g], g});
+        for (int q = 0; q < gk; ++q)
+          L[LKf][g].push_back({bi, i0, P.kf_group_start[q],
+                               P.kf_group_start[q + 1] - P.kf_group_start[q], q});
       if (mixed)
-        for (int t = 0; t < (int)P.kftiles.size(); ++t) pack.push_back({bi, t, 0, 0, 0});
+        for (int t = 0; t < (int)P.kftiles.size(); ++t)
+          L[LPack][g].push_back({bi, t, 0, 0, 0});
     }
   }
   const int H = hmax + 1;
   hs_ = H | 1;                      // odd strides spread lanes over LDS banks
   hsw_ = (H + kSegLen - 1) / kSegLen * kSegLen;  // segment reads stay 16-byte aligned
   hz_ = kforce_hz_for(hmax);
+  // each list: its groups' lists back to back (all_ spans them, grp_ each)
   std::vector<Item> items;
-  auto app = [&](std::vector<Item>& v, int& off, int& n) {
-    xcd_order(v);
-    off = (int)items.size();
-    n = (int)v.size();
-    items.insert(items.end(), v.begin(), v.end());
-  };
-  app(real, off_real_, n_real_);
-  app(tab, off_tab_, n_tab_);
-  app(sk, off_sk_, n_sk_);
-  app(fin, off_fin_, n_fin_);
-  app(kf, off_kf_, n_kf_);
-  app(ion, off_ion_, n_ion_);
-  app(pack, off_pack_, n_pack_);
-  app(sbt, off_sbt_, n_sbt_);
   std::vector<int> ids;
-  off_cellid_ = 0;
-  n_cellid_ = (int)cellid.size();
-  ids.insert(ids.end(), cellid.begin(), cellid.end());
-  off_allid_ = (int)ids.size();
-  n_allid_ = (int)allid.size();
-  ids.insert(ids.end(), allid.begin(), allid.end());
+  all_ = Ranges();
+  grp_.assign(G, Ranges());
+  for (int l = 0; l < kLists; ++l) {
+    all_.off[l] = (int)items.size();
+    for (int g = 0; g < G; ++g) {
+      xcd_order(L[l][g]);
+      grp_[g].off[l] = (int)items.size();
+      grp_[g].n[l] = (int)L[l][g].size();
+      items.insert(items.end(), L[l][g].begin(), L[l][g].end());
+    }
+    all_.n[l] = (int)items.size() - all_.off[l];
+  }
+  for (int l = 0; l < kIdLists; ++l) {
+    all_.ioff[l] = (int)ids.size();
+    for (int g = 0; g < G; ++g) {
+      grp_[g].ioff[l] = (int)ids.size();
+      grp_[g].in[l] = (int)I[l][g].size();
+      ids.insert(ids.end(), I[l][g].begin(), I[l][g].end());
+    }
+    all_.in[l] = (int)ids.size() - all_.ioff[l];
+  }
   if (items.size() > items_cap_) {
     if (d_items_) (void)hipFree(d_items_);
     d_items_ = nullptr;
@@ -880,86 +913,101 @@ int BoxSet::sync_desc(hipStream_t s) {
   return MDG_OK;
 }
 
-void BoxSet::launch_real(hipStream_t s, int mode) {
+// cell lists (cell-list boxes) and real-space forces of the boxes in r
+void BoxSet::launch_cells_real(hipStream_t s, int mode, const Ranges& r) {
   const bool analytic = real_mode_ == MDG_ANALYTIC;
+  launch_cell_build(d_desc_, d_ids_ + r.ioff[ICell], r.in[ICell], s);
   if (precision_ == MDG_MIXED)
-    launch_real32(analytic, d_desc_, d_items_ + off_real_, n_real_, mode, s);
+    launch_real32(analytic, d_desc_, d_items_ + r.off[LReal], r.n[LReal], mode, s);
   else
-    launch_real64(analytic, d_desc_, d_items_ + off_real_, n_real_, mode, s);
+    launch_real64(analytic, d_desc_, d_items_ + r.off[LReal], r.n[LReal], mode, s);
 }
 
 // phase tables, partial structure factors, finalisation, back-projection
-void BoxSet::launch_recip(hipStream_t s) {
+void BoxSet::launch_recip(hipStream_t s, const Ranges& r) {
   const bool mixed = precision_ == MDG_MIXED;
-  launch_phase_tables(mixed, d_desc_, d_items_ + off_tab_, n_tab_, s);
+  const Item* it = d_items_;
+  launch_phase_tables(mixed, d_desc_, it + r.off[LTab], r.n[LTab], s);
   if (mixed) {
-    launch_wb_tables(d_desc_, d_items_ + off_tab_, n_tab_, s);
-    launch_sb_tables(d_desc_, d_items_ + off_sbt_, n_sbt_, s);
-    launch_sk_mfma(skn_, d_desc_, d_items_ + off_sk_, n_sk_, s);
-    launch_sk_finalize(d_desc_, d_items_ + off_fin_, n_fin_, s);
-    launch_kf_pack(d_desc_, d_items_ + off_pack_, n_pack_, s);
-    launch_kforce_mfma(hz_ / 8, d_desc_, d_items_ + off_kf_, n_kf_, s);
+    launch_wb_tables(d_desc_, it + r.off[LTab], r.n[LTab], s);
+    launch_sb_tables(d_desc_, it + r.off[LSbt], r.n[LSbt], s);
+    launch_sk_mfma(skn_, d_desc_, it + r.off[LSk], r.n[LSk], s);
+    launch_sk_finalize(d_desc_, it + r.off[LFin], r.n[LFin], s);
+    launch_kf_pack(d_desc_, it + r.off[LPack], r.n[LPack], s);
+    launch_kforce_mfma(hz_ / 8, d_desc_, it + r.off[LKf], r.n[LKf], s);
   } else {
-    launch_sk64(d_desc_, d_items_ + off_sk_, n_sk_, hs_, hsw_, s);
-    launch_sk_finalize(d_desc_, d_items_ + off_fin_, n_fin_, s);
-    launch_kforce64(hz_, d_desc_, d_items_ + off_kf_, n_kf_, s);
+    launch_sk64(d_desc_, it + r.off[LSk], r.n[LSk], hs_, hsw_, s);
+    launch_sk_finalize(d_desc_, it + r.off[LFin], r.n[LFin], s);
+    launch_kforce64(hz_, d_desc_, it + r.off[LKf], r.n[LKf], s);
   }
 }
 
 void BoxSet::enqueue_forces_parts(hipStream_t s, bool real, int real_write_mode, bool recip) {
-  if (real) {
-    launch_cell_build(d_desc_, d_ids_ + off_cellid_, n_cellid_, s);
-    launch_real(s, real_write_mode);
-  }
-  if (recip && n_sk_ > 0) {
-    launch_recip(s);
-    launch_force_vv(false, d_desc_, d_items_ + off_ion_, n_ion_, s);
+  if (real) launch_cells_real(s, real_write_mode, all_);
+  if (recip && all_.n[LSk] > 0) {
+    launch_recip(s, all_);
+    launch_force_vv(false, d_desc_, d_items_ + all_.off[LIon], all_.n[LIon], s);
   }
 }
 
 void BoxSet::enqueue_force_terms(hipStream_t s, int terms) {
-  if (terms & 1) {
-    launch_cell_build(d_desc_, d_ids_ + off_cellid_, n_cellid_, s);
-    launch_real(s, 0);
-  }
-  if ((terms & 2) && n_sk_ > 0) launch_recip(s);
-  launch_force_vv(false, d_desc_, d_items_ + off_ion_, n_ion_, s, terms);
+  if (terms & 1) launch_cells_real(s, 0, all_);
+  if ((terms & 2) && all_.n[LSk] > 0) launch_recip(s, all_);
+  launch_force_vv(false, d_desc_, d_items_ + all_.off[LIon], all_.n[LIon], s, terms);
 }
 
-// The real-space branch (cell lists, pair forces) and the reciprocal chain
-// read the same positions and write disjoint buffers (f/f0 or the split
-// partials; tables, S(k), the back-projection partials), so the real-space
-// branch runs on a side stream between a fork and a join event.  Both are
-// latency-bound and share the chip better than they run back to back;
-// captured into the step graph, the fork and join become graph edges.
+// Within a step, every box's real-space branch (cell lists, pair forces)
+// and reciprocal chain read the same positions and write disjoint buffers
+// (f/f0 or the split partials; tables, S(k), the back-projection
+// partials), and different boxes share nothing.  So they run on several
+// streams between a fork event and join events: the kernels are
+// latency-bound and share the chip better than they run back to back.
+// Group 0's reciprocal chain runs on the caller's stream and its real
+// space on side stream 0; group g > 0 runs its real space and then its
+// reciprocal chain on side stream g, so that different kinds of kernel
+// overlap (measured: 1.29 ms per step on one stream pair, 1.11-1.15 ms
+// staggered over two groups, slower with three).  Captured into the step
+// graph, the fork and the joins become graph edges.
 void BoxSet::enqueue_step(hipStream_t s, bool vv) {
-  if (n_sk_ > 0 && ensure_side_stream()) {
+  const int G = (int)grp_.size();
+  if (all_.n[LSk] > 0 && ensure_side_streams(G)) {
     (void)hipEventRecord(ev_fork_, s);
-    (void)hipStreamWaitEvent(side_, ev_fork_, 0);
-    launch_cell_build(d_desc_, d_ids_ + off_cellid_, n_cellid_, side_);
-    launch_real(side_, 0);
-    (void)hipEventRecord(ev_join_, side_);
-    launch_recip(s);
-    (void)hipStreamWaitEvent(s, ev_join_, 0);
+    for (int k = 0; k < G; ++k) (void)hipStreamWaitEvent(side_[k], ev_fork_, 0);
+    for (int g = 0; g < G; ++g) {
+      if (g == 0) {
+        launch_cells_real(side_[0], 0, grp_[0]);
+        launch_recip(s, grp_[0]);
+      } else {  // later groups: real space first, then their reciprocal chain
+        launch_cells_real(side_[g], 0, grp_[g]);
+        launch_recip(side_[g], grp_[g]);
+      }
+    }
+    for (int k = 0; k < G; ++k) {
+      (void)hipEventRecord(ev_join_[k], side_[k]);
+      (void)hipStreamWaitEvent(s, ev_join_[k], 0);
+    }
   } else {
-    launch_cell_build(d_desc_, d_ids_ + off_cellid_, n_cellid_, s);
-    launch_real(s, 0);
-    if (n_sk_ > 0) launch_recip(s);
+    launch_cells_real(s, 0, all_);
+    if (all_.n[LSk] > 0) launch_recip(s, all_);
   }
-  launch_force_vv(vv, d_desc_, d_items_ + off_ion_, n_ion_, s);
-  launch_box_scalars(d_desc_, d_ids_ + off_allid_, n_allid_, vv ? 1 : 0, s);
+  launch_force_vv(vv, d_desc_, d_items_ + all_.off[LIon], all_.n[LIon], s);
+  launch_box_scalars(d_desc_, d_ids_ + all_.ioff[IAll], all_.in[IAll], vv ? 1 : 0, s);
 }
 
 void BoxSet::enqueue_scalars(hipStream_t s, bool count_step) {
-  launch_box_scalars(d_desc_, d_ids_ + off_allid_, n_allid_, count_step ? 1 : 0, s);
+  launch_box_scalars(d_desc_, d_ids_ + all_.ioff[IAll], all_.in[IAll], count_step ? 1 : 0, s);
 }
 void BoxSet::enqueue_clear_foc(hipStream_t s) {
-  launch_clear_foc(d_desc_, d_items_ + off_ion_, n_ion_, s);
+  launch_clear_foc(d_desc_, d_items_ + all_.off[LIon], all_.n[LIon], s);
 }
-void BoxSet::enqueue_next_v(hipStream_t s) { launch_next_v(d_desc_, d_items_ + off_ion_, n_ion_, s); }
-void BoxSet::enqueue_next_r(hipStream_t s) { launch_next_r(d_desc_, d_items_ + off_ion_, n_ion_, s); }
+void BoxSet::enqueue_next_v(hipStream_t s) {
+  launch_next_v(d_desc_, d_items_ + all_.off[LIon], all_.n[LIon], s);
+}
+void BoxSet::enqueue_next_r(hipStream_t s) {
+  launch_next_r(d_desc_, d_items_ + all_.off[LIon], all_.n[LIon], s);
+}
 void BoxSet::enqueue_moment_correction(hipStream_t s) {
-  launch_moment_correction(d_desc_, d_ids_ + off_allid_, n_allid_, s);
+  launch_moment_correction(d_desc_, d_ids_ + all_.ioff[IAll], all_.in[IAll], s);
 }
 
 int BoxSet::step(int nsteps, hipStream_t s) {
@@ -1014,26 +1062,33 @@ int BoxSet::profile(int nsteps, hipStream_t s, std::vector<std::string>* names,
     which.push_back(k);
     e += 2;
   };
+  const Ranges& r = all_;
+  const Item* it = d_items_;
   for (int st = 0; st < nsteps; ++st) {
-    if (n_cellid_) rec(0, [&] { launch_cell_build(d_desc_, d_ids_ + off_cellid_, n_cellid_, s); });
-    rec(2, [&] { launch_real(s, 0); });
-    if (n_sk_) {
-      rec(1, [&] { launch_phase_tables(mixed, d_desc_, d_items_ + off_tab_, n_tab_, s); });
-      if (mixed) rec(8, [&] { launch_wb_tables(d_desc_, d_items_ + off_tab_, n_tab_, s); });
-      if (mixed) rec(10, [&] { launch_sb_tables(d_desc_, d_items_ + off_sbt_, n_sbt_, s); });
+    if (r.in[ICell])
+      rec(0, [&] { launch_cell_build(d_desc_, d_ids_ + r.ioff[ICell], r.in[ICell], s); });
+    rec(2, [&] {
+      const bool analytic = real_mode_ == MDG_ANALYTIC;
+      if (mixed) launch_real32(analytic, d_desc_, it + r.off[LReal], r.n[LReal], 0, s);
+      else launch_real64(analytic, d_desc_, it + r.off[LReal], r.n[LReal], 0, s);
+    });
+    if (r.n[LSk]) {
+      rec(1, [&] { launch_phase_tables(mixed, d_desc_, it + r.off[LTab], r.n[LTab], s); });
+      if (mixed) rec(8, [&] { launch_wb_tables(d_desc_, it + r.off[LTab], r.n[LTab], s); });
+      if (mixed) rec(10, [&] { launch_sb_tables(d_desc_, it + r.off[LSbt], r.n[LSbt], s); });
       rec(3, [&] {
-        if (mixed) launch_sk_mfma(skn_, d_desc_, d_items_ + off_sk_, n_sk_, s);
-        else launch_sk64(d_desc_, d_items_ + off_sk_, n_sk_, hs_, hsw_, s);
+        if (mixed) launch_sk_mfma(skn_, d_desc_, it + r.off[LSk], r.n[LSk], s);
+        else launch_sk64(d_desc_, it + r.off[LSk], r.n[LSk], hs_, hsw_, s);
       });
-      rec(4, [&] { launch_sk_finalize(d_desc_, d_items_ + off_fin_, n_fin_, s); });
-      if (mixed) rec(9, [&] { launch_kf_pack(d_desc_, d_items_ + off_pack_, n_pack_, s); });
+      rec(4, [&] { launch_sk_finalize(d_desc_, it + r.off[LFin], r.n[LFin], s); });
+      if (mixed) rec(9, [&] { launch_kf_pack(d_desc_, it + r.off[LPack], r.n[LPack], s); });
       rec(5, [&] {
-        if (mixed) launch_kforce_mfma(hz_ / 8, d_desc_, d_items_ + off_kf_, n_kf_, s);
-        else launch_kforce64(hz_, d_desc_, d_items_ + off_kf_, n_kf_, s);
+        if (mixed) launch_kforce_mfma(hz_ / 8, d_desc_, it + r.off[LKf], r.n[LKf], s);
+        else launch_kforce64(hz_, d_desc_, it + r.off[LKf], r.n[LKf], s);
       });
     }
-    rec(6, [&] { launch_force_vv(true, d_desc_, d_items_ + off_ion_, n_ion_, s); });
-    rec(7, [&] { launch_box_scalars(d_desc_, d_ids_ + off_allid_, n_allid_, 1, s); });
+    rec(6, [&] { launch_force_vv(true, d_desc_, it + r.off[LIon], r.n[LIon], s); });
+    rec(7, [&] { launch_box_scalars(d_desc_, d_ids_ + r.ioff[IAll], r.in[IAll], 1, s); });
   }
   MDG_HIP(hipStreamSynchronize(s));
   std::vector<double> tot(K, 0.0);

@@ -109,6 +109,10 @@ int box_download_state(Box* b, double* r, double* v, double* f, double* f0, hipS
 int box_read_scalars(Box* b, double* scal /*kScalCount*/, hipStream_t s);
 int box_read_error(Box* b, int* err, hipStream_t s);
 
+// Box groups of a batch whose step chains run concurrently (16+ boxes).
+constexpr int kStepGroups = 2;
+constexpr double kGroup0Share = 0.5;  // share of the work in the group that starts k-space first
+
 // A set of boxes stepped together (one launch per kernel per step).
 class BoxSet {
  public:
@@ -142,22 +146,28 @@ class BoxSet {
   int* d_ids_ = nullptr;
   size_t items_cap_ = 0, ids_cap_ = 0, desc_cap_ = 0;
   // offsets into d_items_ / d_ids_ and counts
-  int off_real_ = 0, n_real_ = 0, off_tab_ = 0, n_tab_ = 0, off_sk_ = 0, n_sk_ = 0;
-  int off_fin_ = 0, n_fin_ = 0, off_kf_ = 0, n_kf_ = 0, off_ion_ = 0, n_ion_ = 0;
-  int off_cellid_ = 0, n_cellid_ = 0, off_allid_ = 0, n_allid_ = 0;
-  int off_pack_ = 0, n_pack_ = 0, off_sbt_ = 0, n_sbt_ = 0;
+  // Work lists: item ranges in d_items_ per kernel, box-id ranges in d_ids_.
+  enum { LReal, LTab, LSk, LFin, LKf, LIon, LPack, LSbt, kLists };
+  enum { ICell, IAll, kIdLists };
+  struct Ranges {
+    int off[kLists] = {}, n[kLists] = {};
+    int ioff[kIdLists] = {}, in[kIdLists] = {};
+  };
+  Ranges all_;              // every box (each list is its groups' lists back to back)
+  std::vector<Ranges> grp_; // the box groups whose chains a step runs concurrently
   int skn_ = 1;  // hmax + 1 of the widest box (MFMA structure factor LDS tiles)
   int hs_ = 1, hsw_ = 8, hz_ = 8;  // LDS strides / register row of the k-space kernels
   hipGraphExec_t graph_exec_ = nullptr;
   hipGraph_t graph_ = nullptr;
   int graph_steps_ = 0;
-  // real-space branch of a step (see enqueue_step)
-  hipStream_t side_ = nullptr;
-  hipEvent_t ev_fork_ = nullptr, ev_join_ = nullptr;
+  // side streams of a step's concurrent branches (see enqueue_step)
+  std::vector<hipStream_t> side_;
+  std::vector<hipEvent_t> ev_join_;
+  hipEvent_t ev_fork_ = nullptr;
   bool side_failed_ = false;
-  bool ensure_side_stream();
-  void launch_real(hipStream_t s, int mode);
-  void launch_recip(hipStream_t s);
+  bool ensure_side_streams(int count);
+  void launch_cells_real(hipStream_t s, int mode, const Ranges& r);
+  void launch_recip(hipStream_t s, const Ranges& r);
 };
 
 void set_last_hip_error(hipError_t e, const char* what);
