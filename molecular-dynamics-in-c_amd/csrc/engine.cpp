@@ -821,12 +821,10 @@ int BoxSet::set_boxes(const std::vector<Box*>& boxes) {
         // MFMA structure factor: 4 waves x 32 columns per item
         // (two hz tiles of 16 per item)
         const int cw = 32 * kKfMfmaWaves;
-        const int nt = (P.hmax + 16) / 16, npad = (N + 7) / 8 * 8;
+        const int nt = (P.hmax + 16) / 16;
         for (int ch = 0; ch < P.tile.nchunk; ++ch)
           for (int c0 = 0; c0 < P.ncol; c0 += cw)
             for (int t0 = 0; t0 < nt; t0 += 2) L[LSk][g].push_back({bi, ch, c0, t0, 0});
-        const int nrec = npad / 4 * 16 * nt;
-        for (int e0 = 0; e0 < nrec; e0 += kIonThreads) L[LSbt][g].push_back({bi, e0, 0, 0, 0});
         skn_ = std::max(skn_, P.hmax + 1);
       } else {
         for (int ch = 0; ch < P.tile.nchunk; ++ch)
@@ -927,15 +925,14 @@ void BoxSet::launch_cells_real(hipStream_t s, int mode, const Ranges& r) {
 void BoxSet::launch_recip(hipStream_t s, const Ranges& r) {
   const bool mixed = precision_ == MDG_MIXED;
   const Item* it = d_items_;
-  launch_phase_tables(mixed, d_desc_, it + r.off[LTab], r.n[LTab], s);
   if (mixed) {
-    launch_wb_tables(d_desc_, it + r.off[LTab], r.n[LTab], s);
-    launch_sb_tables(d_desc_, it + r.off[LSbt], r.n[LSbt], s);
+    launch_mixed_tables(d_desc_, it + r.off[LTab], r.n[LTab], s);
     launch_sk_mfma(skn_, d_desc_, it + r.off[LSk], r.n[LSk], s);
     launch_sk_finalize(d_desc_, it + r.off[LFin], r.n[LFin], s);
     launch_kf_pack(d_desc_, it + r.off[LPack], r.n[LPack], s);
     launch_kforce_mfma(hz_ / 8, d_desc_, it + r.off[LKf], r.n[LKf], s);
   } else {
+    launch_phase_tables(false, d_desc_, it + r.off[LTab], r.n[LTab], s);
     launch_sk64(d_desc_, it + r.off[LSk], r.n[LSk], hs_, hsw_, s);
     launch_sk_finalize(d_desc_, it + r.off[LFin], r.n[LFin], s);
     launch_kforce64(hz_, d_desc_, it + r.off[LKf], r.n[LKf], s);
@@ -1046,10 +1043,12 @@ int BoxSet::profile(int nsteps, hipStream_t s, std::vector<std::string>* names,
   int rc = sync_desc(s);
   if (rc != MDG_OK) return rc;
   const bool mixed = precision_ == MDG_MIXED;
-  constexpr int K = 11;
+  // mixed path: "phase_tables" is k_mixed_tables (the phase tables and
+  // both passes' MFMA B operands)
+  constexpr int K = 9;
   const char* kn[K] = {"cell_build", "phase_tables", "real_space", "sk_partial",
                        "sk_finalize", "kforce",      "force_vv",   "box_scalars",
-                       "wb_tables",  "kf_pack",      "sb_tables"};
+                       "kf_pack"};
   std::vector<hipEvent_t> ev(2 * K * std::max(1, nsteps));
   for (auto& e : ev) MDG_HIP(hipEventCreate(&e));
   std::vector<int> used(K, 0), which;
@@ -1073,15 +1072,16 @@ int BoxSet::profile(int nsteps, hipStream_t s, std::vector<std::string>* names,
       else launch_real64(analytic, d_desc_, it + r.off[LReal], r.n[LReal], 0, s);
     });
     if (r.n[LSk]) {
-      rec(1, [&] { launch_phase_tables(mixed, d_desc_, it + r.off[LTab], r.n[LTab], s); });
-      if (mixed) rec(8, [&] { launch_wb_tables(d_desc_, it + r.off[LTab], r.n[LTab], s); });
-      if (mixed) rec(10, [&] { launch_sb_tables(d_desc_, it + r.off[LSbt], r.n[LSbt], s); });
+      rec(1, [&] {
+        if (mixed) launch_mixed_tables(d_desc_, it + r.off[LTab], r.n[LTab], s);
+        else launch_phase_tables(false, d_desc_, it + r.off[LTab], r.n[LTab], s);
+      });
       rec(3, [&] {
         if (mixed) launch_sk_mfma(skn_, d_desc_, it + r.off[LSk], r.n[LSk], s);
         else launch_sk64(d_desc_, it + r.off[LSk], r.n[LSk], hs_, hsw_, s);
       });
       rec(4, [&] { launch_sk_finalize(d_desc_, it + r.off[LFin], r.n[LFin], s); });
-      if (mixed) rec(9, [&] { launch_kf_pack(d_desc_, it + r.off[LPack], r.n[LPack], s); });
+      if (mixed) rec(8, [&] { launch_kf_pack(d_desc_, it + r.off[LPack], r.n[LPack], s); });
       rec(5, [&] {
         if (mixed) launch_kforce_mfma(hz_ / 8, d_desc_, it + r.off[LKf], r.n[LKf], s);
         else launch_kforce64(hz_, d_desc_, it + r.off[LKf], r.n[LKf], s);

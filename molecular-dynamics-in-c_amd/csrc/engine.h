@@ -147,7 +147,7 @@ class BoxSet {
   size_t items_cap_ = 0, ids_cap_ = 0, desc_cap_ = 0;
   // offsets into d_items_ / d_ids_ and counts
   // Work lists: item ranges in d_items_ per kernel, box-id ranges in d_ids_.
-  enum { LReal, LTab, LSk, LFin, LKf, LIon, LPack, LSbt, kLists };
+  enum { LReal, LTab, LSk, LFin, LKf, LIon, LPack, kLists };
   enum { ICell, IAll, kIdLists };
   struct Ranges {
     int off[kLists] = {}, n[kLists] = {};

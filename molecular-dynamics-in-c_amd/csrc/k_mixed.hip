@@ -20,8 +20,8 @@ void launch_real32(bool analytic, const BoxDesc* boxes, const Item* items, int n
     hipLaunchKernelGGL(k_real_win<kRealTable32>, MDG_GRID(n, kRealThreads), boxes, items, mode);
 }
 
-void launch_wb_tables(const BoxDesc* boxes, const Item* items, int n, hipStream_t s) {
-  if (n > 0) hipLaunchKernelGGL(k_wb_tables, MDG_GRID(n, kIonThreads), boxes, items);
+void launch_mixed_tables(const BoxDesc* boxes, const Item* items, int n, hipStream_t s) {
+  if (n > 0) hipLaunchKernelGGL(k_mixed_tables, MDG_GRID(n, kIonThreads), boxes, items);
 }
 
 void launch_kf_pack(const BoxDesc* boxes, const Item* items, int n, hipStream_t s) {
@@ -45,10 +45,6 @@ void launch_kforce_mfma(int kb, const BoxDesc* boxes, const Item* items, int n, 
 }
 
 int kforce_mfma_ions(int) { return 32 * kKfMfmaWaves; }
-
-void launch_sb_tables(const BoxDesc* boxes, const Item* items, int n, hipStream_t s) {
-  if (n > 0) hipLaunchKernelGGL(k_sb_tables, MDG_GRID(n, kIonThreads), boxes, items);
-}
 
 void launch_sk_mfma(int hl, const BoxDesc* boxes, const Item* items, int n, hipStream_t s) {
   if (n <= 0) return;

@@ -686,7 +686,11 @@ __global__ __launch_bounds__(kRealThreads) void k_real_win(const BoxDesc* __rest
         const bool live = cur != 0u;
         const int b = __builtin_clz(cur | 1u);
         cur &= ~(0x80000000u >> b);
-        const WinRec P = pr[w * 32 + b];
+        // A lane out of candidates reads slot 0, which every window stages:
+        // its discarded pair is weighted by zero, and a slot the window did
+        // not stage may hold another kernel's LDS bits (a NaN as FP64 would
+        // survive the multiplication by zero).
+        const WinRec P = pr[live ? w * 32 + b : 0];
         if (live && cur == 0u) {
           nz &= ~(1u << w);
           if (nz) { w = __builtin_ctz(nz); cur = pm[w][lane]; }

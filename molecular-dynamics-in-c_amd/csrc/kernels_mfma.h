@@ -60,37 +60,101 @@ __device__ __forceinline__ u4_t pack_h8(const _Float16 (&v)[8]) {
   return __builtin_bit_cast(u4_t, h);
 }
 
+// two FP16 halves in one 32-bit word, a in the low half
+__device__ __forceinline__ unsigned pack2_f16(_Float16 a, _Float16 b) {
+  return (unsigned)__builtin_bit_cast(unsigned short, a) |
+         ((unsigned)__builtin_bit_cast(unsigned short, b) << 16);
+}
+
 // ----------------------------------------------------------------------
-// B operands: W[hz] of every ion as FP16 hi/lo records, [hz/4][N] x 32 B
-// (8 hi halves: re, im of hz = 4q .. 4q+3; then the 8 lo halves).
+// Every per-step ion table of the mixed k-space, one thread per ion, each
+// phase computed once (FP64 sincospi, rounded to FP32):
+//  * X[h], Y[h] (FP32 complex, h <= hmax) for both passes;
+//  * the back-projection's B operands: W[hz] as FP16 hi/lo records,
+//    [hz/4][N] x 32 B (8 hi halves: re, im of hz = 4q .. 4q+3; then the 8
+//    lo halves), zero past hmax;
+//  * the structure factor's B operands: per group of 4 ions and hz, four
+//    16-byte vectors (Re S hi, Re S lo, Im S hi, Im S lo) holding the 4
+//    ions' (Z W_r, -Z W_i) or (Z W_i, Z W_r) halves, zero past hmax and for
+//    the padding ions N .. npad-1.  A group's 4 threads exchange their
+//    halves through LDS, so each vector is one 16-byte store.
 // ----------------------------------------------------------------------
-__global__ __launch_bounds__(kIonThreads) void k_wb_tables(const BoxDesc* __restrict__ boxes,
-                                                           const Item* __restrict__ items) {
+__global__ __launch_bounds__(kIonThreads) void k_mixed_tables(const BoxDesc* __restrict__ boxes,
+                                                              const Item* __restrict__ items) {
+  __shared__ unsigned sbx[2][4][kIonThreads];  // by hz parity: vector x thread
   const Item it = items[blockIdx.x];
   const BoxDesc& B = boxes[it.box];
-  const int N = B.N;
-  const int i = it.a + threadIdx.x;
-  if (i >= N) return;
-  const double sz = B.rz[i] * B.invL[2];
+  const int N = B.N, tid = threadIdx.x;
+  const int i = it.a + tid;
+  const bool ion = i < N;
+  const bool rec = i < B.npad;  // threads that store structure-factor vectors
+  double sx = 0.0, sy = 0.0, sz = 0.0;
+  float zq = 0.f;
+  if (ion) {
+    sx = B.rx[i] * B.invL[0];
+    sy = B.ry[i] * B.invL[1];
+    sz = B.rz[i] * B.invL[2];
+    zq = (float)B.Z[B.typ[i]];
+    float2* X = reinterpret_cast<float2*>(B.tabx);
+    float2* Y = reinterpret_cast<float2*>(B.taby);
+    for (int h = 0; h <= B.hmax; ++h) {
+      double sn, cs;
+      sincospi(2.0 * (double)h * sx, &sn, &cs);
+      X[(size_t)h * N + i] = make_float2((float)cs, (float)sn);
+      sincospi(2.0 * (double)h * sy, &sn, &cs);
+      Y[(size_t)h * N + i] = make_float2((float)cs, (float)sn);
+    }
+  }
   u4_t* wb = reinterpret_cast<u4_t*>(B.wb);
-  for (int q4 = 0; q4 < 2 * B.kbk; ++q4) {
-    _Float16 hv[8], lv[8];
+  u4_t* sb = reinterpret_cast<u4_t*>(B.sb);
+  const int HP = 16 * B.sknt, HB = 8 * B.kbk;
+  const int q = tid >> 2, e2 = tid & 3;  // the group and vector this thread stores
+  for (int q4 = 0; q4 < max(HB, HP) / 4; ++q4) {
+    float c[4], s[4];
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
       const int hz = 4 * q4 + e;
-      float c = 0.f, s = 0.f;
-      if (hz <= B.hmax) {
+      c[e] = s[e] = 0.f;
+      if (ion && hz <= B.hmax) {
         double sn, cs;
         sincospi(2.0 * (double)hz * sz, &sn, &cs);
-        c = (float)cs;
-        s = (float)sn;
+        c[e] = (float)cs;
+        s[e] = (float)sn;
       }
-      split_f16(c, hv[2 * e], lv[2 * e]);
-      split_f16(s, hv[2 * e + 1], lv[2 * e + 1]);
     }
-    u4_t* dst = wb + ((size_t)q4 * N + i) * 2;
-    dst[0] = pack_h8(hv);
-    dst[1] = pack_h8(lv);
+    if (ion && 4 * q4 < HB) {
+      _Float16 hv[8], lv[8];
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        split_f16(c[e], hv[2 * e], lv[2 * e]);
+        split_f16(s[e], hv[2 * e + 1], lv[2 * e + 1]);
+      }
+      u4_t* dst = wb + ((size_t)q4 * N + i) * 2;
+      dst[0] = pack_h8(hv);
+      dst[1] = pack_h8(lv);
+    }
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int hz = 4 * q4 + e;
+      if (hz >= HP) break;  // uniform
+      const float wr = zq * c[e], wi = zq * s[e];
+      _Float16 a, al, b, bl, d, dl, f, fl;
+      split_f16(wr, a, al);
+      split_f16(-wi, b, bl);
+      split_f16(wi, d, dl);
+      split_f16(wr, f, fl);
+      const int buf = hz & 1;
+      sbx[buf][0][tid] = pack2_f16(a, b);    // Re S: (Z W_r, -Z W_i)
+      sbx[buf][1][tid] = pack2_f16(al, bl);
+      sbx[buf][2][tid] = pack2_f16(d, f);    // Im S: (Z W_i, Z W_r)
+      sbx[buf][3][tid] = pack2_f16(dl, fl);
+      __syncthreads();  // also orders the reads of hz - 1 before the next writes of hz + 1
+      if (rec) {
+        const u4_t v = {sbx[buf][e2][4 * q], sbx[buf][e2][4 * q + 1], sbx[buf][e2][4 * q + 2],
+                        sbx[buf][e2][4 * q + 3]};
+        sb[((size_t)(it.a / 4 + q) * HP + hz) * 4 + e2] = v;
+      }
+    }
   }
 }
 
@@ -332,42 +396,9 @@ namespace mdg {
 //   B[(ion, p)][(hz, out)] from Z W[hz]: Re S takes (ZWr, -ZWi), Im S
 //   takes (ZWi, ZWr).
 // Q is formed per lane (its column, 4 ions per K step) and split into FP16
-// hi/lo; the B records are precomputed.  Three MFMAs per K step and hz
+// hi/lo; the B records come from k_mixed_tables.  Three MFMAs per K step and hz
 // tile (hi*hi + hi*lo + lo*hi), FP32 accumulation over 64 ions, then FP64.
 // ======================================================================
-
-// B records, one thread per (ion group of 4, hz): both outputs, hi and lo.
-__global__ __launch_bounds__(kIonThreads) void k_sb_tables(const BoxDesc* __restrict__ boxes,
-                                                           const Item* __restrict__ items) {
-  const Item it = items[blockIdx.x];
-  const BoxDesc& B = boxes[it.box];
-  const int N = B.N, HP = 16 * B.sknt;
-  const int idx = it.a + threadIdx.x;
-  const int grp = idx / HP, hz = idx - grp * HP;
-  if (grp >= B.npad / 4) return;
-  const float2* W = reinterpret_cast<const float2*>(B.tabw);
-  _Float16 h0[8], l0[8], h1[8], l1[8];
-#pragma unroll
-  for (int e = 0; e < 4; ++e) {
-    const int i = 4 * grp + e;
-    float wr = 0.f, wi = 0.f;
-    if (i < N && hz <= B.hmax) {
-      const float2 w = W[(size_t)hz * N + i];
-      const float z = (float)B.Z[B.typ[i]];
-      wr = z * w.x;
-      wi = z * w.y;
-    }
-    split_f16(wr, h0[2 * e], l0[2 * e]);       // Re S: (ZWr, -ZWi)
-    split_f16(-wi, h0[2 * e + 1], l0[2 * e + 1]);
-    split_f16(wi, h1[2 * e], l1[2 * e]);       // Im S: (ZWi, ZWr)
-    split_f16(wr, h1[2 * e + 1], l1[2 * e + 1]);
-  }
-  u4_t* dst = reinterpret_cast<u4_t*>(B.sb) + ((size_t)grp * HP + hz) * 4;
-  dst[0] = pack_h8(h0);
-  dst[1] = pack_h8(l0);
-  dst[2] = pack_h8(h1);
-  dst[3] = pack_h8(l1);
-}
 
 // One workgroup: 4 waves x 32 columns, one ion chunk, two hz tiles of 16
 // (further hz tiles are further items).  Ion tiles of 32 are staged in LDS

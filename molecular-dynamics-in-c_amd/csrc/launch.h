@@ -30,14 +30,13 @@ void launch_real32(bool analytic, const BoxDesc* boxes, const Item* items, int n
                    hipStream_t s);
 // MFMA back-projection: FP16 hi/lo W operands, packed weight tiles, then
 // the tiles against 32-ion blocks; kb = K blocks (8 hz) of the widest box
-void launch_wb_tables(const BoxDesc* boxes, const Item* items, int n, hipStream_t s);
+void launch_mixed_tables(const BoxDesc* boxes, const Item* items, int n, hipStream_t s);
 void launch_kf_pack(const BoxDesc* boxes, const Item* items, int n, hipStream_t s);
 void launch_kforce_mfma(int kb, const BoxDesc* boxes, const Item* items, int n, hipStream_t s);
 // ions per back-projection workgroup for kb K blocks
 int kforce_mfma_ions(int kb);
 // MFMA structure factor: Z W records, then the partial sums; hl = hmax + 1
 // of the widest box (sizes the LDS tiles)
-void launch_sb_tables(const BoxDesc* boxes, const Item* items, int n, hipStream_t s);
 void launch_sk_mfma(int hl, const BoxDesc* boxes, const Item* items, int n, hipStream_t s);
 
 // register row lengths the back-projection is instantiated for

@@ -84,10 +84,9 @@ int time_candidate(const BoxSpec& base, const mdg_box_params& prm, int precision
       for (size_t q = 0; q < names.size(); ++q) {
         const std::string& n = names[q];
         if (n == "cell_build" || n == "real_space") out->real_ms += ms[q];
-        else if (n == "phase_tables" || n == "sk_partial" || n == "sk_finalize" ||
-                 n == "sb_tables")
+        else if (n == "phase_tables" || n == "sk_partial" || n == "sk_finalize")
           out->sk_ms += ms[q];
-        else if (n == "kforce" || n == "kf_pack" || n == "wb_tables") out->kf_ms += ms[q];
+        else if (n == "kforce" || n == "kf_pack") out->kf_ms += ms[q];
       }
       // k_force_vv reads every split's and group's partial forces: charge
       // those bytes (at 5 TB/s) to the families that create them

@@ -75,6 +75,33 @@ def test_batch_members_match_lone_boxes(gpu):
         mix.close()
 
 
+def test_concurrent_box_groups_match_lone_boxes(gpu):
+    """With 16 or more boxes a step runs two box groups and each box's real
+    space and reciprocal chain on separate streams.  Kernels running side by
+    side leave each other's data in LDS; every box must still step to the
+    same (finite) bits as alone.  Regression: the mixed real space's idle
+    lanes once read a partner slot that the window had not staged."""
+    specs = [(2, "NaCl", H.A_NACL), (3, "KCl", H.A_KCL), (4, "NaCl", H.A_NACL),
+             ((2, 3, 4), "KCl", H.A_KCL), (5, "NaCl", H.A_NACL)] * 4
+    ps = [_params(n, a, salt=s) for n, s, a in specs]
+    sts = [_state(p, 500 + k) for k, p in enumerate(ps)]
+    mix = mdgpu.Batch(ps, mdgpu.MIXED)
+    for k, (r, v, sp) in enumerate(sts):
+        mix.set_state(k, r=r, v=v, species=sp)
+    mix.step(200)
+    for k in range(0, len(ps), 3):
+        lone = mdgpu.Batch([ps[k]], mdgpu.MIXED)
+        r, v, sp = sts[k]
+        lone.set_state(0, r=r, v=v, species=sp)
+        lone.step(200)
+        a_, b_ = lone.state(0), mix.state(k)
+        assert np.isfinite(b_["r"]).all() and np.isfinite(b_["f"]).all(), k
+        for key in ("r", "v", "f"):
+            assert np.array_equal(a_[key], b_[key]), (k, key)
+        lone.close()
+    mix.close()
+
+
 def test_graph_replay_equals_eager_steps(gpu):
     p = _params(4, H.A_NACL)
     r, v, sp = _state(p, 7)
