@@ -959,23 +959,25 @@ void BoxSet::enqueue_force_terms(hipStream_t s, int terms) {
 // partials), and different boxes share nothing.  So they run on several
 // streams between a fork event and join events: the kernels are
 // latency-bound and share the chip better than they run back to back.
-// Group 0's reciprocal chain runs on the caller's stream and its real
-// space on side stream 0; group g > 0 runs its real space and then its
-// reciprocal chain on side stream g, so that different kinds of kernel
-// overlap (measured: 1.29 ms per step on one stream pair, 1.11-1.15 ms
-// staggered over two groups, slower with three).  Captured into the step
-// graph, the fork and the joins become graph edges.
+// The real space of every box runs on side stream 0, group 0's reciprocal
+// chain on the caller's stream from the start, and group 1's reciprocal
+// chain on side stream 1 once the real space is done, so that different
+// kinds of kernel overlap.  Measured on the 256-box batch with one tuning
+// table: 1.22 ms per step with each group's real space beside or before
+// its own chain, 1.135 ms with this order, and worse with any split other
+// than an equal one.  Captured into the step graph, the fork and the joins
+// become graph edges.
 void BoxSet::enqueue_step(hipStream_t s, bool vv) {
   const int G = (int)grp_.size();
   if (all_.n[LSk] > 0 && ensure_side_streams(G)) {
     (void)hipEventRecord(ev_fork_, s);
     for (int k = 0; k < G; ++k) (void)hipStreamWaitEvent(side_[k], ev_fork_, 0);
-    for (int g = 0; g < G; ++g) {
-      if (g == 0) {
-        launch_cells_real(side_[0], 0, grp_[0]);
-        launch_recip(s, grp_[0]);
-      } else {  // later groups: real space first, then their reciprocal chain
-        launch_cells_real(side_[g], 0, grp_[g]);
+    launch_cells_real(side_[0], 0, all_);
+    launch_recip(s, grp_[0]);
+    if (G > 1) {  // the other groups' chains start when the real space is done
+      (void)hipEventRecord(ev_join_[0], side_[0]);
+      for (int g = 1; g < G; ++g) {
+        (void)hipStreamWaitEvent(side_[g], ev_join_[0], 0);
         launch_recip(side_[g], grp_[g]);
       }
     }
