@@ -408,6 +408,8 @@ namespace mdg {
 constexpr int kSkmTile = 32;
 constexpr int kSkmRow = kSkmTile + 2;  // complex per LDS row
 constexpr int kSkmThreads = kKfMfmaWaves * kWave;
+// X/Y staging loads per thread in flight at once
+constexpr int kSkmXYBatch = 4;
 
 // LDS bytes of k_sk_mfma for a launch whose largest box has H = hmax + 1
 __host__ __device__ inline int sk_mfma_lds_bytes(int H) {
@@ -429,11 +431,14 @@ __global__ __launch_bounds__(kSkmThreads) void k_sk_mfma(const BoxDesc* __restri
   lds_u4* xs = (lds_u4*)smem4;         // [hl][kRow4]
   lds_u4* ys = xs + hl * kRow4;        // [hl][kRow4]
   lds_u4* ss = ys + hl * kRow4;        // [8 groups][32 hz][4]
-  const int ex = H * (kSkmTile / 2);   // 16-byte elements per table and tile
+  typedef __attribute__((address_space(3))) f2v_t lds_f2w;
+  lds_f2w* xsf = (lds_f2w*)xs;         // the same rows as complex elements
+  lds_f2w* ysf = (lds_f2w*)ys;
+  const int exf = H * kSkmTile;        // complex elements per table and tile
   constexpr int es = 8 * 32 * 4;
-  const u4_t* gx = reinterpret_cast<const u4_t*>(B.tabx);
-  const u4_t* gy = reinterpret_cast<const u4_t*>(B.taby);
-  const u4_t* gs = reinterpret_cast<const u4_t*>(B.sb);
+  const gbytes gxb = (gbytes)B.tabx;
+  const gbytes gyb = (gbytes)B.taby;
+  const gbytes gsb = (gbytes)B.sb;
   const int jb = it.a * B.chunk, je = min(N, jb + B.chunk);
   const int ngrp = B.npad / 4;
   // this lane's column
@@ -460,23 +465,41 @@ __global__ __launch_bounds__(kSkmThreads) void k_sk_mfma(const BoxDesc* __restri
 #pragma unroll
     for (int v = 0; v < 16; ++v) acc[t][v] = 0.f;
   for (int j0 = jb; j0 < je; j0 += kSkmTile) {
+    // Staging, with the global loads issued in batches so that several are
+    // in flight at once: the records of the tile's 8 ion groups for both hz
+    // tiles (zero past the box's hz), then X and Y of one ion (h < H) per
+    // element, four elements per thread at a time.
     __syncthreads();  // the previous tile's reads are done
-    // X and Y rows of the tile
-    for (int e = tid; e < 2 * ex; e += kSkmThreads) {
-      const int tb = e >= ex, q = e - tb * ex, h = q >> 4, p = q & 15;
-      const int i = j0 + 2 * p;
-      u4_t v = {0u, 0u, 0u, 0u};
-      if (i + 2 <= N) v = (tb ? gy : gx)[((size_t)h * N + i) >> 1];
-      if (tb) ys[h * kRow4 + p] = v;
-      else xs[h * kRow4 + p] = v;
-    }
-    // records of the tile's 8 ion groups, both hz tiles (zero past the box's hz)
-    for (int q = tid; q < es; q += kSkmThreads) {
+    u4_t vss[es / kSkmThreads];
+#pragma unroll
+    for (int m = 0; m < es / kSkmThreads; ++m) {
+      const int q = tid + m * kSkmThreads;
       const int grp = q >> 7, rem = q & 127, hz = 16 * t0 + (rem >> 2);
       const int gg = (j0 >> 2) + grp;
-      u4_t v = {0u, 0u, 0u, 0u};
-      if (gg < ngrp && hz < HP) v = gs[((size_t)gg * HP + 16 * t0) * 4 + rem];
-      ss[q] = v;
+      vss[m] = u4_t{0u, 0u, 0u, 0u};
+      if (gg < ngrp && hz < HP)
+        vss[m] = gload<u4_t>(gsb, 16u * (unsigned)(((size_t)gg * HP + 16 * t0) * 4 + rem));
+    }
+    for (int e0 = 0; e0 < 2 * exf; e0 += kSkmXYBatch * kSkmThreads) {
+      f2v_t v[kSkmXYBatch];
+#pragma unroll
+      for (int m = 0; m < kSkmXYBatch; ++m) {
+        const int e = e0 + tid + m * kSkmThreads;
+        const int tb = e >= exf, q = e - tb * exf, i = j0 + q % kSkmTile;
+        v[m] = f2v_t{0.f, 0.f};
+        if (e < 2 * exf && i < N)
+          v[m] = gload<f2v_t>(tb ? gyb : gxb, 8u * (unsigned)((size_t)(q / kSkmTile) * N + i));
+      }
+      if (e0 == 0) {
+#pragma unroll
+        for (int m = 0; m < es / kSkmThreads; ++m) ss[tid + m * kSkmThreads] = vss[m];
+      }
+#pragma unroll
+      for (int m = 0; m < kSkmXYBatch; ++m) {
+        const int e = e0 + tid + m * kSkmThreads;
+        const int tb = e >= exf, q = e - tb * exf;
+        if (e < 2 * exf) (tb ? ysf : xsf)[(q / kSkmTile) * kSkmRow + q % kSkmTile] = v[m];
+      }
     }
     __syncthreads();
 #pragma unroll
