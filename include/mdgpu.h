@@ -132,11 +132,15 @@ int  mdg_batch_force_terms(mdg_batch *b, int terms);
 int  mdg_batch_step(mdg_batch *b, int nsteps);
 int  mdg_batch_scalars(mdg_batch *b, int box, mdg_scalars *out);
 int  mdg_batch_sync(mdg_batch *b);
-/* Kernel timing over the next `nsteps` steps, measured with hipEvents on
- * the batch's stream (the graph is bypassed while timing).  names/ms_avg:
- * up to `cap` entries, returns the count in *n. */
+/* Kernel timing over the next `nsteps` steps, launched eagerly in the
+ * step's own schedule (box groups on several streams); every launch
+ * carries a start and a stop hipEvent stamped by its own dispatch
+ * (hipExtLaunchKernel).  Per kernel name:
+ * ms_step = ms per step summed over its launches, launches = launches per
+ * step (ms_step / launches is its average launch duration).  Up to `cap`
+ * entries; the count is returned in *n. */
 int  mdg_batch_profile(mdg_batch *b, int nsteps, int cap, char names[][32],
-                       double *ms_avg, int *n);
+                       double *ms_step, int *launches, int *n);
 /* Algorithmic work of one step of box `box`: accepted real-space pairs
  * (unordered), k vectors, and ions; pairs are counted on the device at the
  * last force evaluation. */

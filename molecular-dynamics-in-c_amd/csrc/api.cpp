@@ -284,18 +284,20 @@ int mdg_batch_scalars(mdg_batch* b, int box, mdg_scalars* out) {
   return err_code(e);
 }
 
-int mdg_batch_profile(mdg_batch* b, int nsteps, int cap, char names[][32], double* ms_avg,
-                      int* n) {
-  if (!b || nsteps <= 0 || !names || !ms_avg || !n) return MDG_E_ARG;
+int mdg_batch_profile(mdg_batch* b, int nsteps, int cap, char names[][32], double* ms_step,
+                      int* launches, int* n) {
+  if (!b || nsteps <= 0 || !names || !ms_step || !launches || !n) return MDG_E_ARG;
   std::vector<std::string> nm;
   std::vector<double> t;
-  int rc = b->set->profile(nsteps, b->stream, &nm, &t);
+  std::vector<int> c;
+  int rc = b->set->profile(nsteps, b->stream, true, &nm, &t, &c);
   if (rc != MDG_OK) return rc;
   int k = 0;
   for (; k < (int)nm.size() && k < cap; ++k) {
     strncpy(names[k], nm[k].c_str(), 31);
     names[k][31] = 0;
-    ms_avg[k] = t[k];
+    ms_step[k] = t[k];
+    launches[k] = c[k];
   }
   *n = k;
   return MDG_OK;

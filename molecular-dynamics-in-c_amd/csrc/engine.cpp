@@ -912,13 +912,55 @@ int BoxSet::sync_desc(hipStream_t s) {
 }
 
 // cell lists (cell-list boxes) and real-space forces of the boxes in r
+// Kernels as profile() names them.  Mixed path: "phase_tables" is
+// k_mixed_tables (the phase tables and both passes' MFMA B operands).
+enum {
+  KCell, KTables, KReal, KSk, KFin, KKf, KVv, KScal, KPack, kKernelNames
+};
+static const char* const kKernelName[kKernelNames] = {
+    "cell_build", "phase_tables", "real_space", "sk_partial", "sk_finalize",
+    "kforce",     "force_vv",     "box_scalars", "kf_pack"};
+
+thread_local LaunchEvents g_launch_events;
+
+// Under profile(), the launch carries a start and a stop event that the
+// kernel's own dispatch stamps (the same timestamps a kernel trace reads).
+template <class F>
+void BoxSet::timed(int kernel, hipStream_t, F&& launch) {
+  if (!timer_) {
+    launch();
+    return;
+  }
+  hipEvent_t a = nullptr, b = nullptr;
+  if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) {
+    if (a) (void)hipEventDestroy(a);
+    launch();
+    return;
+  }
+  g_launch_events = LaunchEvents{a, b, false};
+  launch();
+  const bool used = g_launch_events.used;
+  g_launch_events = LaunchEvents{};
+  if (!used) {  // nothing to launch (an empty work list)
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    return;
+  }
+  timer_->ev.push_back(a);
+  timer_->ev.push_back(b);
+  timer_->kernel.push_back(kernel);
+}
+
 void BoxSet::launch_cells_real(hipStream_t s, int mode, const Ranges& r) {
   const bool analytic = real_mode_ == MDG_ANALYTIC;
-  launch_cell_build(d_desc_, d_ids_ + r.ioff[ICell], r.in[ICell], s);
-  if (precision_ == MDG_MIXED)
-    launch_real32(analytic, d_desc_, d_items_ + r.off[LReal], r.n[LReal], mode, s);
-  else
-    launch_real64(analytic, d_desc_, d_items_ + r.off[LReal], r.n[LReal], mode, s);
+  if (r.in[ICell])
+    timed(KCell, s, [&] { launch_cell_build(d_desc_, d_ids_ + r.ioff[ICell], r.in[ICell], s); });
+  timed(KReal, s, [&] {
+    if (precision_ == MDG_MIXED)
+      launch_real32(analytic, d_desc_, d_items_ + r.off[LReal], r.n[LReal], mode, s);
+    else
+      launch_real64(analytic, d_desc_, d_items_ + r.off[LReal], r.n[LReal], mode, s);
+  });
 }
 
 // phase tables, partial structure factors, finalisation, back-projection
@@ -926,16 +968,18 @@ void BoxSet::launch_recip(hipStream_t s, const Ranges& r) {
   const bool mixed = precision_ == MDG_MIXED;
   const Item* it = d_items_;
   if (mixed) {
-    launch_mixed_tables(d_desc_, it + r.off[LTab], r.n[LTab], s);
-    launch_sk_mfma(skn_, d_desc_, it + r.off[LSk], r.n[LSk], s);
-    launch_sk_finalize(d_desc_, it + r.off[LFin], r.n[LFin], s);
-    launch_kf_pack(d_desc_, it + r.off[LPack], r.n[LPack], s);
-    launch_kforce_mfma(hz_ / 8, d_desc_, it + r.off[LKf], r.n[LKf], s);
+    timed(KTables, s, [&] { launch_mixed_tables(d_desc_, it + r.off[LTab], r.n[LTab], s); });
+    timed(KSk, s, [&] { launch_sk_mfma(skn_, d_desc_, it + r.off[LSk], r.n[LSk], s); });
+    timed(KFin, s, [&] { launch_sk_finalize(d_desc_, it + r.off[LFin], r.n[LFin], s); });
+    timed(KPack, s, [&] { launch_kf_pack(d_desc_, it + r.off[LPack], r.n[LPack], s); });
+    timed(KKf, s,
+          [&] { launch_kforce_mfma(hz_ / 8, d_desc_, it + r.off[LKf], r.n[LKf], s); });
   } else {
-    launch_phase_tables(false, d_desc_, it + r.off[LTab], r.n[LTab], s);
-    launch_sk64(d_desc_, it + r.off[LSk], r.n[LSk], hs_, hsw_, s);
-    launch_sk_finalize(d_desc_, it + r.off[LFin], r.n[LFin], s);
-    launch_kforce64(hz_, d_desc_, it + r.off[LKf], r.n[LKf], s);
+    timed(KTables, s,
+          [&] { launch_phase_tables(false, d_desc_, it + r.off[LTab], r.n[LTab], s); });
+    timed(KSk, s, [&] { launch_sk64(d_desc_, it + r.off[LSk], r.n[LSk], hs_, hsw_, s); });
+    timed(KFin, s, [&] { launch_sk_finalize(d_desc_, it + r.off[LFin], r.n[LFin], s); });
+    timed(KKf, s, [&] { launch_kforce64(hz_, d_desc_, it + r.off[LKf], r.n[LKf], s); });
   }
 }
 
@@ -969,7 +1013,7 @@ void BoxSet::enqueue_force_terms(hipStream_t s, int terms) {
 // become graph edges.
 void BoxSet::enqueue_step(hipStream_t s, bool vv) {
   const int G = (int)grp_.size();
-  if (all_.n[LSk] > 0 && ensure_side_streams(G)) {
+  if (!serial_ && all_.n[LSk] > 0 && ensure_side_streams(G)) {
     (void)hipEventRecord(ev_fork_, s);
     for (int k = 0; k < G; ++k) (void)hipStreamWaitEvent(side_[k], ev_fork_, 0);
     launch_cells_real(side_[0], 0, all_);
@@ -989,8 +1033,11 @@ void BoxSet::enqueue_step(hipStream_t s, bool vv) {
     launch_cells_real(s, 0, all_);
     if (all_.n[LSk] > 0) launch_recip(s, all_);
   }
-  launch_force_vv(vv, d_desc_, d_items_ + all_.off[LIon], all_.n[LIon], s);
-  launch_box_scalars(d_desc_, d_ids_ + all_.ioff[IAll], all_.in[IAll], vv ? 1 : 0, s);
+  timed(KVv, s,
+        [&] { launch_force_vv(vv, d_desc_, d_items_ + all_.off[LIon], all_.n[LIon], s); });
+  timed(KScal, s, [&] {
+    launch_box_scalars(d_desc_, d_ids_ + all_.ioff[IAll], all_.in[IAll], vv ? 1 : 0, s);
+  });
 }
 
 void BoxSet::enqueue_scalars(hipStream_t s, bool count_step) {
@@ -1040,72 +1087,37 @@ int BoxSet::step(int nsteps, hipStream_t s) {
   return MDG_OK;
 }
 
-int BoxSet::profile(int nsteps, hipStream_t s, std::vector<std::string>* names,
-                    std::vector<double>* ms) {
+int BoxSet::profile(int nsteps, hipStream_t s, bool as_step, std::vector<std::string>* names,
+                    std::vector<double>* ms, std::vector<int>* launches) {
+  if (nsteps <= 0) return MDG_E_ARG;
   int rc = sync_desc(s);
   if (rc != MDG_OK) return rc;
-  const bool mixed = precision_ == MDG_MIXED;
-  // mixed path: "phase_tables" is k_mixed_tables (the phase tables and
-  // both passes' MFMA B operands)
-  constexpr int K = 9;
-  const char* kn[K] = {"cell_build", "phase_tables", "real_space", "sk_partial",
-                       "sk_finalize", "kforce",      "force_vv",   "box_scalars",
-                       "kf_pack"};
-  std::vector<hipEvent_t> ev(2 * K * std::max(1, nsteps));
-  for (auto& e : ev) MDG_HIP(hipEventCreate(&e));
-  std::vector<int> used(K, 0), which;
-  int e = 0;
-  auto rec = [&](int k, auto&& fn) {
-    (void)hipEventRecord(ev[e], s);
-    fn();
-    (void)hipEventRecord(ev[e + 1], s);
-    used[k] = 1;
-    which.push_back(k);
-    e += 2;
-  };
-  const Ranges& r = all_;
-  const Item* it = d_items_;
-  for (int st = 0; st < nsteps; ++st) {
-    if (r.in[ICell])
-      rec(0, [&] { launch_cell_build(d_desc_, d_ids_ + r.ioff[ICell], r.in[ICell], s); });
-    rec(2, [&] {
-      const bool analytic = real_mode_ == MDG_ANALYTIC;
-      if (mixed) launch_real32(analytic, d_desc_, it + r.off[LReal], r.n[LReal], 0, s);
-      else launch_real64(analytic, d_desc_, it + r.off[LReal], r.n[LReal], 0, s);
-    });
-    if (r.n[LSk]) {
-      rec(1, [&] {
-        if (mixed) launch_mixed_tables(d_desc_, it + r.off[LTab], r.n[LTab], s);
-        else launch_phase_tables(false, d_desc_, it + r.off[LTab], r.n[LTab], s);
-      });
-      rec(3, [&] {
-        if (mixed) launch_sk_mfma(skn_, d_desc_, it + r.off[LSk], r.n[LSk], s);
-        else launch_sk64(d_desc_, it + r.off[LSk], r.n[LSk], hs_, hsw_, s);
-      });
-      rec(4, [&] { launch_sk_finalize(d_desc_, it + r.off[LFin], r.n[LFin], s); });
-      if (mixed) rec(8, [&] { launch_kf_pack(d_desc_, it + r.off[LPack], r.n[LPack], s); });
-      rec(5, [&] {
-        if (mixed) launch_kforce_mfma(hz_ / 8, d_desc_, it + r.off[LKf], r.n[LKf], s);
-        else launch_kforce64(hz_, d_desc_, it + r.off[LKf], r.n[LKf], s);
-      });
-    }
-    rec(6, [&] { launch_force_vv(true, d_desc_, it + r.off[LIon], r.n[LIon], s); });
-    rec(7, [&] { launch_box_scalars(d_desc_, d_ids_ + r.ioff[IAll], r.in[IAll], 1, s); });
+  LaunchTimer t;
+  timer_ = &t;
+  serial_ = !as_step;
+  for (int st = 0; st < nsteps; ++st) enqueue_step(s, true);
+  timer_ = nullptr;
+  serial_ = false;
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  std::vector<double> tot(kKernelNames, 0.0);
+  std::vector<int> cnt(kKernelNames, 0);
+  for (size_t q = 0; q < t.kernel.size() && e == hipSuccess; ++q) {
+    float x = 0.f;
+    e = hipEventElapsedTime(&x, t.ev[2 * q], t.ev[2 * q + 1]);
+    tot[t.kernel[q]] += x;
+    cnt[t.kernel[q]] += 1;
   }
-  MDG_HIP(hipStreamSynchronize(s));
-  std::vector<double> tot(K, 0.0);
-  for (size_t q = 0; q < which.size(); ++q) {
-    float t = 0.f;
-    MDG_HIP(hipEventElapsedTime(&t, ev[2 * q], ev[2 * q + 1]));
-    tot[which[q]] += t;
-  }
-  for (auto& x : ev) (void)hipEventDestroy(x);
+  for (auto& x : t.ev) (void)hipEventDestroy(x);
+  MDG_HIP(e);
   names->clear();
   ms->clear();
-  for (int k = 0; k < K; ++k)
-    if (used[k]) {
-      names->push_back(kn[k]);
+  launches->clear();
+  for (int k = 0; k < kKernelNames; ++k)
+    if (cnt[k]) {
+      names->push_back(kKernelName[k]);
       ms->push_back(tot[k] / nsteps);
+      launches->push_back(cnt[k] / nsteps);
     }
   return MDG_OK;
 }

@@ -133,8 +133,13 @@ class BoxSet {
   void enqueue_next_r(hipStream_t s);
   void enqueue_moment_correction(hipStream_t s);
   int step(int nsteps, hipStream_t s);          // graph replay
-  int profile(int nsteps, hipStream_t s, std::vector<std::string>* names,
-              std::vector<double>* ms);
+  // Kernel timing over `nsteps` eager steps: every launch carries a start
+  // and a stop hipEvent that its own dispatch stamps.  as_step: the step's own schedule
+  // (box groups, side streams, kernels overlapping); else every kernel once
+  // over all boxes, back to back.  ms: per kernel, ms per step summed over
+  // its launches; launches: its launches per step.
+  int profile(int nsteps, hipStream_t s, bool as_step, std::vector<std::string>* names,
+              std::vector<double>* ms, std::vector<int>* launches);
   int sync_desc(hipStream_t s);                 // upload dirty descriptors
   void invalidate_graph();
 
@@ -165,6 +170,15 @@ class BoxSet {
   std::vector<hipEvent_t> ev_join_;
   hipEvent_t ev_fork_ = nullptr;
   bool side_failed_ = false;
+  bool serial_ = false;     // profile(): every kernel once, back to back
+  // per-launch event timing while profile() runs; null otherwise
+  struct LaunchTimer {
+    std::vector<hipEvent_t> ev;
+    std::vector<int> kernel;
+  };
+  LaunchTimer* timer_ = nullptr;
+  template <class F>
+  void timed(int kernel, hipStream_t s, F&& launch);
   bool ensure_side_streams(int count);
   void launch_cells_real(hipStream_t s, int mode, const Ranges& r);
   void launch_recip(hipStream_t s, const Ranges& r);

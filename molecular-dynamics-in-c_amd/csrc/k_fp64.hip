@@ -19,23 +19,23 @@ int kforce_hz_for(int hmax) {
 }
 
 void launch_cell_build(const BoxDesc* boxes, const int* ids, int n, hipStream_t s) {
-  if (n > 0) hipLaunchKernelGGL(k_cell_build, MDG_GRID(n, kCellThreads), boxes, ids);
+  if (n > 0) MDG_LAUNCH(k_cell_build, MDG_GRID(n, kCellThreads), boxes, ids);
 }
 
 void launch_phase_tables(bool fp32, const BoxDesc* boxes, const Item* items, int n,
                          hipStream_t s) {
   if (n <= 0) return;
-  if (fp32) hipLaunchKernelGGL(k_phase_tables<float2>, MDG_GRID(n, kIonThreads), boxes, items);
-  else hipLaunchKernelGGL(k_phase_tables<double2>, MDG_GRID(n, kIonThreads), boxes, items);
+  if (fp32) MDG_LAUNCH(k_phase_tables<float2>, MDG_GRID(n, kIonThreads), boxes, items);
+  else MDG_LAUNCH(k_phase_tables<double2>, MDG_GRID(n, kIonThreads), boxes, items);
 }
 
 void launch_real64(bool analytic, const BoxDesc* boxes, const Item* items, int n, int mode,
                    hipStream_t s) {
   if (n <= 0) return;
   if (analytic)
-    hipLaunchKernelGGL(k_real<kRealAnalytic64>, MDG_GRID(n, kRealThreads), boxes, items, mode);
+    MDG_LAUNCH(k_real<kRealAnalytic64>, MDG_GRID(n, kRealThreads), boxes, items, mode);
   else
-    hipLaunchKernelGGL(k_real<kRealTable64>, MDG_GRID(n, kRealThreads), boxes, items, mode);
+    MDG_LAUNCH(k_real<kRealTable64>, MDG_GRID(n, kRealThreads), boxes, items, mode);
 }
 
 void launch_sk64(const BoxDesc* boxes, const Item* items, int n, int hs, int hsw, hipStream_t s) {
@@ -47,50 +47,50 @@ void launch_sk64(const BoxDesc* boxes, const Item* items, int n, int hs, int hsw
                               hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
     raised = true;
   }
-  hipLaunchKernelGGL(k_sk_seg<double>, MDG_GRID_LDS(n, kSkThreads, lds), boxes, items, hs, hsw);
+  MDG_LAUNCH(k_sk_seg<double>, MDG_GRID_LDS(n, kSkThreads, lds), boxes, items, hs, hsw);
 }
 
 void launch_kforce64(int hz, const BoxDesc* boxes, const Item* items, int n, hipStream_t s) {
   if (n <= 0) return;
   switch (hz) {
-    case 8: hipLaunchKernelGGL((k_kforce<double, 8>), MDG_GRID(n, kKfThreads), boxes, items); break;
-    case 16: hipLaunchKernelGGL((k_kforce<double, 16>), MDG_GRID(n, kKfThreads), boxes, items); break;
-    case 24: hipLaunchKernelGGL((k_kforce<double, 24>), MDG_GRID(n, kKfThreads), boxes, items); break;
-    case 32: hipLaunchKernelGGL((k_kforce<double, 32>), MDG_GRID(n, kKfThreads), boxes, items); break;
-    case 48: hipLaunchKernelGGL((k_kforce<double, 48>), MDG_GRID(n, kKfThreads), boxes, items); break;
-    default: hipLaunchKernelGGL((k_kforce<double, 64>), MDG_GRID(n, kKfThreads), boxes, items); break;
+    case 8: MDG_LAUNCH((k_kforce<double, 8>), MDG_GRID(n, kKfThreads), boxes, items); break;
+    case 16: MDG_LAUNCH((k_kforce<double, 16>), MDG_GRID(n, kKfThreads), boxes, items); break;
+    case 24: MDG_LAUNCH((k_kforce<double, 24>), MDG_GRID(n, kKfThreads), boxes, items); break;
+    case 32: MDG_LAUNCH((k_kforce<double, 32>), MDG_GRID(n, kKfThreads), boxes, items); break;
+    case 48: MDG_LAUNCH((k_kforce<double, 48>), MDG_GRID(n, kKfThreads), boxes, items); break;
+    default: MDG_LAUNCH((k_kforce<double, 64>), MDG_GRID(n, kKfThreads), boxes, items); break;
   }
 }
 
 void launch_sk_finalize(const BoxDesc* boxes, const Item* items, int n, hipStream_t s) {
-  if (n > 0) hipLaunchKernelGGL(k_sk_finalize, MDG_GRID(n, kIonThreads), boxes, items);
+  if (n > 0) MDG_LAUNCH(k_sk_finalize, MDG_GRID(n, kIonThreads), boxes, items);
 }
 
 void launch_force_vv(bool vv, const BoxDesc* boxes, const Item* items, int n, hipStream_t s,
                      int terms) {
   if (n <= 0) return;
-  if (vv) hipLaunchKernelGGL(k_force_vv<1>, MDG_GRID(n, kIonThreads), boxes, items, terms);
-  else hipLaunchKernelGGL(k_force_vv<0>, MDG_GRID(n, kIonThreads), boxes, items, terms);
+  if (vv) MDG_LAUNCH(k_force_vv<1>, MDG_GRID(n, kIonThreads), boxes, items, terms);
+  else MDG_LAUNCH(k_force_vv<0>, MDG_GRID(n, kIonThreads), boxes, items, terms);
 }
 
 void launch_clear_foc(const BoxDesc* boxes, const Item* items, int n, hipStream_t s) {
-  if (n > 0) hipLaunchKernelGGL(k_clear_foc, MDG_GRID(n, kIonThreads), boxes, items);
+  if (n > 0) MDG_LAUNCH(k_clear_foc, MDG_GRID(n, kIonThreads), boxes, items);
 }
 
 void launch_next_v(const BoxDesc* boxes, const Item* items, int n, hipStream_t s) {
-  if (n > 0) hipLaunchKernelGGL(k_next_v, MDG_GRID(n, kIonThreads), boxes, items);
+  if (n > 0) MDG_LAUNCH(k_next_v, MDG_GRID(n, kIonThreads), boxes, items);
 }
 
 void launch_next_r(const BoxDesc* boxes, const Item* items, int n, hipStream_t s) {
-  if (n > 0) hipLaunchKernelGGL(k_next_r, MDG_GRID(n, kIonThreads), boxes, items);
+  if (n > 0) MDG_LAUNCH(k_next_r, MDG_GRID(n, kIonThreads), boxes, items);
 }
 
 void launch_box_scalars(const BoxDesc* boxes, const int* ids, int n, int flags, hipStream_t s) {
-  if (n > 0) hipLaunchKernelGGL(k_box_scalars, MDG_GRID(n, kScalThreads), boxes, ids, flags);
+  if (n > 0) MDG_LAUNCH(k_box_scalars, MDG_GRID(n, kScalThreads), boxes, ids, flags);
 }
 
 void launch_moment_correction(const BoxDesc* boxes, const int* ids, int n, hipStream_t s) {
-  if (n > 0) hipLaunchKernelGGL(k_moment_correction, MDG_GRID(n, kScalThreads), boxes, ids);
+  if (n > 0) MDG_LAUNCH(k_moment_correction, MDG_GRID(n, kScalThreads), boxes, ids);
 }
 
 // Plain copy of n16 16-byte words: the reference kernel of known traffic
@@ -105,7 +105,7 @@ __global__ __launch_bounds__(256) void k_copy_probe(const float4* __restrict__ s
 void launch_copy_probe(const void* src, void* dst, size_t n16, hipStream_t s) {
   if (n16 == 0) return;
   const size_t blocks = std::min<size_t>((n16 + 255) / 256, 8192);
-  hipLaunchKernelGGL(k_copy_probe, MDG_GRID(blocks, 256), (const float4*)src, (float4*)dst, n16);
+  MDG_LAUNCH(k_copy_probe, MDG_GRID(blocks, 256), (const float4*)src, (float4*)dst, n16);
 }
 
 }  // namespace mdg

@@ -15,24 +15,24 @@ void launch_real32(bool analytic, const BoxDesc* boxes, const Item* items, int n
                    hipStream_t s) {
   if (n <= 0) return;
   if (analytic)
-    hipLaunchKernelGGL(k_real_win<kRealAnalytic32>, MDG_GRID(n, kRealThreads), boxes, items, mode);
+    MDG_LAUNCH(k_real_win<kRealAnalytic32>, MDG_GRID(n, kRealThreads), boxes, items, mode);
   else
-    hipLaunchKernelGGL(k_real_win<kRealTable32>, MDG_GRID(n, kRealThreads), boxes, items, mode);
+    MDG_LAUNCH(k_real_win<kRealTable32>, MDG_GRID(n, kRealThreads), boxes, items, mode);
 }
 
 void launch_mixed_tables(const BoxDesc* boxes, const Item* items, int n, hipStream_t s) {
-  if (n > 0) hipLaunchKernelGGL(k_mixed_tables, MDG_GRID(n, kIonThreads), boxes, items);
+  if (n > 0) MDG_LAUNCH(k_mixed_tables, MDG_GRID(n, kIonThreads), boxes, items);
 }
 
 void launch_kf_pack(const BoxDesc* boxes, const Item* items, int n, hipStream_t s) {
-  if (n > 0) hipLaunchKernelGGL(k_kf_pack, MDG_GRID(n, kWave), boxes, items);
+  if (n > 0) MDG_LAUNCH(k_kf_pack, MDG_GRID(n, kWave), boxes, items);
 }
 
 void launch_kforce_mfma(int kb, const BoxDesc* boxes, const Item* items, int n, hipStream_t s) {
   if (n <= 0) return;
   const int t = kKfMfmaWaves * kWave;
 #define MDG_KF(K) \
-  hipLaunchKernelGGL((k_kforce_mfma<K>), MDG_GRID(n, t), boxes, items)
+  MDG_LAUNCH((k_kforce_mfma<K>), MDG_GRID(n, t), boxes, items)
   switch (kb) {
     case 1: MDG_KF(1); break;
     case 2: MDG_KF(2); break;
@@ -55,7 +55,7 @@ void launch_sk_mfma(int hl, const BoxDesc* boxes, const Item* items, int n, hipS
                               lds);
     attr_set = lds;
   }
-  hipLaunchKernelGGL(k_sk_mfma, MDG_GRID_LDS(n, kSkmThreads, lds), boxes, items, hl);
+  MDG_LAUNCH(k_sk_mfma, MDG_GRID_LDS(n, kSkmThreads, lds), boxes, items, hl);
 }
 
 }  // namespace mdg

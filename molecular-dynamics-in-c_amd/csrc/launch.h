@@ -2,9 +2,21 @@
 // workgroup per work item on `s`; the caller owns the item arrays.
 #pragma once
 
+#include <hip/hip_ext.h>
+
 #include "mdg_internal.h"
 
 namespace mdg {
+
+// Kernel timing.  While `start` is set, the launchers hand these events to
+// hipExtLaunchKernel, which stamps them with the kernel's own start and end
+// (its dispatch timestamps, no marker packets between kernels), and set
+// `used`.  Set only around eager launches, never during graph capture.
+struct LaunchEvents {
+  hipEvent_t start = nullptr, stop = nullptr;
+  bool used = false;
+};
+extern thread_local LaunchEvents g_launch_events;
 
 // k_fp64.hip (compiled with -ffp-contract=off)
 void launch_cell_build(const BoxDesc* boxes, const int* ids, int n, hipStream_t s);
@@ -43,3 +55,18 @@ void launch_sk_mfma(int hl, const BoxDesc* boxes, const Item* items, int n, hipS
 int kforce_hz_for(int hmax);
 
 }  // namespace mdg
+
+// MDG_LAUNCH(kernel, grid, block, lds, stream, args...): a kernel launch
+// that carries g_launch_events when they are set.
+#define MDG_LAUNCH(...) MDG_LAUNCH_(__VA_ARGS__)
+#define MDG_LAUNCH_(kernel, grid, block, lds, strm, ...)                                 \
+  do {                                                                                  \
+    ::mdg::LaunchEvents& ev_ = ::mdg::g_launch_events;                                  \
+    if (ev_.start) {                                                                    \
+      hipExtLaunchKernelGGL(kernel, grid, block, lds, strm, ev_.start, ev_.stop, 0u,    \
+                            __VA_ARGS__);                                               \
+      ev_.used = true;                                                                  \
+    } else {                                                                            \
+      hipLaunchKernelGGL(kernel, grid, block, lds, strm, __VA_ARGS__);                  \
+    }                                                                                   \
+  } while (0)

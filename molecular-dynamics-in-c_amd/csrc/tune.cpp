@@ -77,8 +77,10 @@ int time_candidate(const BoxSpec& base, const mdg_box_params& prm, int precision
     rc = set.set_boxes(boxes);
     std::vector<std::string> names;
     std::vector<double> ms;
-    if (rc == MDG_OK) rc = set.profile(1, s, &names, &ms);
-    if (rc == MDG_OK) rc = set.profile(3, s, &names, &ms);
+    std::vector<int> launches;
+    // each family once over all copies, back to back (families are timed apart)
+    if (rc == MDG_OK) rc = set.profile(1, s, false, &names, &ms, &launches);
+    if (rc == MDG_OK) rc = set.profile(3, s, false, &names, &ms, &launches);
     if (rc == MDG_OK) {
       *out = Run();
       for (size_t q = 0; q < names.size(); ++q) {

@@ -178,7 +178,7 @@ def lib():
             "mdg_batch_scalars": ([vp, C.c_int, C.POINTER(Scalars)], C.c_int),
             "mdg_batch_sync": ([vp], C.c_int),
             "mdg_batch_profile": ([vp, C.c_int, C.c_int, C.c_void_p, _dp,
-                                   C.POINTER(C.c_int)], C.c_int),
+                                   C.POINTER(C.c_int), C.POINTER(C.c_int)], C.c_int),
             "mdg_batch_work": ([vp, C.c_int, C.POINTER(C.c_longlong), C.POINTER(C.c_int),
                                 C.POINTER(C.c_int)], C.c_int),
             "mdg_error_string": ([C.c_int], C.c_char_p),
@@ -346,12 +346,17 @@ class Batch:
         return dict(pairs=p.value, nk=k.value, n_ions=n.value)
 
     def profile(self, nsteps):
+        """Per kernel name: (ms per step over all its launches, launches per
+        step), timed in the step's own schedule."""
         cap = 16
         names = (C.c_char * 32 * cap)()
         ms = np.zeros(cap)
+        cnt = np.zeros(cap, dtype=np.int32)
         n = C.c_int()
-        check(lib().mdg_batch_profile(self.h, nsteps, cap, names, _p(ms), C.byref(n)), "profile")
-        return {bytes(names[k]).split(b"\0")[0].decode(): float(ms[k]) for k in range(n.value)}
+        check(lib().mdg_batch_profile(self.h, nsteps, cap, names, _p(ms), _ip(cnt), C.byref(n)),
+              "profile")
+        return {bytes(names[k]).split(b"\0")[0].decode(): (float(ms[k]), int(cnt[k]))
+                for k in range(n.value)}
 
 
 def tune(boxes, precision, context_ions=0):

@@ -102,6 +102,31 @@ def test_concurrent_box_groups_match_lone_boxes(gpu):
     mix.close()
 
 
+def test_profile_times_the_step_schedule(gpu):
+    """profile() times every launch of the step as the step schedules it:
+    with 16 or more boxes the reciprocal chain runs once per box group, the
+    real space and the update once per step; and profiling steps the state
+    exactly as step() does."""
+    ps = [_params(2, H.A_NACL)] * 16
+    r, v, sp = _state(ps[0], 1)
+    out = []
+    for prof_first in (True, False):
+        b = mdgpu.Batch(ps, mdgpu.MIXED)
+        for k in range(len(ps)):
+            b.set_state(k, r=r, v=v, species=sp)
+        if prof_first:
+            prof = b.profile(3)
+        else:
+            b.step(3)
+        out.append(b.state(5)["r"])
+        b.close()
+    assert np.array_equal(out[0], out[1])
+    assert prof["real_space"][1] == 1 and prof["force_vv"][1] == 1
+    for k in ("phase_tables", "sk_partial", "sk_finalize", "kf_pack", "kforce"):
+        assert prof[k][1] == 2, (k, prof[k])
+    assert all(ms > 0 for ms, _ in prof.values())
+
+
 def test_graph_replay_equals_eager_steps(gpu):
     p = _params(4, H.A_NACL)
     r, v, sp = _state(p, 7)
