@@ -314,6 +314,26 @@ def main():
     for k in range(len(boxes)):
         st = b.state(k)
         states_now.append((st["r"], st["v"], states[k][2]))
+    # the rest of mdc's step loop (main.c:44-60) armed on every box:
+    # calc_msd each step, control_temp every 10 steps to the box's
+    # temperature, moment_correction every 1000 steps
+    for k in range(len(boxes)):
+        b.set_observe(k, msd=True, t_control_step=10, temp_K=temps[k], t_control_until=10000,
+                      moment_every=1000)
+    b.step(args.warmup)
+    b.sync()
+    t0 = time.perf_counter()
+    b.step(args.steps)
+    b.sync()
+    loop_ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    loop_prof = b.profile(10)
+    step_loop = {"ms_per_step": loop_ms, "ion_steps_per_s": total_ions / (loop_ms * 1e-3),
+                 "observe_ms_per_step": loop_prof.get("observe", (None,))[0],
+                 "scalings_box0": b.observed(0)["scalings"],
+                 "armed": "calc_msd every step, control_temp every 10 steps (box temperature), "
+                          "moment_correction every 1000 steps"}
+    log(f"  with mdc's loop armed: {loop_ms:.3f} ms/step "
+        f"(observe kernel {step_loop['observe_ms_per_step']:.4f} ms)")
     b.close()
 
     kn = sum(w["nk"] * w["n_ions"] for w in work)
@@ -401,6 +421,7 @@ def main():
                                     float(np.median(np.abs(drift))),
                                     "max_abs": float(np.abs(drift).max()), "unit": "1e-18 J"},
         "autotune": tune_rep,
+        "mdc_step_loop_armed": step_loop,
         "hbm_copy_probe": probe,
         "serving": serve,
         "roofline": roof,

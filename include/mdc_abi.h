@@ -100,11 +100,23 @@ typedef struct {
   double D_na_na, D_na_cl, D_cl_cl;
 } huggins_mayer_potential_set;
 
+/* mdc/program/md.h:152-172 (calc_msd's state; init.c:143-151 allocates
+ * the arrays: N + 1 doubles each, kinds_of_ions entries for value and
+ * number_of_ion) */
+typedef struct {
+  int    *number_of_ion;
+  double *rx_old, *ry_old, *rz_old;
+  double *dx, *dy, *dz;
+  double *value;
+  double  Lx2, Ly2, Lz2;
+} msd_calculation_variables;
+
 /* mdc/program/md.h:197-202 and potential.h:28 */
 extern system_property sys;
 extern ion_property ion;
 extern calc_control ctl;
 extern huggins_mayer_potential_set hm;
+extern msd_calculation_variables msd;
 
 /* ---- force path, mdc/program/prototypes.h (same names, same meaning) ---- */
 void   calc_foc(void);          /* prototypes.h:53  rv.c:12-18        */
@@ -120,6 +132,10 @@ void   moment_correction(void); /* prototypes.h:58  rv.c:88-116       */
 void   calc_press(void);        /* prototypes.h:40  pt.c:8-21         */
 void   mk_table(void);          /* prototypes.h:14  control.c:183-248 (host; NaCl/KCl form) */
 double erfcc(double x);         /* prototypes.h:18  ext.c:73-104      */
+
+/* ---- the rest of mdc's step loop (main.c:44-60) ---- */
+void   calc_msd(void);          /* prototypes.h:62  msd.c:20-81       */
+void   control_temp(int d_step, double temp);  /* prototypes.h:42  pt.c:85-112 */
 
 /* ---- additions of the drop-in (no counterpart in mdc) ---- */
 /* 0 = ok, otherwise an MDG_E* code of mdgpu.h; sticky until cleared */

@@ -132,6 +132,31 @@ int  mdg_batch_force_terms(mdg_batch *b, int terms);
 int  mdg_batch_step(mdg_batch *b, int nsteps);
 int  mdg_batch_scalars(mdg_batch *b, int box, mdg_scalars *out);
 int  mdg_batch_sync(mdg_batch *b);
+
+/* The rest of mdc's step loop (mdc/program/main.c:44-60), run on the device
+ * inside every step for the boxes that arm it, in mdc's order after the
+ * integration and calc_kin/calc_press:
+ *   calc_msd           mdc/program/msd.c:20-81, every step;
+ *   control_temp       mdc/program/pt.c:85-112, while step < t_control_until;
+ *   moment_correction  mdc/program/rv.c:88-116, every moment_every steps.
+ * The step is the box's own count of completed steps (mdc's sys.step).
+ * Arming resets the state that mdc keeps in msd and in control_temp's
+ * statics: the MSD origin is the positions after the next step, and the
+ * thermostat's kinetic-energy sum restarts.  All fields 0 disarms. */
+typedef struct {
+  int       msd;              /* 1: calc_msd every step                  */
+  int       t_control_step;   /* ctl.t_control_step; 0: no thermostat    */
+  double    temp_K;           /* ctl.temp                                */
+  long long t_control_until;  /* thermostat while step < this (mdc 10000) */
+  int       moment_every;     /* mdc: 1000; 0: off                       */
+} mdg_observe;
+int  mdg_batch_set_observe(mdg_batch *b, int box, const mdg_observe *o);
+/* msd[4]: MSD per species [A^2] (msd.value) after the last step;
+ * scalings: velocity scalings done since arming.  Either may be NULL. */
+int  mdg_batch_observed(mdg_batch *b, int box, double *msd, long long *scalings);
+/* MSD state: old = positions of the last sample (msd.rx_old, ry_old,
+ * rz_old), d = summed displacements (msd.dx, dy, dz); 3N SoA each, or NULL */
+int  mdg_batch_msd_state(mdg_batch *b, int box, double *old, double *d);
 /* Kernel timing over the next `nsteps` steps, launched eagerly in the
  * step's own schedule (box groups on several streams); every launch
  * carries a start and a stop hipEvent stamped by its own dispatch

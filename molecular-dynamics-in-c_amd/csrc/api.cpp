@@ -249,6 +249,35 @@ int mdg_batch_step(mdg_batch* b, int nsteps) {
   return b->set->step(nsteps, b->stream);
 }
 
+int mdg_batch_set_observe(mdg_batch* b, int box, const mdg_observe* o) {
+  if (!b || !o || box < 0 || box >= (int)b->boxes.size()) return MDG_E_ARG;
+  ObserveOpts x;
+  x.msd = o->msd;
+  x.t_control_step = o->t_control_step;
+  x.temp_K = o->temp_K;
+  x.t_control_until = o->t_control_until;
+  x.moment_every = o->moment_every;
+  int rc = box_set_observe(b->boxes[box], x, b->stream);
+  if (rc != MDG_OK) return rc;
+  return b->set->set_boxes(b->boxes);  // the observe list changes
+}
+
+int mdg_batch_observed(mdg_batch* b, int box, double* msd, long long* scalings) {
+  if (!b || box < 0 || box >= (int)b->boxes.size()) return MDG_E_ARG;
+  double O[kObsLen];
+  int rc = box_read_observe(b->boxes[box], O, nullptr, nullptr, b->stream);
+  if (rc != MDG_OK) return rc;
+  if (msd)
+    for (int k = 0; k < 4; ++k) msd[k] = O[kObsMsd + k];
+  if (scalings) *scalings = (long long)O[kObsScalings];
+  return MDG_OK;
+}
+
+int mdg_batch_msd_state(mdg_batch* b, int box, double* old, double* d) {
+  if (!b || box < 0 || box >= (int)b->boxes.size()) return MDG_E_ARG;
+  return box_read_observe(b->boxes[box], nullptr, old, d, b->stream);
+}
+
 int mdg_batch_sync(mdg_batch* b) {
   if (!b) return MDG_E_ARG;
   MDG_HIP(hipStreamSynchronize(b->stream));

@@ -28,6 +28,7 @@ system_property sys;
 ion_property ion;
 calc_control ctl;
 huggins_mayer_potential_set hm;
+msd_calculation_variables msd;
 
 namespace {
 
@@ -314,6 +315,68 @@ void moment_correction(void) {
   if (!ensure("moment_correction") || !put(kV, "moment_correction")) return;
   g.set->enqueue_moment_correction(g.stream);
   get(kV, "moment_correction");
+}
+
+// msd.c:20-81.  The device box keeps no MSD state between calls: msd's
+// arrays are uploaded, updated on the device and written back.
+void calc_msd(void) {
+  if (!ensure("calc_msd")) return;
+  const int N = sys.N;
+  double* host[6] = {msd.rx_old, msd.ry_old, msd.rz_old, msd.dx, msd.dy, msd.dz};
+  for (double* p : host)
+    if (!p) return fail(MDG_E_ARG, "calc_msd");
+  if (!msd.value || !msd.number_of_ion) return fail(MDG_E_ARG, "calc_msd");
+  BoxDesc& D = g.box->desc;
+  if (!D.msd_on) {
+    D.msd_on = 1;
+    g.box->desc_dirty = true;
+  }
+  if (g.set->sync_desc(g.stream) != MDG_OK || !put(kR, "calc_msd")) return;
+  double* dev[6] = {D.msd_old, D.msd_old + N, D.msd_old + 2 * N,
+                    D.msd_d,   D.msd_d + N,   D.msd_d + 2 * N};
+  for (int k = 0; k < 6; ++k)
+    if (hipMemcpyAsync(dev[k], host[k], sizeof(double) * N, hipMemcpyHostToDevice,
+                       g.stream) != hipSuccess)
+      return fail(MDG_E_HIP, "calc_msd");
+  // bit 0: MSD only; bit 3: the origin is recorded at step 1 only, as mdc
+  g.set->enqueue_observe(g.stream, sys.step, 1 | 8);
+  for (int k = 0; k < 6; ++k)
+    if (hipMemcpyAsync(host[k], dev[k], sizeof(double) * N, hipMemcpyDeviceToHost,
+                       g.stream) != hipSuccess)
+      return fail(MDG_E_HIP, "calc_msd");
+  double O[kObsLen];
+  if (box_read_observe(g.box, O, nullptr, nullptr, g.stream) != MDG_OK)
+    return fail(MDG_E_HIP, "calc_msd");
+  if (sys.step == 1) return;  // origin recorded; nothing else changes
+  msd.Lx2 = sys.Lx / 2.0;
+  msd.Ly2 = sys.Ly / 2.0;
+  msd.Lz2 = sys.Lz / 2.0;
+  for (int k = 0; k < ctl.kinds_of_ions && k < 4; ++k) {
+    msd.value[k] = O[kObsMsd + k];
+    msd.number_of_ion[k] = (int)O[kObsMsdIons + k];
+  }
+}
+
+// pt.c:85-112.  The averaging over steps and the scale factor are scalar
+// work on the host, kept in statics as mdc keeps them; the velocities are
+// scaled on the device.
+void control_temp(int d_step, double temp) {
+  static double oldk = 0.0;
+  static short count = 0;
+  if (d_step <= 0) return fail(MDG_E_ARG, "control_temp");
+  count++;
+  oldk += sys.kin;
+  if (sys.step % d_step != 0) return;
+  const double averaged_K = oldk / (double)count;
+  oldk = 0.0;
+  count = 0;
+  const double tempK = temp / sys.e2t;
+  const double scale = sqrt(tempK / averaged_K);
+  if (!ensure("control_temp") || !put(kV, "control_temp")) return;
+  g.set->enqueue_scale_v(g.stream, scale);
+  if (!get(kV, "control_temp")) return;
+  printf("---------------------------------------------------------------\n");
+  printf(">> Temperature scaling\n");
 }
 
 // pt.c:8-21 (six scalar operations: evaluated on the host)

@@ -342,7 +342,7 @@ struct Layout {
   size_t ion[12], typ, rpf, cell_of, cnt, cwave, start, sorted, s[3], styp;
   size_t cols, segs, quads, khz, kslot, kce, kcf, kab, kab32, kpot, tx, ty, tw, skp, fp;
   size_t wb, kfa, colw, kft, sb;
-  size_t tab, tab32, rpart, rpairs, scal, err, total;
+  size_t tab, tab32, rpart, rpairs, scal, err, msd_old, msd_d, obs, total;
 };
 
 static size_t take(size_t& off, size_t bytes) {
@@ -407,6 +407,9 @@ static Layout layout(const BoxPlan& P, int precision) {
   L.rpairs = take(off, sizeof(unsigned long long) * P.nreal);
   L.scal = take(off, sizeof(double) * kScalCount);
   L.err = take(off, sizeof(int));
+  L.msd_old = take(off, sizeof(double) * 3 * N);
+  L.msd_d = take(off, sizeof(double) * 3 * N);
+  L.obs = take(off, sizeof(double) * kObsLen);
   L.total = (off + 255) & ~(size_t)255;
   return L;
 }
@@ -565,6 +568,9 @@ int box_create(const BoxSpec& spec, int precision, Box** out, std::string* why) 
   D.nreal = P.nreal;
   D.scal = (double*)(base + L.scal);
   D.err = (int*)(base + L.err);
+  D.msd_old = (double*)(base + L.msd_old);
+  D.msd_d = (double*)(base + L.msd_d);
+  D.obs = (double*)(base + L.obs);  // observables off until set_observe()
   if (err != MDG_OK) {
     if (why) *why = "uploading box constants failed";
     box_destroy(b);
@@ -641,6 +647,47 @@ int box_read_scalars(Box* b, double* scal, hipStream_t s) {
 
 int box_read_error(Box* b, int* err, hipStream_t s) {
   MDG_HIP(hipMemcpyAsync(err, b->desc.err, sizeof(int), hipMemcpyDeviceToHost, s));
+  MDG_HIP(hipStreamSynchronize(s));
+  return MDG_OK;
+}
+
+int box_set_observe(Box* b, const ObserveOpts& o, hipStream_t s) {
+  if (!b || o.t_control_step < 0 || o.moment_every < 0 || !(o.temp_K >= 0.0))
+    return MDG_E_ARG;
+  BoxDesc& D = b->desc;
+  const int N = D.N;
+  D.msd_on = o.msd ? 1 : 0;
+  D.t_control_step = o.t_control_step;
+  D.t_control_until = o.t_control_until;
+  // tempK = temp / e2t (pt.c:99) with e2t = 2 / (3 N kB) (init.c:67,75)
+  const double e2t = 2.0 / (3.0 * ((double)N * 1.380658e-5));
+  D.t_control_ke = o.temp_K / e2t;
+  D.moment_every = o.moment_every;
+  b->desc_dirty = true;
+  MDG_HIP(hipMemsetAsync(D.obs, 0, sizeof(double) * kObsLen, s));
+  MDG_HIP(hipMemsetAsync(D.msd_old, 0, sizeof(double) * 3 * (size_t)N, s));
+  MDG_HIP(hipMemsetAsync(D.msd_d, 0, sizeof(double) * 3 * (size_t)N, s));
+  MDG_HIP(hipStreamSynchronize(s));
+  return MDG_OK;
+}
+
+int box_read_observe(Box* b, double* obs, double* old, double* d, hipStream_t s) {
+  const BoxDesc& D = b->desc;
+  const size_t n3 = sizeof(double) * 3 * (size_t)D.N;
+  if (obs) MDG_HIP(hipMemcpyAsync(obs, D.obs, sizeof(double) * kObsLen, hipMemcpyDeviceToHost, s));
+  if (old) MDG_HIP(hipMemcpyAsync(old, D.msd_old, n3, hipMemcpyDeviceToHost, s));
+  if (d) MDG_HIP(hipMemcpyAsync(d, D.msd_d, n3, hipMemcpyDeviceToHost, s));
+  MDG_HIP(hipStreamSynchronize(s));
+  return MDG_OK;
+}
+
+int box_write_observe(Box* b, const double* obs, const double* old, const double* d,
+                      hipStream_t s) {
+  const BoxDesc& D = b->desc;
+  const size_t n3 = sizeof(double) * 3 * (size_t)D.N;
+  if (obs) MDG_HIP(hipMemcpyAsync(D.obs, obs, sizeof(double) * kObsLen, hipMemcpyHostToDevice, s));
+  if (old) MDG_HIP(hipMemcpyAsync(D.msd_old, old, n3, hipMemcpyHostToDevice, s));
+  if (d) MDG_HIP(hipMemcpyAsync(D.msd_d, d, n3, hipMemcpyHostToDevice, s));
   MDG_HIP(hipStreamSynchronize(s));
   return MDG_OK;
 }
@@ -803,6 +850,8 @@ int BoxSet::set_boxes(const std::vector<Box*>& boxes) {
     const int g = group_of[bi];
     std::vector<Item>& real = L[LReal][g];
     I[IAll][g].push_back(bi);
+    const BoxDesc& D = B->desc;
+    if (D.msd_on || D.t_control_step > 0 || D.moment_every > 0) I[IObs][g].push_back(bi);
     if (P.all_pairs) {
       int slot = 0;
       for (int i0 = 0; i0 < N; i0 += kRealThreads)
@@ -915,11 +964,11 @@ int BoxSet::sync_desc(hipStream_t s) {
 // Kernels as profile() names them.  Mixed path: "phase_tables" is
 // k_mixed_tables (the phase tables and both passes' MFMA B operands).
 enum {
-  KCell, KTables, KReal, KSk, KFin, KKf, KVv, KScal, KPack, kKernelNames
+  KCell, KTables, KReal, KSk, KFin, KKf, KVv, KScal, KPack, KObs, kKernelNames
 };
 static const char* const kKernelName[kKernelNames] = {
     "cell_build", "phase_tables", "real_space", "sk_partial", "sk_finalize",
-    "kforce",     "force_vv",     "box_scalars", "kf_pack"};
+    "kforce",     "force_vv",     "box_scalars", "kf_pack",   "observe"};
 
 thread_local LaunchEvents g_launch_events;
 
@@ -1038,6 +1087,12 @@ void BoxSet::enqueue_step(hipStream_t s, bool vv) {
   timed(KScal, s, [&] {
     launch_box_scalars(d_desc_, d_ids_ + all_.ioff[IAll], all_.in[IAll], vv ? 1 : 0, s);
   });
+  // mdc's calc_msd, control_temp and moment_correction of the boxes that
+  // have them armed (main.c:44-60)
+  if (vv && all_.in[IObs])
+    timed(KObs, s, [&] {
+      launch_observe(d_desc_, d_ids_ + all_.ioff[IObs], all_.in[IObs], -1, 7, s);
+    });
 }
 
 void BoxSet::enqueue_scalars(hipStream_t s, bool count_step) {
@@ -1054,6 +1109,12 @@ void BoxSet::enqueue_next_r(hipStream_t s) {
 }
 void BoxSet::enqueue_moment_correction(hipStream_t s) {
   launch_moment_correction(d_desc_, d_ids_ + all_.ioff[IAll], all_.in[IAll], s);
+}
+void BoxSet::enqueue_observe(hipStream_t s, long long step, int phases) {
+  launch_observe(d_desc_, d_ids_ + all_.ioff[IAll], all_.in[IAll], step, phases, s);
+}
+void BoxSet::enqueue_scale_v(hipStream_t s, double scale) {
+  launch_scale_v(d_desc_, d_items_ + all_.off[LIon], all_.n[LIon], scale, s);
 }
 
 int BoxSet::step(int nsteps, hipStream_t s) {

@@ -109,6 +109,23 @@ int box_download_state(Box* b, double* r, double* v, double* f, double* f0, hipS
 int box_read_scalars(Box* b, double* scal /*kScalCount*/, hipStream_t s);
 int box_read_error(Box* b, int* err, hipStream_t s);
 
+// The per-step observables and thermostat of mdc's step loop for one box
+// (main.c:44-60).  Arming resets their state: the MSD origin is recorded
+// at the next step, and the thermostat's running sums restart.
+struct ObserveOpts {
+  int msd = 0;                       // calc_msd every step
+  int t_control_step = 0;            // control_temp period; 0: off
+  double temp_K = 0;                 // control_temp's target temperature
+  long long t_control_until = 10000; // control_temp only while step < this
+  int moment_every = 0;              // moment_correction period; 0: off
+};
+int box_set_observe(Box* b, const ObserveOpts& o, hipStream_t s);
+// obs: kObsLen values; old, d: 3N values each (x, y, z blocks) or null
+int box_read_observe(Box* b, double* obs, double* old, double* d, hipStream_t s);
+// upload msd_old, msd_d (3N values each, or null) and the observable state
+int box_write_observe(Box* b, const double* obs, const double* old, const double* d,
+                      hipStream_t s);
+
 // Box groups of a batch whose step chains run concurrently (16+ boxes).
 constexpr int kStepGroups = 2;
 constexpr double kGroup0Share = 0.5;  // share of the work in the group that starts k-space first
@@ -132,6 +149,9 @@ class BoxSet {
   void enqueue_next_v(hipStream_t s);
   void enqueue_next_r(hipStream_t s);
   void enqueue_moment_correction(hipStream_t s);
+  // k_observe on every box (step >= 0: mdc's sys.step; see launch_observe)
+  void enqueue_observe(hipStream_t s, long long step, int phases);
+  void enqueue_scale_v(hipStream_t s, double scale);
   int step(int nsteps, hipStream_t s);          // graph replay
   // Kernel timing over `nsteps` eager steps: every launch carries a start
   // and a stop hipEvent that its own dispatch stamps.  as_step: the step's own schedule
@@ -153,7 +173,7 @@ class BoxSet {
   // offsets into d_items_ / d_ids_ and counts
   // Work lists: item ranges in d_items_ per kernel, box-id ranges in d_ids_.
   enum { LReal, LTab, LSk, LFin, LKf, LIon, LPack, kLists };
-  enum { ICell, IAll, kIdLists };
+  enum { ICell, IAll, IObs, kIdLists };
   struct Ranges {
     int off[kLists] = {}, n[kLists] = {};
     int ioff[kIdLists] = {}, in[kIdLists] = {};

@@ -93,6 +93,16 @@ void launch_moment_correction(const BoxDesc* boxes, const int* ids, int n, hipSt
   if (n > 0) MDG_LAUNCH(k_moment_correction, MDG_GRID(n, kScalThreads), boxes, ids);
 }
 
+void launch_scale_v(const BoxDesc* boxes, const Item* items, int n, double scale,
+                    hipStream_t s) {
+  if (n > 0) MDG_LAUNCH(k_scale_v, MDG_GRID(n, kIonThreads), boxes, items, scale);
+}
+
+void launch_observe(const BoxDesc* boxes, const int* ids, int n, long long step, int phases,
+                    hipStream_t s) {
+  if (n > 0) MDG_LAUNCH(k_observe, MDG_GRID(n, kScalThreads), boxes, ids, step, phases);
+}
+
 // Plain copy of n16 16-byte words: the reference kernel of known traffic
 // for scaling hardware byte counters, and an HBM bandwidth yardstick.
 __global__ __launch_bounds__(256) void k_copy_probe(const float4* __restrict__ src,

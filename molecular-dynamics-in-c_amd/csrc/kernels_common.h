@@ -365,11 +365,23 @@ __global__ __launch_bounds__(kScalThreads) void k_box_scalars(const BoxDesc* __r
   }
 }
 
-// mdc's moment_correction (rv.c:88-116) for one box per workgroup.
-__global__ __launch_bounds__(kScalThreads) void k_moment_correction(
-    const BoxDesc* __restrict__ boxes, const int* __restrict__ box_ids) {
+// v <- scale * v (the velocity scaling of control_temp, pt.c:101-105)
+__global__ __launch_bounds__(kIonThreads) void k_scale_v(const BoxDesc* __restrict__ boxes,
+                                                         const Item* __restrict__ items,
+                                                         double scale) {
+  const Item it = items[blockIdx.x];
+  const BoxDesc& B = boxes[it.box];
+  const int i = it.a + threadIdx.x;
+  if (i >= B.N) return;
+  B.vx[i] *= scale;
+  B.vy[i] *= scale;
+  B.vz[i] *= scale;
+}
+
+// mdc's moment_correction (rv.c:88-116) of box B by one workgroup: the
+// mean momentum per ion is removed, v -= (sum m v / N) / m.
+__device__ __forceinline__ void moment_correct_box(const BoxDesc& B) {
   __shared__ double mean[3];
-  const BoxDesc& B = boxes[box_ids[blockIdx.x]];
   const int tid = threadIdx.x;
   double v[3] = {0, 0, 0};
   for (int i = tid; i < B.N; i += kScalThreads) {
@@ -390,6 +402,121 @@ __global__ __launch_bounds__(kScalThreads) void k_moment_correction(
     B.vx[i] -= mean[0] / m;
     B.vy[i] -= mean[1] / m;
     B.vz[i] -= mean[2] / m;
+  }
+}
+
+__global__ __launch_bounds__(kScalThreads) void k_moment_correction(
+    const BoxDesc* __restrict__ boxes, const int* __restrict__ box_ids) {
+  moment_correct_box(boxes[box_ids[blockIdx.x]]);
+}
+
+// ----------------------------------------------------------------------
+// The rest of one iteration of mdc's step loop after the integration and
+// calc_kin/calc_press (main.c:44-60), one box per workgroup, in mdc's
+// order:
+//  * calc_msd (msd.c:20-81): at the origin step the positions are
+//    recorded; afterwards each ion's displacement since the last sample,
+//    folded once by the strict half-box tests, is added to its summed
+//    displacement, and the MSD per species is the mean of |d|^2;
+//  * control_temp (pt.c:85-112) while step < t_control_until: the kinetic
+//    energy of every step is summed, and every t_control_step steps the
+//    velocities are scaled by sqrt(target / mean) and the sum restarts;
+//  * moment_correction (rv.c:88-116) every moment_every steps.
+// step >= 0 is mdc's sys.step of the iteration; step < 0 takes the box's
+// own count of completed steps.  phases: bit 0 MSD, bit 1 thermostat,
+// bit 2 moment correction, each also subject to the box's own switches;
+// bit 3: the MSD origin is recorded only at step 1 (mdc's rule), else also
+// at the first sample after the observables were (re)armed.
+// ----------------------------------------------------------------------
+__global__ __launch_bounds__(kScalThreads) void k_observe(const BoxDesc* __restrict__ boxes,
+                                                          const int* __restrict__ box_ids,
+                                                          long long step_arg, int phases) {
+  const BoxDesc& B = boxes[box_ids[blockIdx.x]];
+  const int tid = threadIdx.x;
+  const int N = B.N;
+  double* O = B.obs;
+  const long long step = step_arg >= 0 ? step_arg : (long long)B.scal[kScalStep];
+  if ((phases & 1) && B.msd_on) {
+    const bool origin = step == 1 || (!(phases & 8) && O[kObsMsdInit] == 0.0);
+    __syncthreads();  // every thread has read the origin flag
+    double* ox = B.msd_old;
+    double* oy = ox + N;
+    double* oz = oy + N;
+    if (origin) {
+      for (int i = tid; i < N; i += kScalThreads) {
+        ox[i] = B.rx[i];
+        oy[i] = B.ry[i];
+        oz[i] = B.rz[i];
+      }
+      if (tid == 0) O[kObsMsdInit] = 1.0;
+    } else {
+      double* dx = B.msd_d;
+      double* dy = dx + N;
+      double* dz = dy + N;
+      double v[8] = {0, 0, 0, 0, 0, 0, 0, 0};  // |d|^2 sums, then ion counts, per species
+      for (int i = tid; i < N; i += kScalThreads) {
+        const double rx = B.rx[i], ry = B.ry[i], rz = B.rz[i];
+        double ddx = ox[i] - rx, ddy = oy[i] - ry, ddz = oz[i] - rz;
+        ox[i] = rx;
+        oy[i] = ry;
+        oz[i] = rz;
+        if (ddx > B.halfL[0]) ddx -= B.L[0];
+        if (ddx < -B.halfL[0]) ddx += B.L[0];
+        if (ddy > B.halfL[1]) ddy -= B.L[1];
+        if (ddy < -B.halfL[1]) ddy += B.L[1];
+        if (ddz > B.halfL[2]) ddz -= B.L[2];
+        if (ddz < -B.halfL[2]) ddz += B.L[2];
+        const double sx = dx[i] + ddx, sy = dy[i] + ddy, sz = dz[i] + ddz;
+        dx[i] = sx;
+        dy[i] = sy;
+        dz[i] = sz;
+        const double d2 = sx * sx + sy * sy + sz * sz;
+        const int t = B.typ[i];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          v[k] += t == k ? d2 : 0.0;
+          v[4 + k] += t == k ? 1.0 : 0.0;
+        }
+      }
+      block_sum<kScalThreads, 8>(v);
+      if (tid == 0)
+        for (int k = 0; k < B.kinds && k < 4; ++k) {
+          O[kObsMsd + k] = v[k] / v[4 + k];
+          O[kObsMsdIons + k] = v[4 + k];
+        }
+    }
+  }
+  if ((phases & 2) && B.t_control_step > 0 && step < B.t_control_until) {
+    __shared__ double scale;
+    __shared__ int scaled;
+    if (tid == 0) {
+      double count = O[kObsCount] + 1.0;
+      double oldk = O[kObsOldK] + B.scal[kScalKin];
+      scaled = 0;
+      if (step % B.t_control_step == 0) {
+        const double averaged = oldk / count;
+        oldk = 0.0;
+        count = 0.0;
+        scale = sqrt(B.t_control_ke / averaged);
+        scaled = 1;
+        O[kObsScalings] += 1.0;
+      }
+      O[kObsCount] = count;
+      O[kObsOldK] = oldk;
+    }
+    __syncthreads();
+    if (scaled) {
+      const double sc = scale;
+      for (int i = tid; i < N; i += kScalThreads) {
+        B.vx[i] *= sc;
+        B.vy[i] *= sc;
+        B.vz[i] *= sc;
+      }
+    }
+  }
+  if ((phases & 4) && B.moment_every > 0 && step % B.moment_every == 0) {
+    __syncthreads();  // the scaled velocities are visible to every thread
+    moment_correct_box(B);
   }
 }
 

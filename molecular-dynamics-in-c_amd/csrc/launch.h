@@ -36,6 +36,13 @@ void launch_next_v(const BoxDesc* boxes, const Item* items, int n, hipStream_t s
 void launch_next_r(const BoxDesc* boxes, const Item* items, int n, hipStream_t s);
 void launch_box_scalars(const BoxDesc* boxes, const int* ids, int n, int flags, hipStream_t s);
 void launch_moment_correction(const BoxDesc* boxes, const int* ids, int n, hipStream_t s);
+void launch_scale_v(const BoxDesc* boxes, const Item* items, int n, double scale,
+                    hipStream_t s);
+// mdc's calc_msd, control_temp and moment_correction after a step
+// (main.c:44-60); step < 0: the box's own step count; phases: bit 0 MSD,
+// bit 1 thermostat, bit 2 moment correction, bit 3 MSD origin only at step 1
+void launch_observe(const BoxDesc* boxes, const int* ids, int n, long long step, int phases,
+                    hipStream_t s);
 
 // k_mixed.hip (FP32 pair and k-space arithmetic)
 void launch_real32(bool analytic, const BoxDesc* boxes, const Item* items, int n, int mode,

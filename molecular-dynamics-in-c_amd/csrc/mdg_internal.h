@@ -160,6 +160,27 @@ struct BoxDesc {
   int nreal;
   double *scal;       // kScal* layout below
   int *err;
+  // ---- the per-step observables and thermostat of mdc's step loop
+  //      (main.c:44-60): calc_msd, control_temp, moment_correction ----
+  double *msd_old;    // [3][N] positions at the previous sample (msd.rx_old, ry_old, rz_old)
+  double *msd_d;      // [3][N] summed minimum-image displacements (msd.dx, dy, dz)
+  double *obs;        // kObs* layout below
+  int msd_on;         // calc_msd every step
+  int t_control_step; // control_temp period (ctl.t_control_step); 0: off
+  long long t_control_until;  // control_temp only while step < this (main.c:57: 10000)
+  double t_control_ke;        // target kinetic energy temp / e2t (pt.c:99)
+  int moment_every;   // moment_correction when step % moment_every == 0; 0: off
+};
+
+// per-box observable state (device double array)
+enum {
+  kObsMsd = 0,        // [4] MSD per species (msd.value)
+  kObsMsdIons = 4,    // [4] ions per species (msd.number_of_ion)
+  kObsOldK = 8,       // control_temp's running kinetic-energy sum (its static oldk)
+  kObsCount = 9,      // control_temp's sample count (its static count)
+  kObsMsdInit = 10,   // 1 once the MSD origin is recorded
+  kObsScalings = 11,  // velocity scalings done
+  kObsLen = 12
 };
 
 // per-box scalar layout (device double array)

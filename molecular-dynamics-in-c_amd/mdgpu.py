@@ -139,6 +139,19 @@ class HugginsMayer(C.Structure):
         "C_na_cl", "C_cl_cl", "D_na_na", "D_na_cl", "D_cl_cl")]
 
 
+class MsdVariables(C.Structure):
+    """mdc/program/md.h:152-172"""
+    _fields_ = [("number_of_ion", C.POINTER(C.c_int))] + \
+        [(n, _dp) for n in ("rx_old", "ry_old", "rz_old", "dx", "dy", "dz", "value")] + \
+        [(n, C.c_double) for n in ("Lx2", "Ly2", "Lz2")]
+
+
+class Observe(C.Structure):
+    """mdg_observe: the rest of mdc's step loop (main.c:44-60) per box."""
+    _fields_ = [("msd", C.c_int), ("t_control_step", C.c_int), ("temp_K", C.c_double),
+                ("t_control_until", C.c_longlong), ("moment_every", C.c_int)]
+
+
 DROPIN_ROUTINES = ("calc_foc", "clear_foc", "real_space", "reciprocal_space",
                    "reciprocal_space3", "next_rv_verlet", "next_r", "next_v", "calc_kin",
                    "moment_correction", "calc_press", "mk_table")
@@ -197,6 +210,11 @@ def lib():
             "mdg_dropin_clear_status": ([], None),
             "mdg_dropin_release": ([], None),
             "erfcc": ([C.c_double], C.c_double),
+            "mdg_batch_set_observe": ([vp, C.c_int, C.POINTER(Observe)], C.c_int),
+            "mdg_batch_observed": ([vp, C.c_int, _dp, C.POINTER(C.c_longlong)], C.c_int),
+            "mdg_batch_msd_state": ([vp, C.c_int, _dp, _dp], C.c_int),
+            "calc_msd": ([], None),
+            "control_temp": ([C.c_int, C.c_double], None),
         }
         for name, (args, res) in sig.items():
             f = getattr(L, name)
@@ -340,6 +358,25 @@ class Batch:
         check(lib().mdg_batch_scalars(self.h, b, C.byref(s)), "scalars")
         return s.as_dict()
 
+    def set_observe(self, b=0, msd=False, t_control_step=0, temp_K=0.0,
+                    t_control_until=10000, moment_every=0):
+        """Arm mdc's calc_msd / control_temp / moment_correction cadence for
+        box b inside every step (main.c:44-60); resets their state."""
+        o = Observe(1 if msd else 0, t_control_step, temp_K, t_control_until, moment_every)
+        check(lib().mdg_batch_set_observe(self.h, b, C.byref(o)), "set_observe")
+
+    def observed(self, b=0):
+        msd = np.zeros(4)
+        n = C.c_longlong()
+        check(lib().mdg_batch_observed(self.h, b, _p(msd), C.byref(n)), "observed")
+        return dict(msd=msd, scalings=n.value)
+
+    def msd_state(self, b=0):
+        n = self.n_ions(b)
+        old, d = np.zeros(3 * n), np.zeros(3 * n)
+        check(lib().mdg_batch_msd_state(self.h, b, _p(old), _p(d)), "msd_state")
+        return old.reshape(3, n), d.reshape(3, n)
+
     def work(self, b=0):
         p, k, n = C.c_longlong(), C.c_int(), C.c_int()
         check(lib().mdg_batch_work(self.h, b, C.byref(p), C.byref(k), C.byref(n)), "work")
@@ -412,6 +449,18 @@ def dropin_globals():
     L = lib()
     return (SystemProperty.in_dll(L, "sys"), IonProperty.in_dll(L, "ion"),
             CalcControl.in_dll(L, "ctl"), HugginsMayer.in_dll(L, "hm"))
+
+
+def dropin_msd():
+    return MsdVariables.in_dll(lib(), "msd")
+
+
+def dropin_control_temp(d_step, temp):
+    lib().control_temp(d_step, temp)
+    st = lib().mdg_dropin_status()
+    if st != OK:
+        lib().mdg_dropin_clear_status()
+        raise MdgError(st, "control_temp")
 
 
 def dropin_call(name):

@@ -4,16 +4,20 @@
  * TEST INFRASTRUCTURE ONLY.  Together with ref_harness.c and mdc's own
  * sources it makes two executables in oracle/_ref:
  *   mdc_cpu  every mdc routine from mdc/program (the reference's CPU path);
- *   mdc_gpu  the same program with real.c, rcprcl.c and rv.c left out and
- *            libmdgpu.so linked in their place, which is the link-time
- *            replacement a maintainer makes to use the GPU force path.
+ *   mdc_gpu  the same program with real.c, rcprcl.c, rv.c, pt.c and msd.c
+ *            left out and libmdgpu.so linked in their place, which is the
+ *            link-time replacement a maintainer makes to use the GPU path.
  * The GPU test runs both on the same input and compares what they print.
  *
- * The step loop is mdc's (mdc/program/main.c:26-33): next_rv_verlet,
- * calc_kin, calc_press; without thermostat and file output.
+ * The step loop is mdc's (mdc/program/main.c:26-63) without file output:
+ * next_rv_verlet, calc_kin, calc_press; then, when asked for, calc_msd,
+ * control_temp(t_step, temp) while step < 10000 and moment_correction
+ * every `moment` steps.
  *
- * usage: mdc_{cpu,gpu} n steps seed    (n x n x n NaCl cells, a = 5.64 A,
- *        alpha = 5.6 / L, hm = 27, cutoff L / 2, dt = 2 fs)
+ * usage: mdc_{cpu,gpu} n steps seed [t_step temp msd moment]
+ *        (n x n x n NaCl cells, a = 5.64 A, alpha = 5.6 / L, hm = 27,
+ *        cutoff L / 2, dt = 2 fs; t_step 0 = no thermostat, msd 0/1,
+ *        moment 0 = no moment correction)
  */
 #include <stdio.h>
 #include <stdlib.h>
@@ -25,6 +29,8 @@ void next_rv_verlet(void);
 void calc_kin(void);
 void calc_press(void);
 void moment_correction(void);
+void calc_msd(void);
+void control_temp(int d_step, double temp);
 int ref_setup(int nx, int ny, int nz, double a, double alpha, int hm, double radius,
               double dt_fs);
 
@@ -38,12 +44,19 @@ static double uniform(void) /* in [0, 1) */
 
 int main(int argc, char **argv)
 {
-  int n, steps, i;
+  int n, steps, i, t_step = 0, msd_on = 0, moment = 0, k;
+  double temp = 0.0;
   double a = 5.64, L, sx = 0, sy = 0, sz = 0;
 
-  if (argc != 4) {
-    fprintf(stderr, "usage: %s n steps seed\n", argv[0]);
+  if (argc != 4 && argc != 8) {
+    fprintf(stderr, "usage: %s n steps seed [t_step temp msd moment]\n", argv[0]);
     return 2;
+  }
+  if (argc == 8) {
+    t_step = atoi(argv[4]);
+    temp = atof(argv[5]);
+    msd_on = atoi(argv[6]);
+    moment = atoi(argv[7]);
   }
   n = atoi(argv[1]);
   steps = atoi(argv[2]);
@@ -71,6 +84,14 @@ int main(int argc, char **argv)
     calc_press();
     printf("%d %.17g %.17g %.17g %.17g %.17g %.17g\n", sys.step, sys.pot, sys.kin,
            sys.virX, sys.virY, sys.virZ, sys.pres);
+    if (msd_on) {
+      calc_msd();
+      printf("msd %d", sys.step);
+      for (k = 0; k < ctl.kinds_of_ions; k++) printf(" %.17g", msd.value[k]);
+      printf("\n");
+    }
+    if (t_step > 0 && sys.step < 10000) control_temp(t_step, temp);
+    if (moment > 0 && sys.step % moment == 0) moment_correction();
   }
   for (i = 0; i < sys.N; i++) {
     sx += sys.rx[i];

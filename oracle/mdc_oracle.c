@@ -452,6 +452,93 @@ void orc_nve(orc_box *b, int steps, double *etot)
   }
 }
 
+/* mdc/program/msd.c:20-81.  At step 1 the positions are recorded and
+ * nothing else changes; afterwards each ion's displacement since the last
+ * call, folded once by the strict half-box tests (msd.c:62-67), is added to
+ * its running sum, and value[k] = mean |sum|^2 over the ions of species k,
+ * summed in ion order. */
+void orc_calc_msd(orc_box *b, orc_observer *o)
+{
+  int i, k;
+  if (b->step == 1) {
+    for (i = 0; i < b->N; i++) {
+      o->rx_old[i] = b->rx[i];
+      o->ry_old[i] = b->ry[i];
+      o->rz_old[i] = b->rz[i];
+    }
+    return;
+  }
+  o->Lx2 = b->Lx / 2.0;
+  o->Ly2 = b->Ly / 2.0;
+  o->Lz2 = b->Lz / 2.0;
+  for (k = 0; k < o->kinds; k++) {
+    o->value[k] = 0.0;
+    o->number_of_ion[k] = 0;
+  }
+  for (i = 0; i < b->N; i++) {
+    double x = b->rx[i], y = b->ry[i], z = b->rz[i];
+    double ex = o->rx_old[i] - x, ey = o->ry_old[i] - y, ez = o->rz_old[i] - z;
+    o->rx_old[i] = x;
+    o->ry_old[i] = y;
+    o->rz_old[i] = z;
+    if (ex > o->Lx2) ex -= b->Lx;
+    if (ex < -o->Lx2) ex += b->Lx;
+    if (ey > o->Ly2) ey -= b->Ly;
+    if (ey < -o->Ly2) ey += b->Ly;
+    if (ez > o->Lz2) ez -= b->Lz;
+    if (ez < -o->Lz2) ez += b->Lz;
+    o->dx[i] += ex;
+    o->dy[i] += ey;
+    o->dz[i] += ez;
+    o->value[b->ion[i]] += o->dx[i] * o->dx[i] + o->dy[i] * o->dy[i] + o->dz[i] * o->dz[i];
+    o->number_of_ion[b->ion[i]]++;
+  }
+  for (k = 0; k < o->kinds; k++) o->value[k] /= (double)o->number_of_ion[k];
+}
+
+/* mdc/program/pt.c:85-112: kin is summed every call; every d_step steps
+ * the velocities are scaled by sqrt((temp / e2t) / mean kin) and the sum
+ * restarts.  Returns 1 when it scaled.  (mdc keeps oldk and count in
+ * statics, count as a short.) */
+int orc_control_temp(orc_box *b, orc_observer *o, int d_step, double temp)
+{
+  int i;
+  double averaged, tempK, scale;
+  o->count++;
+  o->oldk += b->kin;
+  if (b->step % d_step != 0) return 0;
+  averaged = o->oldk / (double)o->count;
+  o->oldk = 0.0;
+  o->count = 0;
+  tempK = temp / b->e2t;
+  scale = sqrt(tempK / averaged);
+  for (i = 0; i < b->N; i++) {
+    b->vx[i] *= scale;
+    b->vy[i] *= scale;
+    b->vz[i] *= scale;
+  }
+  return 1;
+}
+
+/* mdc/program/main.c:26-63 without I/O: next_rv_verlet, calc_kin,
+ * calc_press, then calc_msd (if o->msd_on), control_temp while
+ * step < t_until (if t_control_step > 0), moment_correction every
+ * moment_every steps (if > 0).  Steps first_step .. first_step+steps-1. */
+void orc_newton(orc_box *b, orc_observer *o, int first_step, int steps, int t_control_step,
+                double temp, int t_until, int moment_every)
+{
+  int s;
+  for (s = first_step; s < first_step + steps; s++) {
+    b->step = s;
+    orc_next_rv_verlet(b);
+    orc_calc_kin(b);
+    orc_calc_press(b);
+    if (o->msd_on) orc_calc_msd(b, o);
+    if (t_control_step > 0 && s < t_until) o->scalings += orc_control_temp(b, o, t_control_step, temp);
+    if (moment_every > 0 && s % moment_every == 0) orc_moment_correction(b);
+  }
+}
+
 int orc_count_kvectors(int hm)
 {
   int s = (int)sqrtf((float)hm), hx, hy, hz, n = 0;

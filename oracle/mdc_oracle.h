@@ -25,6 +25,10 @@
  *   orc_moment_correction  rv.c:88-116
  *   orc_calc_press         pt.c:8-21
  *   orc_nve                main.c:26-33 without thermostat and I/O
+ *   orc_calc_msd           msd.c:20-81
+ *   orc_control_temp       pt.c:85-112
+ *   orc_newton             main.c:26-63 without I/O (MSD, thermostat and
+ *                          moment-correction cadence included)
  *
  * Differences from the reference that do not change results on the same
  * inputs: `int` ion counts (mdc uses `short`, which caps N at 32767), an
@@ -54,6 +58,13 @@
  *   reciprocal_space3 compiled mdc (pot bit-equal)
  *   moment_correction compiled mdc (bit-equal); zero total momentum
  *   nve              compiled mdc trajectory; golden fixture
+ *   calc_msd         compiled mdc (msd arrays and values bit-equal over a
+ *                    trajectory); uniform translation across the box edge
+ *                    gives MSD = |n t d|^2 (closed form)
+ *   control_temp     compiled mdc (velocities bit-equal, incl. statics over
+ *                    several periods); after a scaling with constant kin
+ *                    the kinetic energy is the target (closed form)
+ *   newton           compiled mdc loop of main.c:26-63 (no I/O), bit-equal
  */
 #ifndef MDC_ORACLE_H
 #define MDC_ORACLE_H
@@ -106,6 +117,20 @@ typedef struct {
   double  presX, presY, presZ, pres;
 } orc_box;
 
+/* calc_msd's state (mdc/program/md.h:152-172) and control_temp's statics
+ * (pt.c:88-89) for one box; the six arrays are caller-owned [N]. */
+typedef struct {
+  int     kinds;
+  int     msd_on;
+  double *rx_old, *ry_old, *rz_old, *dx, *dy, *dz;
+  double  value[4];
+  int     number_of_ion[4];
+  double  Lx2, Ly2, Lz2;
+  double  oldk;
+  int     count;
+  int     scalings;         /* velocity scalings done (not in mdc) */
+} orc_observer;
+
 double orc_erfcc(double x);
 void   orc_potential_nacl(double kk, orc_potential *pp);
 void   orc_potential_kcl(double kk, orc_potential *pp);
@@ -128,6 +153,11 @@ void   orc_calc_press(orc_box *b);
  * If etot != NULL, etot[s] = kin + pot_real + pot_recip/4 + pot_self after
  * step s (the energy whose gradient the forces are). */
 void   orc_nve(orc_box *b, int steps, double *etot);
+void   orc_calc_msd(orc_box *b, orc_observer *o);
+int    orc_control_temp(orc_box *b, orc_observer *o, int d_step, double temp);
+/* main.c:26-63 without I/O (see mdc_oracle.c) */
+void   orc_newton(orc_box *b, orc_observer *o, int first_step, int steps, int t_control_step,
+                  double temp, int t_until, int moment_every);
 /* number of half-sphere k vectors that reciprocal_space visits */
 int    orc_count_kvectors(int hm);
 

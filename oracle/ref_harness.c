@@ -2,7 +2,7 @@
  * ref_harness.c -- drives the unmodified mdc sources as a shared library.
  *
  * TEST INFRASTRUCTURE ONLY.  oracle/Makefile compiles this file together
- * with mdc/program/{ext,pt,control,init,real,rcprcl,rv}.c exactly as they
+ * with mdc/program/{ext,pt,control,init,real,rcprcl,rv,msd}.c exactly as they
  * lie in the reference checkout (include path pointed there, -fcommon
  * because md.h defines its globals) into oracle/_ref/libmdcref.so.  It is
  * used to pin the oracle restatement bit for bit and to write golden
@@ -12,6 +12,7 @@
  * sets parameters, copies arrays in and out, and sequences calls the way
  * mdc/program/main.c:14-33 does (minus thermostat and I/O).
  */
+#include <stdlib.h>
 #include <string.h>
 #include "md.h"
 #include "prototypes.h"
@@ -122,6 +123,65 @@ void ref_reciprocal_space3(void) { reciprocal_space3(); }
 void ref_calc_kin(void) { calc_kin(); }
 void ref_calc_press(void) { calc_press(); }
 void ref_moment_correction(void) { moment_correction(); }
+
+/* ---- calc_msd (msd.c:20-81) and control_temp (pt.c:85-112) ---- */
+void ref_calc_msd(void) { calc_msd(); }
+void ref_control_temp(int d_step, double temp) { control_temp(d_step, temp); }
+
+/* out: rx_old, ry_old, rz_old, dx, dy, dz (N each); value, number_of_ion
+ * (kinds_of_ions each) */
+void ref_get_msd(double *arrays, double *value, int *number)
+{
+  double *a[6];
+  int k;
+  a[0] = msd.rx_old; a[1] = msd.ry_old; a[2] = msd.rz_old;
+  a[3] = msd.dx;     a[4] = msd.dy;     a[5] = msd.dz;
+  for (k = 0; k < 6; k++) memcpy(arrays + (size_t)k * sys.N, a[k], sizeof(double) * sys.N);
+  for (k = 0; k < ctl.kinds_of_ions; k++) {
+    value[k] = msd.value[k];
+    number[k] = msd.number_of_ion[k];
+  }
+}
+
+/* control_temp keeps its running sums in statics; a call at a multiple of
+ * d_step restarts them.  Make one on a copy of the velocities and restore
+ * them, so each test starts from mdc's initial (zero) statics. */
+void ref_reset_control_temp(void)
+{
+  int n = sys.N, st = sys.step;
+  double kin = sys.kin;
+  double *v = (double *)malloc(sizeof(double) * 3 * n);
+  memcpy(v, sys.vx, sizeof(double) * n);
+  memcpy(v + n, sys.vy, sizeof(double) * n);
+  memcpy(v + 2 * n, sys.vz, sizeof(double) * n);
+  sys.step = 1;
+  sys.kin = 1.0;
+  control_temp(1, 1.0);
+  memcpy(sys.vx, v, sizeof(double) * n);
+  memcpy(sys.vy, v + n, sizeof(double) * n);
+  memcpy(sys.vz, v + 2 * n, sizeof(double) * n);
+  free(v);
+  sys.kin = kin;
+  sys.step = st;
+}
+
+/* mdc/program/main.c:26-63 without I/O: next_rv_verlet, calc_kin,
+ * calc_press, calc_msd (msd_on), control_temp while step < t_until
+ * (t_control_step > 0), moment_correction every moment_every steps */
+void ref_newton(int first_step, int nsteps, int t_control_step, double temp, int t_until,
+                int moment_every, int msd_on)
+{
+  int s;
+  for (s = 0; s < nsteps; s++) {
+    sys.step = first_step + s;
+    next_rv_verlet();
+    calc_kin();
+    calc_press();
+    if (msd_on) calc_msd();
+    if (t_control_step > 0 && sys.step < t_until) control_temp(t_control_step, temp);
+    if (moment_every > 0 && sys.step % moment_every == 0) moment_correction();
+  }
+}
 
 /* steps of mdc/program/main.c:26-33: next_rv_verlet, calc_kin, calc_press */
 void ref_run(int first_step, int nsteps)

@@ -54,5 +54,12 @@ int main(void)
   F(huggins_mayer_potential_set, C_na_cl); F(huggins_mayer_potential_set, C_cl_cl);
   F(huggins_mayer_potential_set, D_na_na); F(huggins_mayer_potential_set, D_na_cl);
   F(huggins_mayer_potential_set, D_cl_cl);
+  printf("msd_calculation_variables %zu\n", sizeof(msd_calculation_variables));
+  F(msd_calculation_variables, number_of_ion); F(msd_calculation_variables, rx_old);
+  F(msd_calculation_variables, ry_old); F(msd_calculation_variables, rz_old);
+  F(msd_calculation_variables, dx); F(msd_calculation_variables, dy);
+  F(msd_calculation_variables, dz); F(msd_calculation_variables, value);
+  F(msd_calculation_variables, Lx2); F(msd_calculation_variables, Ly2);
+  F(msd_calculation_variables, Lz2);
   return 0;
 }

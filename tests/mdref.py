@@ -54,6 +54,14 @@ class OrcBox(C.Structure):
     ]
 
 
+class OrcObserver(C.Structure):
+    _fields_ = [("kinds", C.c_int), ("msd_on", C.c_int)] + \
+        [(n, _dp) for n in ("rx_old", "ry_old", "rz_old", "dx", "dy", "dz")] + \
+        [("value", C.c_double * 4), ("number_of_ion", C.c_int * 4),
+         ("Lx2", C.c_double), ("Ly2", C.c_double), ("Lz2", C.c_double),
+         ("oldk", C.c_double), ("count", C.c_int), ("scalings", C.c_int)]
+
+
 _lib = None
 
 
@@ -79,6 +87,14 @@ def lib():
         L.orc_potential_kcl.argtypes = [C.c_double, C.POINTER(OrcPotential)]
         L.orc_count_kvectors.argtypes = [C.c_int]
         L.orc_count_kvectors.restype = C.c_int
+        op = C.POINTER(OrcObserver)
+        L.orc_calc_msd.argtypes = [bp, op]
+        L.orc_calc_msd.restype = None
+        L.orc_control_temp.argtypes = [bp, op, C.c_int, C.c_double]
+        L.orc_control_temp.restype = C.c_int
+        L.orc_newton.argtypes = [bp, op, C.c_int, C.c_int, C.c_int, C.c_double, C.c_int,
+                                 C.c_int]
+        L.orc_newton.restype = None
         _lib = L
     return _lib
 
@@ -125,6 +141,14 @@ class Oracle:
         L.orc_set_loc(C.byref(b))
         L.orc_mk_table(C.byref(b))
         b.step = 1
+        # calc_msd's arrays and control_temp's statics (zero, as mdc's calloc)
+        self.msd_old = np.zeros((3, n))
+        self.msd_d = np.zeros((3, n))
+        self.obs = OrcObserver()
+        o = self.obs
+        o.kinds = 2
+        o.rx_old, o.ry_old, o.rz_old = (_ptr(self.msd_old[k]) for k in range(3))
+        o.dx, o.dy, o.dz = (_ptr(self.msd_d[k]) for k in range(3))
 
     @property
     def N(self):
@@ -152,6 +176,22 @@ class Oracle:
         e = np.zeros(steps) if energies else None
         lib().orc_nve(C.byref(self.b), steps, _ptr(e) if energies else None)
         return e
+
+    def calc_msd(self):
+        lib().orc_calc_msd(C.byref(self.b), C.byref(self.obs))
+
+    def control_temp(self, d_step, temp):
+        return lib().orc_control_temp(C.byref(self.b), C.byref(self.obs), d_step, temp)
+
+    def newton(self, first_step, steps, t_control_step=0, temp=0.0, t_until=10000,
+               moment_every=0, msd=False):
+        """main.c:26-63 without I/O, steps first_step .. first_step+steps-1."""
+        self.obs.msd_on = 1 if msd else 0
+        lib().orc_newton(C.byref(self.b), C.byref(self.obs), first_step, steps,
+                         t_control_step, temp, t_until, moment_every)
+
+    def msd_values(self):
+        return np.array(self.obs.value[:2])
 
     def scalars(self):
         b = self.b
@@ -210,6 +250,10 @@ def ref():
         R.ref_erfcc.argtypes = [C.c_double]
         R.ref_erfcc.restype = C.c_double
         R.ref_run.argtypes = [C.c_int, C.c_int]
+        R.ref_control_temp.argtypes = [C.c_int, C.c_double]
+        R.ref_get_msd.argtypes = [_dp, _dp, _ip]
+        R.ref_newton.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, C.c_int, C.c_int,
+                                 C.c_int]
         _ref = R
     return _ref
 
@@ -261,3 +305,23 @@ class Reference:
 
     def run(self, first_step, nsteps):
         ref().ref_run(first_step, nsteps)
+
+    def newton(self, first_step, nsteps, t_control_step=0, temp=0.0, t_until=10000,
+               moment_every=0, msd=False):
+        ref().ref_newton(first_step, nsteps, t_control_step, temp, t_until, moment_every,
+                         1 if msd else 0)
+
+    def control_temp(self, d_step, temp):
+        ref().ref_control_temp(d_step, temp)
+
+    def reset_control_temp(self):
+        ref().ref_reset_control_temp()
+
+    def msd(self):
+        """(old positions 3xN, summed displacements 3xN, value[2], number[2])"""
+        arr = np.zeros(6 * self.N)
+        val = np.zeros(4)
+        num = np.zeros(4, dtype=np.int32)
+        ref().ref_get_msd(_ptr(arr), _ptr(val), _ptr(num, C.c_int))
+        arr = arr.reshape(6, self.N)
+        return arr[:3], arr[3:], val[:2], num[:2]

@@ -60,7 +60,8 @@ def test_ctypes_mirror_matches_header_layout(tmp_path):
     for cname, cls in (("system_property", mdgpu.SystemProperty),
                        ("calc_control", mdgpu.CalcControl),
                        ("ion_property", mdgpu.IonProperty),
-                       ("huggins_mayer_potential_set", mdgpu.HugginsMayer)):
+                       ("huggins_mayer_potential_set", mdgpu.HugginsMayer),
+                       ("msd_calculation_variables", mdgpu.MsdVariables)):
         assert int(lines[cname]) == C.sizeof(cls), cname
         for f, _ in cls._fields_:
             assert int(lines[f"{cname}.{f}"]) == getattr(cls, f).offset, (cname, f)

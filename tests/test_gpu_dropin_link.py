@@ -23,26 +23,42 @@ REF = os.path.join(ROOT, "oracle", "_ref")
 PROGS = {k: os.path.join(REF, k) for k in ("mdc_cpu", "mdc_gpu")}
 
 
-def run(prog, n, steps, seed):
-    p = subprocess.run([PROGS[prog], str(n), str(steps), str(seed)], capture_output=True,
+def run(prog, n, steps, seed, loop=()):
+    p = subprocess.run([PROGS[prog], str(n), str(steps), str(seed), *loop], capture_output=True,
                        text=True, timeout=120)
     assert p.returncode == 0, (prog, p.returncode, p.stderr[-2000:])
-    rows, final = [], None
+    rows, msd, final, notes = [], [], None, 0
     for ln in p.stdout.splitlines():
         f = ln.split()
+        if not f:
+            continue
         if f[0] == "final":
             final = np.array([float(x) for x in f[1:]])
-        else:
+        elif f[0] == "msd":
+            msd.append([float(x) for x in f[1:]])
+        elif f[0].isdigit():
             rows.append([float(x) for x in f[1:]])
-    return np.array(rows), final, p.stderr
+        else:   # control_temp's own report lines (pt.c:107-108)
+            notes += 1
+    return np.array(rows), final, p.stderr, np.array(msd), notes
 
 
 @pytest.mark.skipif(not all(os.path.exists(p) for p in PROGS.values()),
                     reason="oracle/_ref drop-in programs not built (no reference checkout)")
-@pytest.mark.parametrize("n,steps", [(3, 20), (4, 10)])
-def test_mdc_program_linked_to_libmdgpu_matches_mdc(gpu, n, steps):
-    cpu, fcpu, _ = run("mdc_cpu", n, steps, 11)
-    dev, fdev, err = run("mdc_gpu", n, steps, 11)
+@pytest.mark.parametrize("n,steps,loop", [(3, 20, ()), (4, 10, ()),
+                                          (3, 30, ("5", "1500", "1", "10"))])
+def test_mdc_program_linked_to_libmdgpu_matches_mdc(gpu, n, steps, loop):
+    """loop: the rest of main.c's loop -- control_temp every 5 steps to
+    1500 K, calc_msd every step, moment_correction every 10 steps."""
+    cpu, fcpu, _, mcpu, ncpu = run("mdc_cpu", n, steps, 11, loop)
+    dev, fdev, err, mdev, ndev = run("mdc_gpu", n, steps, 11, loop)
+    if loop:
+        assert ndev == ncpu == 2 * 6    # scalings at steps 5, 10, ..., 30
+        assert mcpu.shape == mdev.shape == (steps, 3)
+        assert np.array_equal(mdev[:, 0], mcpu[:, 0])
+        # the MSD is a small difference of positions: the positions'
+        # rounding-level differences show in it relatively magnified
+        assert np.allclose(mdev[1:, 1:], mcpu[1:, 1:], rtol=1e-9, atol=0)
     assert "libmdgpu" not in err, err        # the drop-in reports failures on stderr
     assert cpu.shape == dev.shape == (steps, 6)
     # pot, kin, virX, virY, virZ, pres per step
