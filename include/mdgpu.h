@@ -157,6 +157,60 @@ int  mdg_batch_observed(mdg_batch *b, int box, double *msd, long long *scalings)
 /* MSD state: old = positions of the last sample (msd.rx_old, ry_old,
  * rz_old), d = summed displacements (msd.dx, dy, dz); 3N SoA each, or NULL */
 int  mdg_batch_msd_state(mdg_batch *b, int box, double *old, double *d);
+
+/* ------------------------------------------------------- state read-back */
+/* One box's state at a recorded step, in pinned host memory that is valid
+ * for the duration of the callback. */
+typedef struct {
+  int                box, n_ions;
+  long long          step;         /* the box's completed steps          */
+  const double      *r, *v;        /* 3N SoA each: x block, y, z [A], [A/time unit] */
+  const int         *species;      /* N                                   */
+  mdg_scalars        scalars;      /* as mdg_batch_scalars                */
+  double             msd[4];       /* as mdg_batch_observed (0 unarmed)   */
+} mdg_frame;
+typedef void (*mdg_frame_fn)(const mdg_frame *f, void *user);
+/* Steps every box nsteps (a multiple of every) and after each `every`
+ * steps hands every box's state to fn on the calling thread, while the
+ * device goes on stepping: at that point of the step stream one kernel
+ * gathers all boxes' r, v, scalars and MSD into a device staging slot,
+ * a second stream copies the slot to pinned host memory, and frame k is
+ * delivered while the steps of frame k + 1 run (two slots).  fn may be
+ * NULL (copy only).  The stepping itself is mdg_batch_step's, so the
+ * trajectory is the same bits as without recording. */
+int  mdg_batch_step_recorded(mdg_batch *b, int nsteps, int every, mdg_frame_fn fn, void *user);
+
+/* mdc's output files (mdc/program/files.c), written from the device-resident
+ * boxes through mdg_batch_step_recorded: per box, dir/box<k>/out with one
+ * print_to_file line per frame (files.c:44-65), and, by the bits of
+ * `what`, dir/box<k>/positions (MDG_WRITE_POSITIONS, r_position,
+ * files.c:142-158) and velocities (MDG_WRITE_VELOCITIES, r_velocities,
+ * files.c:161-181).  Each file starts with mdc's single blank
+ * (files.c:29-32).  dir/box<k> must exist. */
+#define MDG_WRITE_POSITIONS  1
+#define MDG_WRITE_VELOCITIES 2
+int  mdg_batch_write_mdc_files(mdg_batch *b, int nsteps, int every, int what, const char *dir);
+
+/* The text of mdc's formats, snprintf-like: writes at most cap bytes
+ * (NUL-terminated when cap > 0) and returns the length the whole text
+ * needs, or a negative MDG_E_* code.
+ *   mdg_format_out:        one print_to_file line (files.c:44-65), with
+ *                          perMol, e2t and pp2gpa derived from N as
+ *                          unit_converter does (init.c:48, 75, 78);
+ *   mdg_format_positions:  one r_position record (files.c:142-158);
+ *   mdg_format_velocities: one r_velocities record (files.c:161-181). */
+typedef struct {
+  int    step;
+  double kin, pot, pres;        /* sys.kin, sys.pot, sys.pres          */
+  int    n_ions, natoms_in_mol_unit;
+  double L[3];                  /* sys.Lx, Ly, Lz                      */
+  int    cells[3];              /* sys.nx, ny, nz                      */
+  int    kinds;                 /* ctl.kinds_of_ions (<= 4)            */
+  double msd[4];                /* msd.value                           */
+} mdg_out_fields;
+int  mdg_format_out(char *buf, size_t cap, const mdg_out_fields *f);
+int  mdg_format_positions(char *buf, size_t cap, int n, const int *species, const double *r);
+int  mdg_format_velocities(char *buf, size_t cap, int n, const int *species, const double *v);
 /* Kernel timing over the next `nsteps` steps, launched eagerly in the
  * step's own schedule (box groups on several streams); every launch
  * carries a start and a stop hipEvent stamped by its own dispatch

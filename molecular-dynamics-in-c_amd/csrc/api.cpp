@@ -5,6 +5,7 @@
 #include <string>
 #include <vector>
 
+#include "batch.h"
 #include "engine.h"
 #include "host_setup.h"
 #include "launch.h"
@@ -17,13 +18,6 @@ static const double kSqrtPi = 1.77245385090551602731;
 static const double kPiPi = 9.869604401089358619;
 static const double kSecd2Fs = 4.074972637944947;
 
-struct mdg_batch {
-  int precision = MDG_FP64;
-  int real_mode = MDG_TABLE;
-  std::vector<Box*> boxes;
-  BoxSet* set = nullptr;
-  hipStream_t stream = nullptr;
-};
 
 namespace mdg {
 
@@ -181,6 +175,7 @@ int mdg_batch_create(int nbox, const mdg_box_params* params, int precision, mdg_
       return rc;
     }
     b->boxes.push_back(box);
+    b->cells.push_back({params[k].nx, params[k].ny, params[k].nz});
     // start from mdc's lattice at rest
     const int N = spec.N;
     std::vector<double> r(3 * (size_t)N);
@@ -298,18 +293,7 @@ int mdg_batch_scalars(mdg_batch* b, int box, mdg_scalars* out) {
   int e = 0;
   rc = box_read_error(b->boxes[box], &e, b->stream);
   if (rc != MDG_OK) return rc;
-  out->pot = S[kScalPot];
-  out->pot_real = S[kScalPotReal];
-  out->pot_recip = S[kScalPotRecip];
-  out->pot_self = S[kScalPotSelf];
-  out->etot = S[kScalEtot];
-  out->vir[0] = S[kScalVirX]; out->vir[1] = S[kScalVirY]; out->vir[2] = S[kScalVirZ];
-  out->kin = S[kScalKin];
-  out->kin_xyz[0] = S[kScalKinX]; out->kin_xyz[1] = S[kScalKinY]; out->kin_xyz[2] = S[kScalKinZ];
-  out->pres = S[kScalPres];
-  out->pres_xyz[0] = S[kScalPresX]; out->pres_xyz[1] = S[kScalPresY];
-  out->pres_xyz[2] = S[kScalPresZ];
-  out->step = (long long)S[kScalStep];
+  fill_scalars(S, out);
   return err_code(e);
 }
 

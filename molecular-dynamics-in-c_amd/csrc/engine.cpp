@@ -1113,6 +1113,9 @@ void BoxSet::enqueue_moment_correction(hipStream_t s) {
 void BoxSet::enqueue_observe(hipStream_t s, long long step, int phases) {
   launch_observe(d_desc_, d_ids_ + all_.ioff[IAll], all_.in[IAll], step, phases, s);
 }
+void BoxSet::enqueue_snapshot(hipStream_t s, const long long* off, double* dst) {
+  launch_snapshot(d_desc_, d_items_ + all_.off[LIon], all_.n[LIon], off, dst, s);
+}
 void BoxSet::enqueue_scale_v(hipStream_t s, double scale) {
   launch_scale_v(d_desc_, d_items_ + all_.off[LIon], all_.n[LIon], scale, s);
 }

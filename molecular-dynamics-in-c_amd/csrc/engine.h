@@ -152,6 +152,8 @@ class BoxSet {
   // k_observe on every box (step >= 0: mdc's sys.step; see launch_observe)
   void enqueue_observe(hipStream_t s, long long step, int phases);
   void enqueue_scale_v(hipStream_t s, double scale);
+  // launch_snapshot over every box (off: device array, one entry per box)
+  void enqueue_snapshot(hipStream_t s, const long long* off, double* dst);
   int step(int nsteps, hipStream_t s);          // graph replay
   // Kernel timing over `nsteps` eager steps: every launch carries a start
   // and a stop hipEvent that its own dispatch stamps.  as_step: the step's own schedule

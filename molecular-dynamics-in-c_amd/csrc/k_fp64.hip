@@ -98,6 +98,11 @@ void launch_scale_v(const BoxDesc* boxes, const Item* items, int n, double scale
   if (n > 0) MDG_LAUNCH(k_scale_v, MDG_GRID(n, kIonThreads), boxes, items, scale);
 }
 
+void launch_snapshot(const BoxDesc* boxes, const Item* items, int n, const long long* off,
+                     double* dst, hipStream_t s) {
+  if (n > 0) MDG_LAUNCH(k_snapshot, MDG_GRID(n, kIonThreads), boxes, items, off, dst);
+}
+
 void launch_observe(const BoxDesc* boxes, const int* ids, int n, long long step, int phases,
                     hipStream_t s) {
   if (n > 0) MDG_LAUNCH(k_observe, MDG_GRID(n, kScalThreads), boxes, ids, step, phases);

@@ -365,6 +365,34 @@ __global__ __launch_bounds__(kScalThreads) void k_box_scalars(const BoxDesc* __r
   }
 }
 
+// Copies each box's r, v, scalars and observables into one staging
+// buffer, box b at dst + off[b] as [r 3N][v 3N][scalars][observables]
+// (one launch over the per-ion items, so a frame of many boxes is one
+// contiguous device-to-host copy).
+__global__ __launch_bounds__(kIonThreads) void k_snapshot(const BoxDesc* __restrict__ boxes,
+                                                          const Item* __restrict__ items,
+                                                          const long long* __restrict__ off,
+                                                          double* __restrict__ dst) {
+  const Item it = items[blockIdx.x];
+  const BoxDesc& B = boxes[it.box];
+  const int N = B.N;
+  double* o = dst + off[it.box];
+  const int i = it.a + threadIdx.x;
+  if (i < N) {
+    o[i] = B.rx[i];
+    o[N + i] = B.ry[i];
+    o[2 * N + i] = B.rz[i];
+    o[3 * N + i] = B.vx[i];
+    o[4 * N + i] = B.vy[i];
+    o[5 * N + i] = B.vz[i];
+  }
+  if (it.a == 0) {
+    const int t = threadIdx.x;
+    if (t < kScalCount) o[6 * N + t] = B.scal[t];
+    if (t < kObsLen) o[6 * N + kScalCount + t] = B.obs[t];
+  }
+}
+
 // v <- scale * v (the velocity scaling of control_temp, pt.c:101-105)
 __global__ __launch_bounds__(kIonThreads) void k_scale_v(const BoxDesc* __restrict__ boxes,
                                                          const Item* __restrict__ items,

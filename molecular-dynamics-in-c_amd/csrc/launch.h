@@ -36,6 +36,10 @@ void launch_next_v(const BoxDesc* boxes, const Item* items, int n, hipStream_t s
 void launch_next_r(const BoxDesc* boxes, const Item* items, int n, hipStream_t s);
 void launch_box_scalars(const BoxDesc* boxes, const int* ids, int n, int flags, hipStream_t s);
 void launch_moment_correction(const BoxDesc* boxes, const int* ids, int n, hipStream_t s);
+// r, v, scalars and observables of every box into one staging buffer
+// (box b at dst + off[b]: [r 3N][v 3N][kScalCount][kObsLen] doubles)
+void launch_snapshot(const BoxDesc* boxes, const Item* items, int n, const long long* off,
+                     double* dst, hipStream_t s);
 void launch_scale_v(const BoxDesc* boxes, const Item* items, int n, double scale,
                     hipStream_t s);
 // mdc's calc_msd, control_temp and moment_correction after a step

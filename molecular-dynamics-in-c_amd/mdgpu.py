@@ -152,6 +152,24 @@ class Observe(C.Structure):
                 ("t_control_until", C.c_longlong), ("moment_every", C.c_int)]
 
 
+class OutFields(C.Structure):
+    """mdg_out_fields: the values of one line of mdc's files/out."""
+    _fields_ = [("step", C.c_int), ("kin", C.c_double), ("pot", C.c_double),
+                ("pres", C.c_double), ("n_ions", C.c_int), ("natoms_in_mol_unit", C.c_int),
+                ("L", C.c_double * 3), ("cells", C.c_int * 3), ("kinds", C.c_int),
+                ("msd", C.c_double * 4)]
+
+
+class Frame(C.Structure):
+    _fields_ = [("box", C.c_int), ("n_ions", C.c_int), ("step", C.c_longlong),
+                ("r", _dp), ("v", _dp), ("species", C.POINTER(C.c_int)),
+                ("scalars", Scalars), ("msd", C.c_double * 4)]
+
+
+FRAME_FN = C.CFUNCTYPE(None, C.POINTER(Frame), C.c_void_p)
+WRITE_POSITIONS, WRITE_VELOCITIES = 1, 2
+
+
 DROPIN_ROUTINES = ("calc_foc", "clear_foc", "real_space", "reciprocal_space",
                    "reciprocal_space3", "next_rv_verlet", "next_r", "next_v", "calc_kin",
                    "moment_correction", "calc_press", "mk_table")
@@ -214,6 +232,13 @@ def lib():
             "mdg_batch_observed": ([vp, C.c_int, _dp, C.POINTER(C.c_longlong)], C.c_int),
             "mdg_batch_msd_state": ([vp, C.c_int, _dp, _dp], C.c_int),
             "calc_msd": ([], None),
+            "mdg_batch_step_recorded": ([vp, C.c_int, C.c_int, FRAME_FN, C.c_void_p], C.c_int),
+            "mdg_batch_write_mdc_files": ([vp, C.c_int, C.c_int, C.c_int, C.c_char_p], C.c_int),
+            "mdg_format_out": ([C.c_char_p, C.c_size_t, C.POINTER(OutFields)], C.c_int),
+            "mdg_format_positions": ([C.c_char_p, C.c_size_t, C.c_int, C.POINTER(C.c_int),
+                                      _dp], C.c_int),
+            "mdg_format_velocities": ([C.c_char_p, C.c_size_t, C.c_int, C.POINTER(C.c_int),
+                                       _dp], C.c_int),
             "control_temp": ([C.c_int, C.c_double], None),
         }
         for name, (args, res) in sig.items():
@@ -377,6 +402,30 @@ class Batch:
         check(lib().mdg_batch_msd_state(self.h, b, _p(old), _p(d)), "msd_state")
         return old.reshape(3, n), d.reshape(3, n)
 
+    def step_recorded(self, nsteps, every, keep=True):
+        """Step nsteps, reading every box back after each `every` steps while
+        the device goes on stepping.  keep: return the frames as a list of
+        dicts (copies); else the frames are copied to the host and dropped."""
+        frames = []
+
+        def cb(fp, _user):
+            f = fp.contents
+            n = f.n_ions
+            frames.append(dict(box=f.box, step=f.step,
+                               r=np.ctypeslib.as_array(f.r, (3 * n,)).reshape(3, n).copy(),
+                               v=np.ctypeslib.as_array(f.v, (3 * n,)).reshape(3, n).copy(),
+                               scalars=f.scalars.as_dict(), msd=np.array(f.msd[:])))
+
+        fn = FRAME_FN(cb) if keep else C.cast(None, FRAME_FN)
+        check(lib().mdg_batch_step_recorded(self.h, nsteps, every, fn, None), "step_recorded")
+        return frames
+
+    def write_mdc_files(self, nsteps, every, what, directory):
+        """mdc's files/out (and positions / velocities by `what`) per box
+        under directory/box<k>/, one record per `every` steps."""
+        check(lib().mdg_batch_write_mdc_files(self.h, nsteps, every, what,
+                                              os.fsencode(directory)), "write_mdc_files")
+
     def work(self, b=0):
         p, k, n = C.c_longlong(), C.c_int(), C.c_int()
         check(lib().mdg_batch_work(self.h, b, C.byref(p), C.byref(k), C.byref(n)), "work")
@@ -442,6 +491,36 @@ def serve(jobs, cfg):
     st = ServeStats()
     check(lib().mdg_serve(arr, len(jobs), C.byref(cfg), res, C.byref(st)), "serve")
     return list(res), st
+
+
+def _text(fn, *args):
+    n = fn(None, 0, *args)
+    if n < 0:
+        raise MdgError(n, "format")
+    buf = C.create_string_buffer(n + 1)
+    fn(buf, n + 1, *args)
+    return buf.raw[:n].decode()
+
+
+def format_out(step, kin, pot, pres, n_ions, L, cells, msd=(0.0, 0.0), natoms_in_mol_unit=2):
+    """One line of mdc's files/out (files.c:44-65)."""
+    f = OutFields(step, kin, pot, pres, n_ions, natoms_in_mol_unit, (C.c_double * 3)(*L),
+                  (C.c_int * 3)(*cells), len(msd), (C.c_double * 4)(*msd))
+    return _text(lib().mdg_format_out, C.byref(f))
+
+
+def format_positions(species, r):
+    """One r_position record (files.c:142-158); r: 3 x N."""
+    sp = np.ascontiguousarray(species, dtype=np.int32)
+    r = np.ascontiguousarray(r, dtype=np.float64)
+    return _text(lib().mdg_format_positions, sp.size, _ip(sp), _p(r))
+
+
+def format_velocities(species, v):
+    """One r_velocities record (files.c:161-181); v: 3 x N [A / time unit]."""
+    sp = np.ascontiguousarray(species, dtype=np.int32)
+    v = np.ascontiguousarray(v, dtype=np.float64)
+    return _text(lib().mdg_format_velocities, sp.size, _ip(sp), _p(v))
 
 
 # ---- the drop-in surface -------------------------------------------------

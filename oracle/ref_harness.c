@@ -2,7 +2,7 @@
  * ref_harness.c -- drives the unmodified mdc sources as a shared library.
  *
  * TEST INFRASTRUCTURE ONLY.  oracle/Makefile compiles this file together
- * with mdc/program/{ext,pt,control,init,real,rcprcl,rv,msd}.c exactly as they
+ * with mdc/program/{ext,pt,control,init,real,rcprcl,rv,msd,files}.c exactly as they
  * lie in the reference checkout (include path pointed there, -fcommon
  * because md.h defines its globals) into oracle/_ref/libmdcref.so.  It is
  * used to pin the oracle restatement bit for bit and to write golden
@@ -182,6 +182,20 @@ void ref_newton(int first_step, int nsteps, int t_control_step, double temp, int
     if (moment_every > 0 && sys.step % moment_every == 0) moment_correction();
   }
 }
+
+/* ---- files.c's writers, pointed at `path` (mdc opens ../files/...) ---- */
+static int to_file(FILE **fp, const char *path, void (*writer)(void))
+{
+  *fp = fopen(path, "w");
+  if (!*fp) return -1;
+  writer();
+  fclose(*fp);
+  *fp = NULL;
+  return 0;
+}
+int ref_print_to_file(const char *path) { return to_file(&fpout, path, print_to_file); }
+int ref_r_position(const char *path) { return to_file(&fpout_positions, path, r_position); }
+int ref_r_velocities(const char *path) { return to_file(&fpout_velocities, path, r_velocities); }
 
 /* steps of mdc/program/main.c:26-33: next_rv_verlet, calc_kin, calc_press */
 void ref_run(int first_step, int nsteps)

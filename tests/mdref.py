@@ -254,6 +254,9 @@ def ref():
         R.ref_get_msd.argtypes = [_dp, _dp, _ip]
         R.ref_newton.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, C.c_int, C.c_int,
                                  C.c_int]
+        for n in ("ref_print_to_file", "ref_r_position", "ref_r_velocities"):
+            getattr(R, n).argtypes = [C.c_char_p]
+            getattr(R, n).restype = C.c_int
         _ref = R
     return _ref
 
@@ -316,6 +319,11 @@ class Reference:
 
     def reset_control_temp(self):
         ref().ref_reset_control_temp()
+
+    def write(self, which, path):
+        """files.c's print_to_file / r_position / r_velocities into path."""
+        assert getattr(ref(), "ref_" + which)(os.fsencode(str(path))) == 0
+        return open(path).read()
 
     def msd(self):
         """(old positions 3xN, summed displacements 3xN, value[2], number[2])"""
