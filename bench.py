@@ -334,6 +334,17 @@ def main():
                           "moment_correction every 1000 steps"}
     log(f"  with mdc's loop armed: {loop_ms:.3f} ms/step "
         f"(observe kernel {step_loop['observe_ms_per_step']:.4f} ms)")
+    # every box's r, v, scalars and MSD read back to pinned host memory
+    # every 10 steps, overlapped with the stepping (the data behind mdc's
+    # files/out, positions and velocities)
+    nrec = max(10, args.steps // 10 * 10)
+    b.sync()
+    t0 = time.perf_counter()
+    b.step_recorded(nrec, 10, keep=False)
+    rec_ms = (time.perf_counter() - t0) * 1e3 / nrec
+    step_loop["recorded_every_10_steps_ms_per_step"] = rec_ms
+    step_loop["frame_bytes"] = 48 * total_ions
+    log(f"  recorded every 10 steps: {rec_ms:.3f} ms/step ({48 * total_ions / 1e6:.1f} MB per frame)")
     b.close()
 
     kn = sum(w["nk"] * w["n_ions"] for w in work)
