@@ -14,10 +14,11 @@
  * control_temp(t_step, temp) while step < 10000 and moment_correction
  * every `moment` steps.
  *
- * usage: mdc_{cpu,gpu} n steps seed [t_step temp msd moment]
- *        (n x n x n NaCl cells, a = 5.64 A, alpha = 5.6 / L, hm = 27,
- *        cutoff L / 2, dt = 2 fs; t_step 0 = no thermostat, msd 0/1,
- *        moment 0 = no moment correction)
+ * usage: mdc_{cpu,gpu}[_material] n steps seed [t_step temp msd moment [a]]
+ *        (n x n x n cells of the material that control.c sets up, lattice
+ *        constant a (default 5.64 A), alpha = 5.6 / L, hm = 27, cutoff
+ *        L / 2, dt = 2 fs; t_step 0 = no thermostat, msd 0/1, moment 0 =
+ *        no moment correction)
  */
 #include <stdio.h>
 #include <stdlib.h>
@@ -48,11 +49,12 @@ int main(int argc, char **argv)
   double temp = 0.0;
   double a = 5.64, L, sx = 0, sy = 0, sz = 0;
 
-  if (argc != 4 && argc != 8) {
-    fprintf(stderr, "usage: %s n steps seed [t_step temp msd moment]\n", argv[0]);
+  if (argc != 4 && argc != 8 && argc != 9) {
+    fprintf(stderr, "usage: %s n steps seed [t_step temp msd moment [a]]\n", argv[0]);
     return 2;
   }
-  if (argc == 8) {
+  if (argc == 9) a = atof(argv[8]);
+  if (argc >= 8) {
     t_step = atoi(argv[4]);
     temp = atof(argv[5]);
     msd_on = atoi(argv[6]);

@@ -24,7 +24,8 @@ PROGS = {k: os.path.join(REF, k) for k in ("mdc_cpu", "mdc_gpu")}
 
 
 def run(prog, n, steps, seed, loop=()):
-    p = subprocess.run([PROGS[prog], str(n), str(steps), str(seed), *loop], capture_output=True,
+    path = PROGS.get(prog, os.path.join(REF, prog))
+    p = subprocess.run([path, str(n), str(steps), str(seed), *loop], capture_output=True,
                        text=True, timeout=120)
     assert p.returncode == 0, (prog, p.returncode, p.stderr[-2000:])
     rows, msd, final, notes = [], [], None, 0
@@ -66,3 +67,27 @@ def test_mdc_program_linked_to_libmdgpu_matches_mdc(gpu, n, steps, loop):
         scale = np.abs(cpu[:, c]).max()
         assert np.abs(dev[:, c] - cpu[:, c]).max() <= 1e-12 * scale, c
     assert np.allclose(fdev, fcpu, rtol=1e-12, atol=0)
+
+
+# mdc's other tabulated ionic materials (mdc/examples/<m>: control.c,
+# potential.h, real.c replacing program/'s), lattice constants from each
+# example's get_control_param: the same drop-in, other tables, charges,
+# lattices and ion counts (NaCl SX1 216, AgI 108, CaF2 324 ions)
+MATERIALS = {"NaCl_sx1": 5.63, "AgI": 5.08, "CaF2": 5.9}
+
+
+@pytest.mark.parametrize("mat", sorted(MATERIALS))
+def test_other_tabulated_materials_match_mdc(gpu, mat):
+    progs = [os.path.join(REF, f"mdc_{k}_{mat}") for k in ("cpu", "gpu")]
+    if not all(os.path.exists(p) for p in progs):
+        pytest.skip("oracle/_ref material programs not built (no reference checkout)")
+    loop = ("0", "0", "0", "0", str(MATERIALS[mat]))
+    cpu, fcpu, _, _, _ = run(f"mdc_cpu_{mat}", 3, 10, 5, loop)
+    dev, fdev, err, _, _ = run(f"mdc_gpu_{mat}", 3, 10, 5, loop)
+    assert "libmdgpu" not in err, err
+    assert cpu.shape == dev.shape == (10, 6)
+    for c in range(6):
+        scale = np.abs(cpu[:, c]).max()
+        assert np.abs(dev[:, c] - cpu[:, c]).max() <= 1e-12 * scale, (mat, c)
+    assert np.allclose(fdev, fcpu, rtol=1e-12, atol=0)
+
