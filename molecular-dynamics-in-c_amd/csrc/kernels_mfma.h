@@ -34,6 +34,15 @@ typedef float f16_t __attribute__((ext_vector_type(16)));
 typedef unsigned int u4_t __attribute__((ext_vector_type(4)));
 typedef float f4v_t __attribute__((ext_vector_type(4)));
 typedef float f2v_t __attribute__((ext_vector_type(2)));
+typedef _Float16 h2_t __attribute__((ext_vector_type(2)));
+
+// (re, im) as FP16 hi and lo pairs, hi = fp16(q), lo = fp16(q - hi): one
+// packed conversion per pair and two exact FP32 residuals
+__device__ __forceinline__ void split_f16x2(f2v_t q, h2_t& hi, h2_t& lo) {
+  hi = __builtin_convertvector(q, h2_t);
+  const f2v_t back = __builtin_convertvector(hi, f2v_t);
+  lo = __builtin_convertvector(f2v_t{q.x - back.x, q.y - back.y}, h2_t);
+}
 
 // Byte pointer in the global address space: a load through it takes a
 // uniform 64-bit base and a 32-bit per-lane byte offset.
@@ -412,8 +421,9 @@ constexpr int kSkmThreads = kKfMfmaWaves * kWave;
 constexpr int kSkmXYBatch = 4;
 
 // LDS bytes of k_sk_mfma for a launch whose largest box has H = hmax + 1
+// (X and Y rows h = 0 .. H - 1 plus a zero row each, and the Z W records)
 __host__ __device__ inline int sk_mfma_lds_bytes(int H) {
-  return 2 * H * kSkmRow * 8 + 8 * 32 * 4 * 16;
+  return 2 * (H + 1) * kSkmRow * 8 + 8 * 32 * 4 * 16;
 }
 
 __global__ __launch_bounds__(kSkmThreads) void k_sk_mfma(const BoxDesc* __restrict__ boxes,
@@ -428,9 +438,9 @@ __global__ __launch_bounds__(kSkmThreads) void k_sk_mfma(const BoxDesc* __restri
   const int nt = min(2, B.sknt - t0);  // its hz tiles with k vectors (1 or 2)
   constexpr int kRow4 = kSkmRow / 2;   // LDS row in 16-byte units
   typedef __attribute__((address_space(3))) u4_t lds_u4;  // LDS stores, not flat
-  lds_u4* xs = (lds_u4*)smem4;         // [hl][kRow4]
-  lds_u4* ys = xs + hl * kRow4;        // [hl][kRow4]
-  lds_u4* ss = ys + hl * kRow4;        // [8 groups][32 hz][4]
+  lds_u4* xs = (lds_u4*)smem4;         // [hl + 1][kRow4]
+  lds_u4* ys = xs + (hl + 1) * kRow4;  // [hl + 1][kRow4]
+  lds_u4* ss = ys + (hl + 1) * kRow4;  // [8 groups][32 hz][4]
   typedef __attribute__((address_space(3))) f2v_t lds_f2w;
   lds_f2w* xsf = (lds_f2w*)xs;         // the same rows as complex elements
   lds_f2w* ysf = (lds_f2w*)ys;
@@ -441,11 +451,14 @@ __global__ __launch_bounds__(kSkmThreads) void k_sk_mfma(const BoxDesc* __restri
   const gbytes gsb = (gbytes)B.sb;
   const int jb = it.a * B.chunk, je = min(N, jb + B.chunk);
   const int ngrp = B.npad / 4;
+  // Row H of X and Y stays zero (staging writes rows < H): lanes without a
+  // column read it and contribute nothing.
+  if (tid < 2 * kRow4) (tid < kRow4 ? xs : ys)[H * kRow4 + tid % kRow4] = u4_t{0u, 0u, 0u, 0u};
   // this lane's column
   const int c0 = it.b + 32 * w;
   const int c = c0 + r;
   const bool cv = c < ncol;
-  int ax = 0, ay = 0;
+  int ax = H, ay = H;
   float sx = 1.f, sy = 1.f;
   if (cv) {
     const int hx = B.cols[c].hx, hy = B.cols[c].hy;
@@ -505,19 +518,19 @@ __global__ __launch_bounds__(kSkmThreads) void k_sk_mfma(const BoxDesc* __restri
 #pragma unroll
     for (int s = 0; s < kSkmTile / 8; ++s) {
       const int u = 8 * s + 4 * g;  // tile-local first ion of this lane's K slice
-      h8_t ah, al;
+      h2_t hi[4], lo[4];
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
         const f2_t x = xr[u + e], y = yr[u + e];
         const float xi = sx * x.y, yi = sy * y.y;
-        float qr = x.x * y.x - xi * yi, qi = x.x * yi + xi * y.x;
-        if (!cv) qr = qi = 0.f;
-        _Float16 h0, l0, h1, l1;
-        split_f16(qr, h0, l0);
-        split_f16(qi, h1, l1);
-        ah[2 * e] = h0; al[2 * e] = l0;
-        ah[2 * e + 1] = h1; al[2 * e + 1] = l1;
+        split_f16x2(f2v_t{x.x * y.x - xi * yi, x.x * yi + xi * y.x}, hi[e], lo[e]);
       }
+      const h8_t ah = __builtin_shufflevector(
+          __builtin_shufflevector(hi[0], hi[1], 0, 1, 2, 3),
+          __builtin_shufflevector(hi[2], hi[3], 0, 1, 2, 3), 0, 1, 2, 3, 4, 5, 6, 7);
+      const h8_t al = __builtin_shufflevector(
+          __builtin_shufflevector(lo[0], lo[1], 0, 1, 2, 3),
+          __builtin_shufflevector(lo[2], lo[3], 0, 1, 2, 3), 0, 1, 2, 3, 4, 5, 6, 7);
       const int grp = 2 * s + g;  // tile-local ion group of this K slice
 #pragma unroll
       for (int t = 0; t < 2; ++t) {  // a second tile past the box's hz adds zeros
