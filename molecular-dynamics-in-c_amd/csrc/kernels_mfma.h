@@ -419,6 +419,8 @@ constexpr int kSkmRow = kSkmTile + 2;  // complex per LDS row
 constexpr int kSkmThreads = kKfMfmaWaves * kWave;
 // X/Y staging loads per thread in flight at once
 constexpr int kSkmXYBatch = 4;
+// X/Y elements per thread fetched one tile ahead (all of them for hmax < 24)
+constexpr int kSkmPre = 6;
 
 // LDS bytes of k_sk_mfma for a launch whose largest box has H = hmax + 1
 // (X and Y rows h = 0 .. H - 1 plus a zero row each, and the Z W records)
@@ -477,13 +479,26 @@ __global__ __launch_bounds__(kSkmThreads) void k_sk_mfma(const BoxDesc* __restri
   for (int t = 0; t < 2; ++t)
 #pragma unroll
     for (int v = 0; v < 16; ++v) acc[t][v] = 0.f;
-  for (int j0 = jb; j0 < je; j0 += kSkmTile) {
-    // Staging, with the global loads issued in batches so that several are
-    // in flight at once: the records of the tile's 8 ion groups for both hz
-    // tiles (zero past the box's hz), then X and Y of one ion (h < H) per
-    // element, four elements per thread at a time.
-    __syncthreads();  // the previous tile's reads are done
-    u4_t vss[es / kSkmThreads];
+  // Staging of one ion tile: the records of its 8 ion groups for both hz
+  // tiles (zero past the box's hz), and X and Y of one ion (h < H) per
+  // element.  The records and the first kSkmPre elements of each thread
+  // are fetched into registers one tile ahead, so their loads are in
+  // flight while the MFMAs of the current tile run; elements beyond that
+  // (large hmax) are loaded at the tile itself, kSkmXYBatch at a time.
+  u4_t vss[es / kSkmThreads];
+  f2v_t vpre[kSkmPre];
+  auto xy_elem = [&](int j0, int e) {
+    const int tb = e >= exf, q = e - tb * exf, i = j0 + q % kSkmTile;
+    f2v_t v = f2v_t{0.f, 0.f};
+    if (e < 2 * exf && i < N)
+      v = gload<f2v_t>(tb ? gyb : gxb, 8u * (unsigned)((size_t)(q / kSkmTile) * N + i));
+    return v;
+  };
+  auto xy_store = [&](int e, f2v_t v) {
+    const int tb = e >= exf, q = e - tb * exf;
+    if (e < 2 * exf) (tb ? ysf : xsf)[(q / kSkmTile) * kSkmRow + q % kSkmTile] = v;
+  };
+  auto fetch = [&](int j0) {
 #pragma unroll
     for (int m = 0; m < es / kSkmThreads; ++m) {
       const int q = tid + m * kSkmThreads;
@@ -493,56 +508,62 @@ __global__ __launch_bounds__(kSkmThreads) void k_sk_mfma(const BoxDesc* __restri
       if (gg < ngrp && hz < HP)
         vss[m] = gload<u4_t>(gsb, 16u * (unsigned)(((size_t)gg * HP + 16 * t0) * 4 + rem));
     }
-    for (int e0 = 0; e0 < 2 * exf; e0 += kSkmXYBatch * kSkmThreads) {
-      f2v_t v[kSkmXYBatch];
 #pragma unroll
-      for (int m = 0; m < kSkmXYBatch; ++m) {
-        const int e = e0 + tid + m * kSkmThreads;
-        const int tb = e >= exf, q = e - tb * exf, i = j0 + q % kSkmTile;
-        v[m] = f2v_t{0.f, 0.f};
-        if (e < 2 * exf && i < N)
-          v[m] = gload<f2v_t>(tb ? gyb : gxb, 8u * (unsigned)((size_t)(q / kSkmTile) * N + i));
+    for (int m = 0; m < kSkmPre; ++m) vpre[m] = xy_elem(j0, tid + m * kSkmThreads);
+  };
+  // the tile loop for NT hz tiles per item (a box with hmax < 16 has one:
+  // no MFMAs on its all-zero second tile)
+  auto tiles = [&](auto ntc) {
+    constexpr int NT = decltype(ntc)::value;
+    if (jb < je) fetch(jb);
+    for (int j0 = jb; j0 < je; j0 += kSkmTile) {
+      __syncthreads();  // the previous tile's reads are done
+#pragma unroll
+      for (int m = 0; m < es / kSkmThreads; ++m) ss[tid + m * kSkmThreads] = vss[m];
+#pragma unroll
+      for (int m = 0; m < kSkmPre; ++m) xy_store(tid + m * kSkmThreads, vpre[m]);
+      for (int e0 = kSkmPre * kSkmThreads; e0 < 2 * exf; e0 += kSkmXYBatch * kSkmThreads) {
+        f2v_t v[kSkmXYBatch];
+#pragma unroll
+        for (int m = 0; m < kSkmXYBatch; ++m) v[m] = xy_elem(j0, e0 + tid + m * kSkmThreads);
+#pragma unroll
+        for (int m = 0; m < kSkmXYBatch; ++m) xy_store(e0 + tid + m * kSkmThreads, v[m]);
       }
-      if (e0 == 0) {
+      __syncthreads();
+      if (j0 + kSkmTile < je) fetch(j0 + kSkmTile);
 #pragma unroll
-        for (int m = 0; m < es / kSkmThreads; ++m) ss[tid + m * kSkmThreads] = vss[m];
-      }
+      for (int s = 0; s < kSkmTile / 8; ++s) {
+        const int u = 8 * s + 4 * g;  // tile-local first ion of this lane's K slice
+        h2_t hi[4], lo[4];
 #pragma unroll
-      for (int m = 0; m < kSkmXYBatch; ++m) {
-        const int e = e0 + tid + m * kSkmThreads;
-        const int tb = e >= exf, q = e - tb * exf;
-        if (e < 2 * exf) (tb ? ysf : xsf)[(q / kSkmTile) * kSkmRow + q % kSkmTile] = v[m];
+        for (int e = 0; e < 4; ++e) {
+          const f2_t x = xr[u + e], y = yr[u + e];
+          const float xi = sx * x.y, yi = sy * y.y;
+          split_f16x2(f2v_t{x.x * y.x - xi * yi, x.x * yi + xi * y.x}, hi[e], lo[e]);
+        }
+        const h8_t ah = __builtin_shufflevector(
+            __builtin_shufflevector(hi[0], hi[1], 0, 1, 2, 3),
+            __builtin_shufflevector(hi[2], hi[3], 0, 1, 2, 3), 0, 1, 2, 3, 4, 5, 6, 7);
+        const h8_t al = __builtin_shufflevector(
+            __builtin_shufflevector(lo[0], lo[1], 0, 1, 2, 3),
+            __builtin_shufflevector(lo[2], lo[3], 0, 1, 2, 3), 0, 1, 2, 3, 4, 5, 6, 7);
+        const int grp = 2 * s + g;  // tile-local ion group of this K slice
+#pragma unroll
+        for (int t = 0; t < NT; ++t) {
+          const lds_u4* rec = ss + (grp * 32 + 16 * t + (r >> 1)) * 4 + 2 * (r & 1);
+          const h8_t bh = __builtin_bit_cast(h8_t, rec[0]);
+          const h8_t bl = __builtin_bit_cast(h8_t, rec[1]);
+          acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, bh, acc[t], 0, 0, 0);
+          acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, bl, acc[t], 0, 0, 0);
+          acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_f16(al, bh, acc[t], 0, 0, 0);
+        }
       }
     }
-    __syncthreads();
-#pragma unroll
-    for (int s = 0; s < kSkmTile / 8; ++s) {
-      const int u = 8 * s + 4 * g;  // tile-local first ion of this lane's K slice
-      h2_t hi[4], lo[4];
-#pragma unroll
-      for (int e = 0; e < 4; ++e) {
-        const f2_t x = xr[u + e], y = yr[u + e];
-        const float xi = sx * x.y, yi = sy * y.y;
-        split_f16x2(f2v_t{x.x * y.x - xi * yi, x.x * yi + xi * y.x}, hi[e], lo[e]);
-      }
-      const h8_t ah = __builtin_shufflevector(
-          __builtin_shufflevector(hi[0], hi[1], 0, 1, 2, 3),
-          __builtin_shufflevector(hi[2], hi[3], 0, 1, 2, 3), 0, 1, 2, 3, 4, 5, 6, 7);
-      const h8_t al = __builtin_shufflevector(
-          __builtin_shufflevector(lo[0], lo[1], 0, 1, 2, 3),
-          __builtin_shufflevector(lo[2], lo[3], 0, 1, 2, 3), 0, 1, 2, 3, 4, 5, 6, 7);
-      const int grp = 2 * s + g;  // tile-local ion group of this K slice
-#pragma unroll
-      for (int t = 0; t < 2; ++t) {  // a second tile past the box's hz adds zeros
-        const lds_u4* rec = ss + (grp * 32 + 16 * t + (r >> 1)) * 4 + 2 * (r & 1);
-        const h8_t bh = __builtin_bit_cast(h8_t, rec[0]);
-        const h8_t bl = __builtin_bit_cast(h8_t, rec[1]);
-        acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, bh, acc[t], 0, 0, 0);
-        acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, bl, acc[t], 0, 0, 0);
-        acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_f16(al, bh, acc[t], 0, 0, 0);
-      }
-    }
-  }
+  };
+  if (nt == 2)
+    tiles(std::integral_constant<int, 2>{});
+  else
+    tiles(std::integral_constant<int, 1>{});
   if (c0 >= ncol) return;
   // result (row = column 8 (v/4) + 4 g + v%4, col = (hz r/2, out r%2)):
   // lanes r and r^1 hold Re S and Im S of the same (column, hz)
