@@ -109,6 +109,30 @@ int  mdg_ewald_params(double rc, double l_max, double eps, double *alpha, int *h
  * identical bits alone or in any mix. */
 int  mdg_batch_create(int nbox, const mdg_box_params *params, int precision,
                       mdg_batch **out);
+/* Boxes of any of mdc's tabulated materials (mdc/examples/<m>: the tables,
+ * lookup, charges and masses that the material's set_potential and mk_table
+ * make, control.c), in table mode: an orthorhombic box of n_ions ions whose
+ * positions and species come from mdg_batch_set_state.  The tables are
+ * copied at creation. */
+typedef struct {
+  int           n_ions;
+  double        L[3];                /* box [A]                                  */
+  int           cells[3];            /* unit cells per axis (sys.nx, ny, nz)     */
+  int           natoms_in_mol_unit;  /* ctl.natoms_in_mol_unit                   */
+  int           kinds;               /* ctl.kinds_of_ions, <= 4                  */
+  double        Z[4], m[4];          /* ion.Z, ion.m per species                 */
+  int           lookup[16];          /* sys.lookup: pair type at ti * kinds + tj */
+  int           ntypes;              /* kinds (kinds + 1) / 2                    */
+  int           table_division;      /* sys.table_division (td + 1 bins)         */
+  const double *pe, *fe;             /* sys.pe1r1, sys.fe1r1: ntypes x (td + 1)  */
+  double        alpha;               /* sys.a1; 0 or > 10 turns Ewald off        */
+  int           hm;                  /* sys.hm                                   */
+  double        rc;                  /* sys.radius                               */
+  double        dt_fs;               /* ctl.delta_time_fs                        */
+  int           cells_mode;          /* MDG_CELLS_AUTO, MDG_ALL_PAIRS, MDG_CELL_LIST */
+} mdg_table_box;
+int  mdg_batch_create_tabulated(int nbox, const mdg_table_box *boxes, int precision,
+                                mdg_batch **out);
 int  mdg_batch_destroy(mdg_batch *b);
 int  mdg_batch_n_ions(const mdg_batch *b, int box);
 /* device bytes held by box `box` (all its arrays), for HBM budgeting */

@@ -176,6 +176,7 @@ int mdg_batch_create(int nbox, const mdg_box_params* params, int precision, mdg_
     }
     b->boxes.push_back(box);
     b->cells.push_back({params[k].nx, params[k].ny, params[k].nz});
+    b->mol_unit.push_back(2);  // rock salt: two ions per formula unit
     // start from mdc's lattice at rest
     const int N = spec.N;
     std::vector<double> r(3 * (size_t)N);
@@ -190,6 +191,83 @@ int mdg_batch_create(int nbox, const mdg_box_params* params, int precision, mdg_
   }
   b->set = new BoxSet(precision, rmode);
   int rc = b->set->set_boxes(b->boxes);
+  if (rc != MDG_OK) {
+    mdg_batch_destroy(b);
+    return rc;
+  }
+  *out = b;
+  return MDG_OK;
+}
+
+int mdg_batch_create_tabulated(int nbox, const mdg_table_box* tb, int precision,
+                               mdg_batch** out) {
+  if (!out || nbox <= 0 || !tb) return MDG_E_ARG;
+  if (precision != MDG_FP64 && precision != MDG_MIXED) return MDG_E_ARG;
+  *out = nullptr;
+  int ndev = 0;
+  if (mdg_device_count(&ndev) != MDG_OK) {
+    fprintf(stderr, "libmdgpu: no HIP device available\n");
+    return MDG_E_NODEVICE;
+  }
+  std::vector<BoxSpec> specs(nbox);
+  for (int k = 0; k < nbox; ++k) {
+    const mdg_table_box& t = tb[k];
+    const int nt = t.kinds * (t.kinds + 1) / 2;
+    if (t.n_ions <= 0 || t.kinds < 1 || t.kinds > 4 || t.ntypes != nt || !t.pe || !t.fe ||
+        !(t.L[0] > 0) || !(t.L[1] > 0) || !(t.L[2] > 0) || !(t.rc > 0.5) || !(t.dt_fs > 0) ||
+        t.hm < 0 || t.cells[0] <= 0 || t.cells[1] <= 0 || t.cells[2] <= 0 ||
+        t.natoms_in_mol_unit <= 0 || t.table_division < (int)((t.rc - 0.5) * 1000.0))
+      return MDG_E_ARG;
+    for (int q = 0; q < t.kinds * t.kinds; ++q)
+      if (t.lookup[q] < 0 || t.lookup[q] >= nt) return MDG_E_ARG;
+    BoxSpec& s = specs[k];
+    s.N = t.n_ions;
+    for (int d = 0; d < 3; ++d) s.L[d] = t.L[d];
+    s.alpha = t.alpha;
+    if (t.alpha > 0) {  // init.c:38-41
+      s.p2a2 = kPiPi / (t.alpha * t.alpha);
+      s.a3 = t.alpha / kSqrtPi;
+    }
+    s.hm = t.hm;
+    s.rc = t.rc;
+    s.rc2 = t.rc * t.rc;
+    s.dt = t.dt_fs / kSecd2Fs;
+    s.dt2 = s.dt * s.dt;
+    s.real_mode = MDG_TABLE;
+    s.cells = t.cells_mode;
+    s.kinds = t.kinds;
+    for (int q = 0; q < t.kinds; ++q) {
+      s.Z[q] = t.Z[q];
+      s.m[q] = t.m[q];
+    }
+    for (int q = 0; q < 16; ++q) s.lookup[q] = q < t.kinds * t.kinds ? t.lookup[q] : 0;
+    s.pe = t.pe;
+    s.fe = t.fe;
+    s.table_nb = t.table_division + 1;
+    s.table_ntypes = nt;
+  }
+  mdg_batch* b = new mdg_batch();
+  b->precision = precision;
+  b->real_mode = MDG_TABLE;
+  if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete b;
+    return MDG_E_HIP;
+  }
+  for (int k = 0; k < nbox; ++k) {
+    Box* box = nullptr;
+    std::string why;
+    const int rc = box_create(specs[k], precision, &box, &why);  // copies the tables
+    if (rc != MDG_OK) {
+      if (!why.empty()) fprintf(stderr, "libmdgpu: box %d: %s\n", k, why.c_str());
+      mdg_batch_destroy(b);
+      return rc;
+    }
+    b->boxes.push_back(box);
+    b->cells.push_back({tb[k].cells[0], tb[k].cells[1], tb[k].cells[2]});
+    b->mol_unit.push_back(tb[k].natoms_in_mol_unit);
+  }
+  b->set = new BoxSet(precision, MDG_TABLE);
+  const int rc = b->set->set_boxes(b->boxes);
   if (rc != MDG_OK) {
     mdg_batch_destroy(b);
     return rc;

@@ -14,7 +14,8 @@ struct mdg_batch {
   int precision = MDG_FP64;
   int real_mode = MDG_TABLE;
   std::vector<mdg::Box*> boxes;
-  std::vector<std::array<int, 3>> cells;  // conventional cells nx, ny, nz per box
+  std::vector<std::array<int, 3>> cells;  // unit cells nx, ny, nz per box (files/out)
+  std::vector<int> mol_unit;              // ions per formula unit (files/out per mole)
   mdg::BoxSet* set = nullptr;
   hipStream_t stream = nullptr;
 };

@@ -188,6 +188,7 @@ namespace {
 struct MdcFiles {
   std::vector<FILE*> out, pos, vel;
   std::vector<std::array<int, 3>> cells;
+  std::vector<int> mol_unit, kinds;
   std::vector<std::array<double, 3>> L;
   std::vector<char> text;
   int what = 0, status = MDG_OK;
@@ -219,12 +220,12 @@ void write_frame(const mdg_frame* f, void* user) {
   o.pot = f->scalars.pot;
   o.pres = f->scalars.pres;
   o.n_ions = f->n_ions;
-  o.natoms_in_mol_unit = 2;  // rock salt: two ions per formula unit (control.c:37)
+  o.natoms_in_mol_unit = W.mol_unit[k];
   for (int d = 0; d < 3; ++d) {
     o.cells[d] = W.cells[k][d];
     o.L[d] = W.L[k][d];
   }
-  o.kinds = 2;
+  o.kinds = W.kinds[k];
   for (int q = 0; q < 4; ++q) o.msd[q] = f->msd[q];
   emit(W, W.out[k], [&](char* b, size_t c) { return mdg_format_out(b, c, &o); });
   if (W.what & MDG_WRITE_POSITIONS)
@@ -246,6 +247,8 @@ extern "C" int mdg_batch_write_mdc_files(mdg_batch* b, int nsteps, int every, in
   MdcFiles W;
   W.what = what;
   W.cells = b->cells;
+  W.mol_unit = b->mol_unit;
+  for (Box* x : b->boxes) W.kinds.push_back(x->desc.kinds);
   W.out.assign(nb, nullptr);
   W.pos.assign(nb, nullptr);
   W.vel.assign(nb, nullptr);

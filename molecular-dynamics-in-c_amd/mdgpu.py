@@ -167,6 +167,39 @@ class Frame(C.Structure):
 
 
 FRAME_FN = C.CFUNCTYPE(None, C.POINTER(Frame), C.c_void_p)
+
+
+class TableBox(C.Structure):
+    """mdg_table_box: a box of one of mdc's tabulated materials."""
+    _fields_ = [("n_ions", C.c_int), ("L", C.c_double * 3), ("cells", C.c_int * 3),
+                ("natoms_in_mol_unit", C.c_int), ("kinds", C.c_int),
+                ("Z", C.c_double * 4), ("m", C.c_double * 4), ("lookup", C.c_int * 16),
+                ("ntypes", C.c_int), ("table_division", C.c_int), ("pe", _dp), ("fe", _dp),
+                ("alpha", C.c_double), ("hm", C.c_int), ("rc", C.c_double),
+                ("dt_fs", C.c_double), ("cells_mode", C.c_int)]
+
+
+def table_box(n_ions, L, cells, mol_unit, Z, m, lookup, pe, fe, alpha, hm, rc, dt_fs=2.0,
+              cells_mode=CELLS_AUTO):
+    """A TableBox from mdc's per-material data (the numpy tables are kept
+    alive on the returned object until the batch has copied them)."""
+    k = len(Z)
+    tb = TableBox()
+    tb.n_ions, tb.kinds, tb.ntypes = n_ions, k, k * (k + 1) // 2
+    tb.L[:] = list(L)
+    tb.cells[:] = list(cells)
+    tb.natoms_in_mol_unit = mol_unit
+    for q in range(k):
+        tb.Z[q], tb.m[q] = Z[q], m[q]
+    for q, t in enumerate(lookup):
+        tb.lookup[q] = int(t)
+    pe = np.ascontiguousarray(pe, dtype=np.float64)
+    fe = np.ascontiguousarray(fe, dtype=np.float64)
+    tb.table_division = pe.size // tb.ntypes - 1
+    tb.pe, tb.fe = _p(pe), _p(fe)
+    tb._keep = (pe, fe)
+    tb.alpha, tb.hm, tb.rc, tb.dt_fs, tb.cells_mode = alpha, hm, rc, dt_fs, cells_mode
+    return tb
 WRITE_POSITIONS, WRITE_VELOCITIES = 1, 2
 
 
@@ -199,6 +232,8 @@ def lib():
             "mdg_batch_create": ([C.c_int, C.POINTER(BoxParams), C.c_int, C.POINTER(vp)],
                                  C.c_int),
             "mdg_batch_destroy": ([vp], C.c_int),
+            "mdg_batch_create_tabulated": ([C.c_int, C.POINTER(TableBox), C.c_int,
+                                            C.POINTER(vp)], C.c_int),
             "mdg_batch_n_ions": ([vp, C.c_int], C.c_int),
             "mdg_batch_box_bytes": ([vp, C.c_int], C.c_size_t),
             "mdg_batch_set_state": ([vp, C.c_int, _dp, _dp, C.POINTER(C.c_int)], C.c_int),
@@ -331,6 +366,18 @@ class Batch:
         self.h = h
         self.nbox = len(params)
         self.precision = precision
+
+    @classmethod
+    def tabulated(cls, boxes, precision=FP64):
+        """Boxes of mdc's tabulated materials (TableBox, see table_box); their
+        positions and species come from set_state."""
+        arr = (TableBox * len(boxes))(*boxes)
+        h = C.c_void_p()
+        check(lib().mdg_batch_create_tabulated(len(boxes), arr, precision, C.byref(h)),
+              "batch_create_tabulated")
+        self = cls.__new__(cls)
+        self.h, self.nbox, self.precision = h, len(boxes), precision
+        return self
 
     def close(self):
         if self.h:

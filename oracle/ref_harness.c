@@ -106,6 +106,20 @@ void ref_get_scalars(double *out)
 
 int ref_table_division(void) { return sys.table_division; }
 
+/* the material's species data as its set_potential and mk_table made it */
+int ref_kinds(void) { return ctl.kinds_of_ions; }
+int ref_natoms_in_mol_unit(void) { return ctl.natoms_in_mol_unit; }
+void ref_get_ions(double *Z, double *m, int *lookup, double *L)
+{
+  int k;
+  for (k = 0; k < ctl.kinds_of_ions; k++) {
+    Z[k] = ion.Z[k];
+    m[k] = ion.m[k];
+  }
+  for (k = 0; k < ctl.kinds_of_ions * ctl.kinds_of_ions; k++) lookup[k] = sys.lookup[k];
+  L[0] = sys.Lx; L[1] = sys.Ly; L[2] = sys.Lz;
+}
+
 void ref_get_table(double *pe, double *fe)
 {
   size_t n = (size_t)3 * (size_t)(sys.table_division + 1);

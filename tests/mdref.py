@@ -227,17 +227,25 @@ def jitter_positions(r, L, amp, seed):
 
 
 # ---------------------------------------------------------------- reference
-_ref = None
+_refs = {}
 
 
-def ref_available():
-    return os.path.exists(REF_SO)
+def ref_available(material=None):
+    return os.path.exists(ref_path(material))
 
 
-def ref():
-    global _ref
-    if _ref is None:
-        R = C.CDLL(REF_SO)
+def ref_path(material=None):
+    """oracle/_ref/libmdcref.so (mdc/program), or the harness built with the
+    material plugin of mdc/examples/<material> ('.' in the name as '_')."""
+    if material is None:
+        return REF_SO
+    return os.path.join(ROOT, "oracle", "_ref", f"libmdcref_{material}.so")
+
+
+def ref(material=None):
+    path = ref_path(material)
+    if path not in _refs:
+        R = C.CDLL(path)
         R.ref_setup.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_int,
                                 C.c_double, C.c_double]
         R.ref_setup.restype = C.c_int
@@ -257,18 +265,20 @@ def ref():
         for n in ("ref_print_to_file", "ref_r_position", "ref_r_velocities"):
             getattr(R, n).argtypes = [C.c_char_p]
             getattr(R, n).restype = C.c_int
-        _ref = R
-    return _ref
+        R.ref_get_ions.argtypes = [_dp, _dp, _ip, _dp]
+        _refs[path] = R
+    return _refs[path]
 
 
 class Reference:
     """The compiled mdc sources (one global system per process)."""
 
-    def __init__(self, nx, ny, nz, a, alpha, hm, radius, dt_fs=2.0):
-        self.N = ref().ref_setup(nx, ny, nz, a, alpha, hm, radius, dt_fs)
+    def __init__(self, nx, ny, nz, a, alpha, hm, radius, dt_fs=2.0, material=None):
+        self.R = ref(material)
+        self.N = self.R.ref_setup(nx, ny, nz, a, alpha, hm, radius, dt_fs)
 
     def set_state(self, r=None, v=None, f=None):
-        R = ref()
+        R = self.R
         if r is not None:
             r = np.ascontiguousarray(r, dtype=np.float64)
             R.ref_set_positions(_ptr(r[0]), _ptr(r[1]), _ptr(r[2]))
@@ -281,48 +291,48 @@ class Reference:
 
     def state(self):
         out = np.zeros(12 * self.N)
-        ref().ref_get_state(_ptr(out))
+        self.R.ref_get_state(_ptr(out))
         out = out.reshape(4, 3, self.N)
         return dict(r=out[0], v=out[1], f=out[2], f0=out[3])
 
     def species(self):
         out = np.zeros(self.N, dtype=np.int32)
-        ref().ref_get_species(_ptr(out, C.c_int))
+        self.R.ref_get_species(_ptr(out, C.c_int))
         return out
 
     def scalars(self):
         out = np.zeros(12)
-        ref().ref_get_scalars(_ptr(out))
+        self.R.ref_get_scalars(_ptr(out))
         keys = ("pot", "virX", "virY", "virZ", "kin", "kinX", "kinY", "kinZ",
                 "presX", "presY", "presZ", "pres")
         return dict(zip(keys, out))
 
     def table(self):
-        n = 3 * (ref().ref_table_division() + 1)
+        n = 3 * (self.R.ref_table_division() + 1)
         pe, fe = np.zeros(n), np.zeros(n)
-        ref().ref_get_table(_ptr(pe), _ptr(fe))
+        self.R.ref_get_table(_ptr(pe), _ptr(fe))
         return pe, fe
 
     def call(self, name):
-        getattr(ref(), "ref_" + name)()
+        getattr(self.R, "ref_" + name)()
 
     def run(self, first_step, nsteps):
-        ref().ref_run(first_step, nsteps)
+        self.R.ref_run(first_step, nsteps)
 
     def newton(self, first_step, nsteps, t_control_step=0, temp=0.0, t_until=10000,
                moment_every=0, msd=False):
-        ref().ref_newton(first_step, nsteps, t_control_step, temp, t_until, moment_every,
+        self.R.ref_newton(first_step, nsteps, t_control_step, temp, t_until, moment_every,
                          1 if msd else 0)
 
     def control_temp(self, d_step, temp):
-        ref().ref_control_temp(d_step, temp)
+        self.R.ref_control_temp(d_step, temp)
 
     def reset_control_temp(self):
-        ref().ref_reset_control_temp()
+        self.R.ref_reset_control_temp()
 
     def write(self, which, path):
         """files.c's print_to_file / r_position / r_velocities into path."""
-        assert getattr(ref(), "ref_" + which)(os.fsencode(str(path))) == 0
+        assert getattr(self.R, "ref_" + which)(os.fsencode(str(path))) == 0
         return open(path).read()
 
     def msd(self):
@@ -330,6 +340,15 @@ class Reference:
         arr = np.zeros(6 * self.N)
         val = np.zeros(4)
         num = np.zeros(4, dtype=np.int32)
-        ref().ref_get_msd(_ptr(arr), _ptr(val), _ptr(num, C.c_int))
+        self.R.ref_get_msd(_ptr(arr), _ptr(val), _ptr(num, C.c_int))
         arr = arr.reshape(6, self.N)
         return arr[:3], arr[3:], val[:2], num[:2]
+
+    def ions(self):
+        """The material's species data: kinds, Z, m, lookup, L, ions per mol unit."""
+        k = self.R.ref_kinds()
+        Z, m, L = np.zeros(4), np.zeros(4), np.zeros(3)
+        lk = np.zeros(16, dtype=np.int32)
+        self.R.ref_get_ions(_ptr(Z), _ptr(m), _ptr(lk, C.c_int), _ptr(L))
+        return dict(kinds=k, Z=Z[:k], m=m[:k], lookup=lk[:k * k], L=L,
+                    mol_unit=self.R.ref_natoms_in_mol_unit())
