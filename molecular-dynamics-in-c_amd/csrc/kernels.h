@@ -195,36 +195,12 @@ __device__ __forceinline__ void fold(const BoxCtx& c, double& dx, double& dy, do
 // an unevaluated FP32 pair (x, xlo): the exponential multiplies r's
 // rounding error by r / rho (about 9 for alkali halides), which would
 // otherwise dominate the pair force error near contact.
-__device__ __forceinline__ void pair32(const BoxCtx& c, const Row32& w, double dr, float& p,
-                                       float& f) {
-  const float kLog2e = 1.4426950408889634f;
-  const float kLn2 = 0.6931471805599453f;
-  const float r2 = (float)dr;
-  const float r2lo = (float)(dr - (double)r2);
-  const float rinv = __frsqrt_rn(r2);
-  const float r = r2 * rinv;
-  const float rlo = 0.5f * (fmaf(-r, r, r2) + r2lo) * rinv;  // r + rlo = sqrt(dr)
-  const float rinv2 = rinv * rinv;
-  const float z = c.alpha * r;
-  const float erfc_r = erfc_cheb(z) * rinv;
-  const float g = exp2f(-z * z * kLog2e);
-  const float a = (w.sig - r) - rlo;                 // sigma - r is exact near contact
-  const float x = a * c.inv_rho_l2e;
-  const float xlo = fmaf(a, c.inv_rho_l2e, -x) + a * c.inv_rho_l2e_lo;
-  const float eb = w.cb * exp2f(x) * fmaf(kLn2, xlo, 1.0f);
-  const float r6 = rinv2 * rinv2 * rinv2;
-  const float r8 = r6 * rinv2;
-  p = w.zz * erfc_r + eb - w.c6 * r6 - w.d8 * r8;
-  f = w.zz * (c.two_a3 * g + erfc_r) * rinv2 +
-      (eb * c.inv_rho * rinv - 6.0f * w.c6 * r8 - 8.0f * w.d8 * r8 * rinv2);
-}
-
-// pair32 with the hardware reciprocal and base-2 exponential (v_rcp_f32,
-// v_exp_f32, about 1 ulp) in place of the correctly rounded forms.  The
-// library forms only add a rescaling for results in the denormal range;
-// here the exponents are about -20 (Gaussian, (alpha rc)^2 log2 e at an
-// Ewald accuracy of 1e-6) and -32 (Born-Mayer at 10 A), and a result
-// below 2^-126 would be flushed to a zero that is negligible anyway.
+// The hardware reciprocal square root, reciprocal and base-2 exponential
+// (v_rsq_f32, v_rcp_f32, v_exp_f32, about 1 ulp) stand in for the
+// correctly rounded forms, which only add a rescaling for results in the
+// denormal range; here the exponents are about -20 (Gaussian, (alpha rc)^2
+// log2 e at an Ewald accuracy of 1e-6) and -32 (Born-Mayer at 10 A), and a
+// result below 2^-126 would be flushed to a zero that is negligible anyway.
 __device__ __forceinline__ void pair32_fast(const BoxCtx& c, const Row32& w, double dr, float& p,
                                             float& f) {
   const float kLog2e = 1.4426950408889634f;
@@ -308,7 +284,6 @@ __device__ __forceinline__ void real_tile(const BoxDesc& B, const BoxCtx& c, con
       if (dr <= c.rc2m) m |= 1ull << u;
     }
   }
-  float ffx = 0.f, ffy = 0.f, ffz = 0.f, fpot = 0.f, fvx = 0.f, fvy = 0.f, fvz = 0.f;
   while (__ballot(m != 0ull)) {
     if (m == 0ull) continue;
     const int u = __builtin_ctzll(m);
@@ -343,37 +318,7 @@ __device__ __forceinline__ void real_tile(const BoxDesc& B, const BoxCtx& c, con
         a.vx += f * dx * dx; a.vy += f * dy * dy; a.vz += f * dz * dz;
         ++a.np;
       }
-    } else {
-      // FP32 pair arithmetic; the fold and the cutoff test stay FP64.
-      // Tile sums (at most 64 pairs) in FP32, running sums in FP64.
-      float p, f;
-      if constexpr (KIND == kRealAnalytic32) {
-        // the FP64 cutoff test accepts exactly the FP64 path's pairs
-        const double dr = fma(dx, dx, fma(dy, dy, dz * dz));
-        if (!(dr <= c.rc2)) continue;
-        pair32(c, jt[u] == 0 ? h.row[0] : h.row[1], dr, p, f);
-      } else {  // kRealTable32: FP64 bin as mdc, FP32 table values
-        const double dr = dx * dx + dy * dy + dz * dz;
-        if (!(dr <= c.rc2)) continue;
-        const int bin = (int)((sqrt(dr) - 0.5) * 1000.0);
-        if (bin < 0 || bin >= c.nb) { err |= kErrRange; continue; }
-        const int s = jt[u];
-        const float2* row = s == 0 ? h.t32[0] : (s == 1 ? h.t32[1] : (s == 2 ? h.t32[2] : h.t32[3]));
-        const float2 pf = row[bin];
-        p = pf.x; f = pf.y;
-      }
-      const float fdx = (float)dx, fdy = (float)dy, fdz = (float)dz;
-      ffx = fmaf(f, fdx, ffx); ffy = fmaf(f, fdy, ffy); ffz = fmaf(f, fdz, ffz);
-      if (jid[u] > h.i) {
-        fpot += p;
-        fvx = fmaf(f * fdx, fdx, fvx); fvy = fmaf(f * fdy, fdy, fvy); fvz = fmaf(f * fdz, fdz, fvz);
-        ++a.np;
-      }
     }
-  }
-  if constexpr (KIND == kRealAnalytic32 || KIND == kRealTable32) {
-    a.fx += ffx; a.fy += ffy; a.fz += ffz; a.pot += fpot;
-    a.vx += fvx; a.vy += fvy; a.vz += fvz;
   }
 }
 
@@ -395,6 +340,8 @@ __device__ __forceinline__ void real_store(const BoxDesc& B, const HomeCtx& h, i
 template <int KIND>
 __global__ __launch_bounds__(kRealThreads) void k_real(const BoxDesc* __restrict__ boxes,
                                                        const Item* __restrict__ items, int mode) {
+  static_assert(KIND == kRealTable64 || KIND == kRealAnalytic64,
+                "the mixed real space is k_real_win");
   __shared__ double jx[kRealThreads], jy[kRealThreads], jz[kRealThreads];
   __shared__ int jt[kRealThreads], jid[kRealThreads];
   __shared__ float4 jf[kRealThreads];
