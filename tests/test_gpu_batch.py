@@ -254,3 +254,26 @@ def test_autotuned_tilings_keep_parity(gpu):
             lone.close()
     finally:
         mdgpu.tune_clear()
+
+
+def test_largest_box_kcl_32768_mixed_within_1e5(gpu):
+    """The largest box of the job stream: KCl 16^3 cells (32768 ions,
+    L = 100.6 A), rc = 10 A, eps 1e-6 (hm near 1958, hmax 44: three hz
+    tiles of the structure factor, six K blocks of the back-projection,
+    cell lists).  Mixed forces within 1e-5 of FP64 and the energy within
+    1e-6; the FP64 path itself is pinned to the oracle at 4096 ions."""
+    p = _params(16, H.A_KCL, salt="KCl")
+    assert p.hm > 1500 and int(np.sqrt(p.hm)) >= 40
+    r, v, sp = _state(p, 17, temp=1300.0, amp=0.2)
+    res = {}
+    for prec in (mdgpu.FP64, mdgpu.MIXED):
+        b = mdgpu.Batch([p], prec)
+        b.set_state(0, r=r, v=v, species=sp)
+        b.forces()
+        res[prec] = (b.state(0)["f"], b.scalars(0), b.work(0))
+        b.close()
+    (f64, s64, w64), (f32, s32, w32) = res[mdgpu.FP64], res[mdgpu.MIXED]
+    assert w64["n_ions"] == 32768 and w32["pairs"] == w64["pairs"]
+    assert H.rel_err(f32, f64) <= MIXED_TOL
+    assert H.rms_rel_err(f32, f64) <= MIXED_TOL
+    assert s32["etot"] == pytest.approx(s64["etot"], rel=1e-6)

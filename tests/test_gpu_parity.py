@@ -188,3 +188,25 @@ def test_batch_trajectory_matches_oracle(gpu):
     assert sc["step"] == 10
     assert sc["kin"] == pytest.approx(o.b.kin, rel=1e-11)
     assert sc["pot"] == pytest.approx(o.b.pot, rel=1e-11)
+
+
+def test_dropin_beyond_ordered_size_uses_cells(gpu):
+    """Above N = 8192 the drop-in's real space runs on cell lists (partner
+    order differs from mdc's, so the forces agree to rounding, not bit for
+    bit): NaCl 11^3 cells, 10648 ions, rc = 10 A, against the oracle's
+    all-pairs loop."""
+    a, hm = mdgpu.ewald_params(10.0, 11 * H.A_NACL, 1e-6)
+    o = mdref.Oracle(11, 11, 11, H.A_NACL, a, min(hm, 60), 10.0, 2.0)
+    d = H.DropIn(o)
+    try:
+        r, v = next(states(o, seeds=(61,), amp=0.2))
+        fo, so = oracle_forces(o, r)
+        d.set_state(r=r, v=v, f=np.zeros_like(r))
+        d.call("calc_foc")
+        assert o.N == 10648
+        assert H.rel_err(d.f, fo) <= TOL
+        assert d.sys.pot == pytest.approx(so["pot"], rel=TOL)
+        for k in ("virX", "virY", "virZ"):
+            assert getattr(d.sys, k) == pytest.approx(so[k], rel=TOL)
+    finally:
+        d.release()
