@@ -182,3 +182,10 @@ def test_tune_without_device_fails_loudly():
     with pytest.raises(mdgpu.MdgError) as e:
         mdgpu.tune([p], mdgpu.MIXED)
     assert e.value.code == mdgpu.E_NODEVICE
+
+
+def test_empty_batch_is_an_argument_error():
+    with pytest.raises(mdgpu.MdgError) as e:
+        mdgpu.Batch([])
+    assert e.value.code == mdgpu.E_ARG
+
