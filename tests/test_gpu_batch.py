@@ -277,3 +277,4 @@ def test_largest_box_kcl_32768_mixed_within_1e5(gpu):
     assert H.rel_err(f32, f64) <= MIXED_TOL
     assert H.rms_rel_err(f32, f64) <= MIXED_TOL
     assert s32["etot"] == pytest.approx(s64["etot"], rel=1e-6)
+
