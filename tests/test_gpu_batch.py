@@ -8,7 +8,7 @@ import mdref
 
 pytestmark = pytest.mark.gpu
 
-# north-star bound for the mixed path: relative force error <= 1e-5 vs FP64
+# the mixed path's bound: relative force error <= 1e-5 against FP64
 MIXED_TOL = 1e-5
 
 
