@@ -869,7 +869,7 @@ int BoxSet::set_boxes(const std::vector<Box*>& boxes) {
       if (mixed) {
         // MFMA structure factor: 4 waves x 32 columns per item
         // (two hz tiles of 16 per item)
-        const int cw = 32 * kKfMfmaWaves;
+        const int cw = 32 * kSkmWaves;
         const int nt = (P.hmax + 16) / 16;
         for (int ch = 0; ch < P.tile.nchunk; ++ch)
           for (int c0 = 0; c0 < P.ncol; c0 += cw)

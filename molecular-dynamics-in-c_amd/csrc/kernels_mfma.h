@@ -256,8 +256,17 @@ __global__ __launch_bounds__(kWave) void k_kf_pack(const BoxDesc* __restrict__ b
 template <int KB>
 constexpr int kf_tile_units() { return KB * 2 * kWave + kKfTileCols; }
 
+// Workgroups per CU the back-projection is compiled for (its waves per
+// SIMD, one wave of each workgroup per SIMD): up to four K blocks the
+// registers fit four (left to itself the compiler used 132 VGPRs for three
+// K blocks, three waves; capped, 106 without spills, and the 256-box step
+// is 3% faster); wider boxes, whose W operands take 16 VGPRs per K block,
+// keep their registers.
 template <int KB>
-__global__ __launch_bounds__(kKfMfmaWaves* kWave) void k_kforce_mfma(
+constexpr int kf_min_waves() { return KB <= 4 ? 4 : 1; }
+
+template <int KB>
+__global__ __launch_bounds__(kKfMfmaWaves* kWave, kf_min_waves<KB>()) void k_kforce_mfma(
     const BoxDesc* __restrict__ boxes, const Item* __restrict__ items) {
   constexpr int U = kf_tile_units<KB>();
   constexpr int NT = kKfMfmaWaves * kWave;
@@ -416,7 +425,7 @@ namespace mdg {
 // tile's Z W records of the item's hz tiles.
 constexpr int kSkmTile = 32;
 constexpr int kSkmRow = kSkmTile + 2;  // complex per LDS row
-constexpr int kSkmThreads = kKfMfmaWaves * kWave;
+constexpr int kSkmThreads = kSkmWaves * kWave;
 // X/Y staging loads per thread in flight at once
 constexpr int kSkmXYBatch = 4;
 // X/Y elements per thread fetched one tile ahead (all of them for hmax < 24)

@@ -65,6 +65,7 @@ struct KfTile {
 };
 constexpr int kKfTileCols = 8;
 constexpr int kKfMfmaWaves = 4;     // waves per back-projection workgroup
+constexpr int kSkmWaves = 4;        // waves per MFMA structure-factor workgroup (32 columns each)
 constexpr int kSkmChunk = 512;      // mixed structure factor: ions per FP32 partial sum
 
 // Work item of a launch over many boxes.
