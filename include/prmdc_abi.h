@@ -1,0 +1,163 @@
+/*
+ * prmdc_abi.h -- the C ABI of libmdgpu_pr.so: pr-mdc's force routines
+ * (pr-mdc, the Parrinello-Rahman constant-pressure variant of mdc) computed
+ * on the GPU over pr-mdc's own global objects.
+ *
+ * A maintainer links pr-mdc without real.c and ewald.c and adds
+ * -lmdgpu_pr; pr-mdc's rv.c (calc_foc), cell.c (calc_sigma, set_cutoff,
+ * cell_force, cyclic_bc_dr), pt.c (control_press_PR) and the rest stay its
+ * own.  The structs below are laid out field for field as pr-mdc's
+ * (pr-mdc/program/md.h:23-191); pr-mdc's md.h defines the objects, and
+ * ELF interposition binds this library's references to them.
+ *
+ * This header and mdc_abi.h name different structs with the same names
+ * (mdc and pr-mdc are different programs): include one of them, never both.
+ */
+#ifndef PRMDC_ABI_H
+#define PRMDC_ABI_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* pr-mdc/program/md.h:25-43 */
+typedef struct {
+  int skip_ewald;
+  int calc_max;
+  double delta_time_fs;
+  int kinds_of_ions;
+  int natoms_in_mol_unit;
+  int natoms_in_unit_cell;
+  int kinds_of_interaction;
+} calc_control;
+
+/* pr-mdc/program/md.h:45-84.  The 3x3 matrices are pr-mdc's dmat2d
+ * row-pointer arrays (init.c:240-253): m[i][j]. */
+typedef struct {
+  int nx, ny, nz;
+  double vol;
+  double *rx, *ry, *rz;
+  double *sx, *sy, *sz;
+  double *vx, *vy, *vz;
+  double *fx0, *fy0, *fz0;
+  double *fx, *fy, *fz;
+  double **hu, **hur;
+  double **hm, **hmr;
+  double **hmt, **hmtr;
+  double **hmv, **hmvt;
+  double **hmf;
+  double **g, **sigma;
+  double Lx, Ly, Lz;
+  double angle_ab, angle_bc, angle_ca;
+} cell_parameters;
+
+/* pr-mdc/program/md.h:86-138 */
+typedef struct {
+  int *ion;
+  int N;
+  int step;
+  double dt, dt2;
+  double kB, kk;
+  double pot, kin;
+  double perMol;
+  double e2t, pp2gpa;
+  double a1;
+  int hm, hm_sqrt;
+  double radius, radius2;
+  double cutoff, cutoff2;
+  double *rcsi, *rsni;
+  int table_division;
+  double *pe1r1;
+  double *fe1r1;
+  int *lookup;
+} system_property;
+
+/* pr-mdc/program/md.h:140-155 */
+typedef struct {
+  int t_control_step;
+  int p_control_step;
+  double pres_scalling_factor;
+  double temp_set;
+  double temp_set_initial;
+  double pres_trace;
+  double **pres_set;
+  double **pres_tensor;
+  double **virial;
+  double **kin_tensor;
+} pressure_temperature_set;
+
+/* pr-mdc/program/md.h:157-160 */
+typedef struct {
+  double *m;
+  double *Z;
+} ion_property;
+
+/* pr-mdc/program/md.h:219-227: the objects the routines below read and
+ * write (pr-mdc's md.h defines them; so does this library, for callers
+ * without pr-mdc) */
+extern calc_control ctl;
+extern cell_parameters cell;
+extern system_property sys;
+extern pressure_temperature_set pt;
+extern ion_property ion;
+
+/*
+ * The force routines.  Each reads the positions cell.r*, the species
+ * sys.ion, the charges ion.Z, the cell matrices cell.hm and cell.hmr (and
+ * cell.g, cell.vol for ewald2) as pr-mdc has set them, and adds to the
+ * forces cell.f* what pr-mdc's routine adds.
+ *
+ * real_space: Born-Mayer-Huggins pairs from the table sys.pe1r1/fe1r1
+ *   within sys.cutoff under the triclinic minimum image; sets sys.pot and
+ *   the virial tensor pt.virial (pr-mdc/program/real.c:11-77).
+ * ewald1: the Ewald real-space term, Mori's erfc per pair; adds to sys.pot
+ *   and pt.virial (pr-mdc/program/ewald.c:11-68).
+ * ewald2: the reciprocal-space sum over the half sphere |h|^2 <= sys.hm of
+ *   the lattice with reciprocal vectors cell.g; adds to sys.pot
+ *   (ewald.c:75-159).  sys.rcsi and sys.rsni, pr-mdc's per-vector scratch
+ *   arrays, are not written.
+ * ewald3: the self term, adds -sum Z^2 a1/sqrt(pi) to sys.pot
+ *   (ewald.c:162-173).
+ * calc_alpha: pr-mdc's alpha scan (ewald.c:178-193): 200 passes of
+ *   ewald1-3 from alpha = 0.01 in steps of 0.01, printing alpha and the
+ *   potential, then exit(0) as pr-mdc does.
+ *
+ * A routine that fails (no device, an allocation) prints once to stderr,
+ * leaves the outputs unchanged and sets the status mdg_pr_status returns.
+ */
+void real_space(void);
+void ewald1(void);
+void ewald2(void);
+void ewald3(void);
+void calc_alpha(void);
+
+/* No counterpart in pr-mdc: 0 while every routine succeeded, else the
+ * first error code (the MDG_E_* codes of mdgpu.h: 1 no device, 2 HIP
+ * error, 3 bad argument, 4 device allocation, 7 a pair inside the table's
+ * 0.5 A start or a cutoff past the table's end). */
+int mdg_pr_status(void);
+/* No counterpart in pr-mdc: clears that status (and re-arms the one
+ * message per failure). */
+void mdg_pr_clear_status(void);
+
+/* No counterpart in pr-mdc: how many partner splits the pair routines and
+ * how many reciprocal-vector splits ewald2's back-projection spread each
+ * 64-ion chunk over (0 = chosen from the size, the default).  With 1, each
+ * ion's force is summed in pr-mdc's order.  Returns 0 or MDG_E_ARG. */
+int mdg_pr_set_splits(int pair, int kspace);
+
+/*
+ * No counterpart in pr-mdc: times the device work of one force evaluation
+ * (real_space, ewald1, ewald2, ewald3) on the state the last calls left on
+ * the device, `reps` times, inputs resident (no host copies).  Writes the
+ * average milliseconds per evaluation to *ms_eval and, when `ms_kernel` is
+ * not NULL, the average per routine in the order above (4 values).  The
+ * device force buffer holds garbage afterwards; the next routine call
+ * uploads the forces again.  Returns 0 or an error code.
+ */
+int mdg_pr_profile(int reps, double *ms_eval, double *ms_kernel);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,157 @@
+"""libmdgpu_pr.so -- pr-mdc's force routines on the GPU -- against the
+oracle's restatement of pr-mdc (oracle/pr_oracle.c, pinned bit for bit to
+pr-mdc's compiled sources) and against pr-mdc's own program.
+
+Every routine is called through the C ABI over pr-mdc's globals (bound to
+numpy arrays), on NaCl systems in cubic and sheared (triclinic) cells of
+64 to 1728 ions.  The bar is the plain FP64 one: forces, energy and the
+virial tensor within 1e-12 of the oracle relative to their largest
+magnitude; with one split the table routine's forces are bit-identical."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import prmdgpu
+import prref
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+REF = os.path.join(ROOT, "oracle", "_ref")
+
+SYSTEMS = [(2, (0.0, 0.0, 0.0), 1), (2, (0.1, 0.05, -0.08), 2), (3, (-0.2, 0.15, 0.1), 3),
+           (4, (0.15, -0.1, 0.2), 4), (6, (0.05, 0.1, -0.12), 5)]
+
+
+def rel(a, b):
+    return float(np.abs(np.asarray(a) - np.asarray(b)).max() / max(np.abs(b).max(), 1e-300))
+
+
+def gpu_run(s, routines):
+    """Binds s into libmdgpu_pr's globals, runs the routines, returns
+    (forces, pot, virial)."""
+    L = prmdgpu.lib()
+    g = s.f.copy()
+    keep = s.f
+    s.f = g
+    s.virial.a[:] = 0.0
+    s.pot = 0.0
+    s.bind(L)
+    for n in routines:
+        print(f"prmdc {n} N={s.N}", flush=True)
+        prmdgpu.call(n)
+    pot = s.read(L)
+    vir = s.virial.a.copy()
+    s.f = keep
+    return g, pot, vir
+
+
+@pytest.fixture(autouse=True)
+def auto_splits():
+    prmdgpu.set_splits(0, 0)
+    yield
+    prmdgpu.set_splits(0, 0)
+
+
+@pytest.mark.parametrize("n,shear,seed", SYSTEMS)
+def test_each_routine_matches_oracle(gpu, n, shear, seed):
+    s = prref.pr_system(n=n, shear=shear, seed=seed)
+    done = []
+    for name in prref.ROUTINES:
+        done.append(name)
+        f, pot, vir = gpu_run(s, done)
+        f0, pot0, vir0 = prref.oracle_run(s, done)
+        assert rel(f, f0) < 1e-12, (name, rel(f, f0))
+        assert abs(pot - pot0) <= 1e-12 * abs(pot0), name
+        assert rel(vir, vir0) < 1e-12, name
+        assert np.array_equal(vir, vir.T), name
+
+
+@pytest.mark.parametrize("n,shear,seed", SYSTEMS[1:4])
+def test_one_split_table_forces_are_bit_identical(gpu, n, shear, seed):
+    s = prref.pr_system(n=n, shear=shear, seed=seed)
+    prmdgpu.set_splits(1, 1)
+    f, pot, vir = gpu_run(s, ("real_space",))
+    f0, pot0, vir0 = prref.oracle_run(s, ("real_space",))
+    assert np.array_equal(f, f0)
+    assert abs(pot - pot0) <= 1e-13 * abs(pot0)
+    f, pot, vir = gpu_run(s, prref.ROUTINES)
+    f1, _, _ = prref.oracle_run(s, prref.ROUTINES)
+    assert rel(f, f1) < 1e-12
+
+
+def test_splits_do_not_change_results(gpu):
+    s = prref.pr_system(n=4, shear=(0.1, 0.05, -0.08), seed=8)
+    ref = gpu_run(s, prref.ROUTINES)
+    for pair, ks in ((1, 1), (3, 7), (64, 309)):
+        prmdgpu.set_splits(pair, ks)
+        out = gpu_run(s, prref.ROUTINES)
+        assert rel(out[0], ref[0]) < 1e-13, (pair, ks)
+        assert abs(out[1] - ref[1]) <= 1e-13 * abs(ref[1])
+
+
+def test_reciprocal_cutoffs_and_alpha(gpu):
+    """Other hm and alpha than pr-mdc's defaults, and the skip of hm = 0."""
+    s = prref.pr_system(n=3, shear=(0.1, -0.05, 0.08), seed=6)
+    for a1, hmax in ((0.3, 50), (0.9, 100), (0.5, 0)):
+        s.a1, s.hmax = a1, hmax
+        f, pot, vir = gpu_run(s, ("ewald1", "ewald2", "ewald3"))
+        f0, pot0, vir0 = prref.oracle_run(s, ("ewald1", "ewald2", "ewald3"))
+        assert rel(f, f0) < 1e-12 and abs(pot - pot0) <= 1e-12 * abs(pot0), (a1, hmax)
+
+
+def test_empty_system(gpu):
+    s = prref.pr_system(n=2)
+    s.N = 0
+    f, pot, vir = gpu_run(s, prref.ROUTINES)
+    assert pot == 0.0 and not vir.any()
+
+
+def test_pair_inside_table_start_is_an_error(gpu):
+    s = prref.pr_system(n=2, seed=3)
+    s.r[:, 1] = s.r[:, 0] + np.array([0.3, 0.0, 0.0])   # 0.3 A apart: before the table's 0.5 A
+    with pytest.raises(prmdgpu.MdgPrError) as e:
+        gpu_run(s, ("real_space",))
+    assert e.value.code == prmdgpu.E_RANGE
+    assert prmdgpu.lib().mdg_pr_status() == 0          # cleared by the raise
+    gpu_run(prref.pr_system(n=2), ("real_space",))      # and the library still works
+
+
+def test_profile_times_resident_evaluation(gpu):
+    s = prref.pr_system(n=4, shear=(0.1, 0.05, -0.08), seed=8)
+    gpu_run(s, prref.ROUTINES)
+    ms, per = prmdgpu.profile(5)
+    assert ms > 0 and len(per) == 4 and all(p > 0 for p in per)
+    assert ms == pytest.approx(sum(per), rel=1e-9)
+
+
+def run(prog, *args):
+    p = subprocess.run([os.path.join(REF, prog), *map(str, args)], capture_output=True, text=True,
+                       timeout=120)
+    assert p.returncode == 0, (prog, p.stderr[-2000:])
+    rows = [ln.split() for ln in p.stdout.splitlines() if ln.strip()]
+    steps = np.array([[float(x) for x in r[1:]] for r in rows if r[0].isdigit()])
+    final = np.array([float(x) for x in rows[-1][1:]]) if rows[-1][0] == "final" else None
+    return steps, final, p.stderr
+
+
+@pytest.mark.skipif(not all(os.path.exists(os.path.join(REF, p)) for p in ("prmdc_cpu",
+                                                                           "prmdc_gpu")),
+                    reason="oracle/_ref pr-mdc programs not built (no reference checkout)")
+@pytest.mark.parametrize("n,steps,shear", [(2, 20, (0.1, 0.05, -0.08)), (3, 20, (0.0, 0.0, 0.0)),
+                                           (3, 15, (-0.2, 0.15, 0.1))])
+def test_prmdc_program_linked_to_libmdgpu_pr_matches_prmdc(gpu, n, steps, shear):
+    """pr-mdc's NPT step loop (Parrinello-Rahman cell dynamics, triclinic
+    cell) with real.c and ewald.c replaced by libmdgpu_pr."""
+    print(f"prmdc program n={n} steps={steps} shear={shear}", flush=True)
+    a, fa, _ = run("prmdc_cpu", n, steps, 17, *shear)
+    b, fb, err = run("prmdc_gpu", n, steps, 17, *shear)
+    assert "libmdgpu_pr" not in err, err
+    assert a.shape == b.shape == (steps, 13)
+    for c in range(a.shape[1]):
+        assert rel(b[:, c], a[:, c]) < 1e-10, c
+    assert rel(fb, fa) < 1e-10
+    # the cell did move (constant pressure), and it stayed triclinic if it started so
+    assert not np.array_equal(a[0, 4:], a[-1, 4:])
