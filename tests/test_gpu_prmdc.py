@@ -37,14 +37,17 @@ def gpu_run(s, routines):
     keep = s.f
     s.f = g
     s.virial.a[:] = 0.0
-    s.pot = 0.0
+    pot0 = s.pot
     s.bind(L)
-    for n in routines:
-        print(f"prmdc {n} N={s.N}", flush=True)
-        prmdgpu.call(n)
-    pot = s.read(L)
-    vir = s.virial.a.copy()
-    s.f = keep
+    try:
+        for n in routines:
+            print(f"prmdc {n} N={s.N}", flush=True)
+            prmdgpu.call(n)
+        pot = s.read(L)
+        vir = s.virial.a.copy()
+    finally:
+        s.f, s.pot = keep, pot0
+        s.virial.a[:] = 0.0
     return g, pot, vir
 
 
