@@ -235,6 +235,74 @@ def serving(njobs, steps_scale, budget_gb, tune=True):
 COPY_PROBE_BYTES = 1 << 30
 
 
+PR_ROUTINES = ("real_space", "ewald1", "ewald2", "ewald3")
+
+
+def pr_pairs_within_cutoff(s):
+    """Unordered pairs of a pr-mdc system within its cutoff under the
+    triclinic minimum image (scaled-coordinate fold)."""
+    sc = np.linalg.solve(s.hm.a.T, s.r)
+    n, cnt = s.N, 0
+    for i0 in range(0, n, 256):
+        d = sc[:, i0:i0 + 256, None] - sc[:, None, :]
+        d -= np.where(d >= 0.5, 1.0, 0.0)
+        d += np.where(d < -0.5, 1.0, 0.0)
+        c = np.einsum("ka,kij->aij", s.hm.a, d)
+        r2 = (c * c).sum(axis=0)
+        idx = np.arange(i0, min(i0 + 256, n))[:, None] < np.arange(n)[None, :]
+        cnt += int(((r2 <= s.cutoff * s.cutoff) & idx).sum())
+    return cnt
+
+
+def prmdc_leg():
+    """pr-mdc's force routines (libmdgpu_pr: real_space, ewald1, ewald2,
+    ewald3, FP64) on NaCl in a sheared (triclinic) cell at pr-mdc's own
+    cutoff (half the smallest face distance), alpha (3.8 / that distance)
+    and hm = 27: the device-resident force evaluation, the drop-in per call
+    with its host copies, parity against the oracle, and the oracle's own
+    time on one host core."""
+    import prmdgpu
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import prref
+    rows = []
+    for n in (3, 7):
+        s = prref.pr_system(n=n, shear=(0.1, 0.05, -0.08), seed=n)
+        L = prmdgpu.lib()
+        s.f[:] = 0.0
+        s.bind(L)
+        for name in PR_ROUTINES:
+            prmdgpu.call(name)
+        f_gpu = s.f.copy()
+        s.f[:] = 0.0
+        t0 = time.perf_counter()
+        f_orc, _, _ = prref.oracle_run(s, PR_ROUTINES)
+        cpu_ms = (time.perf_counter() - t0) * 1e3
+        err = float(np.abs(f_gpu - f_orc).max() / np.abs(f_orc).max())
+        reps = 20
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            for name in PR_ROUTINES:
+                prmdgpu.call(name)
+        dropin_ms = (time.perf_counter() - t0) * 1e3 / reps
+        ms, per = prmdgpu.profile(50)
+        npairs = pr_pairs_within_cutoff(s)
+        rows.append({
+            "ions": s.N, "cells": f"{n}x{n}x{n}", "cutoff_A": s.cutoff, "alpha": s.a1,
+            "device_ms_per_force_eval": ms,
+            "device_ms_per_routine": dict(zip(PR_ROUTINES, per)),
+            "dropin_ms_per_force_eval_with_host_copies": dropin_ms,
+            "cpu_oracle_ms_per_force_eval": cpu_ms, "cpu_cores": 1,
+            "device_speedup_vs_cpu_oracle": cpu_ms / ms,
+            "pairs_within_cutoff": npairs,
+            "pair_terms_per_s_device": 2 * npairs / ((per[0] + per[1]) * 1e-3),
+            "max_rel_force_err_vs_oracle": err})
+        log(f"  pr-mdc N={s.N}: device {ms:.3f} ms/eval, drop-in {dropin_ms:.3f} ms, "
+            f"oracle {cpu_ms:.1f} ms, err {err:.1e}")
+    return {"workload": "pr-mdc NaCl (5.63 A cells), cell sheared b += 0.1 a, "
+                        "c += 0.05 a - 0.08 b, pr-mdc's cutoff/alpha, hm = 27, FP64",
+            "sizes": rows}
+
+
 def copy_probe():
     """libmdgpu's plain copy kernel (k_copy_probe) over 1 GiB, three times:
     the kernel of known traffic (1 GiB read, 1 GiB written per dispatch)
@@ -256,6 +324,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-prmdc", action="store_true", help="skip pr-mdc's force routines")
     ap.add_argument("--no-tune", action="store_true", help="built-in tiling rule only")
     ap.add_argument("--tune-file", default=None,
                     help="tuning table: loaded if it exists, else written after tuning "
@@ -419,6 +488,11 @@ def main():
         serve = serving(args.serve_jobs, args.serve_steps_scale, args.serve_budget_gb,
                         tune=not args.no_tune)
 
+    prmdc = None
+    if not (args.no_prmdc or args.only_batched):
+        log("pr-mdc force routines (libmdgpu_pr)")
+        prmdc = prmdc_leg()
+
     cpu = None
     if not (args.no_cpu or args.only_batched):
         log("CPU baseline")
@@ -446,6 +520,7 @@ def main():
         "mdc_step_loop_armed": step_loop,
         "hbm_copy_probe": probe,
         "serving": serve,
+        "prmdc_triclinic_force_path": prmdc,
         "roofline": roof,
         "cpu_baseline": cpu,
     }

@@ -15,6 +15,7 @@ import mdref
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 INC = os.path.join(ROOT, "include")
 MDC_PROGRAM = "/root/reference/mdc/program"
+PRMDC_PROGRAM = "/root/reference/pr-mdc/program"
 
 
 def declared_symbols():
@@ -189,3 +190,57 @@ def test_empty_batch_is_an_argument_error():
         mdgpu.Batch([])
     assert e.value.code == mdgpu.E_ARG
 
+
+
+def test_pr_library_exports_what_its_header_declares():
+    """libmdgpu_pr.so (pr-mdc's routines) exports exactly include/prmdc_abi.h."""
+    import prmdgpu
+    src = re.sub(r"/\*.*?\*/", "", open(os.path.join(INC, "prmdc_abi.h")).read(), flags=re.S)
+    decl = set(re.findall(r"^\s*(?:int|void)\s+(\w+)\s*\(", src, flags=re.M))
+    decl |= set(re.findall(r"^extern\s+\w+\s+(\w+);", src, flags=re.M))
+    assert {"real_space", "ewald1", "ewald2", "ewald3", "calc_alpha", "cell", "pt"} <= decl
+    out = subprocess.run(["nm", "-D", "--defined-only", prmdgpu.LIB_PATH], check=True,
+                         capture_output=True, text=True).stdout
+    exp = {ln.split()[-1] for ln in out.splitlines() if ln.strip()}
+    assert decl == exp, (sorted(decl - exp), sorted(exp - decl))
+    prmdgpu.lib()
+
+
+def _pr_layout(tmp_path, prmdc_headers):
+    exe = tmp_path / ("pr_probe_ref" if prmdc_headers else "pr_probe_abi")
+    cmd = ["gcc", "-std=c99", "-fcommon", "-o", str(exe),
+           os.path.join(ROOT, "tests", "prmdc_layout.c")]
+    cmd += ["-DUSE_PRMDC_HEADERS", "-I" + PRMDC_PROGRAM] if prmdc_headers else ["-I" + INC]
+    subprocess.run(cmd, check=True, capture_output=True)
+    return subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout
+
+
+def test_pr_ctypes_mirror_matches_header_layout(tmp_path):
+    import prmdgpu
+    lines = dict(ln.rsplit(" ", 1) for ln in _pr_layout(tmp_path, False).splitlines())
+    for cname, cls in (("calc_control", prmdgpu.CalcControl),
+                       ("cell_parameters", prmdgpu.CellParameters),
+                       ("system_property", prmdgpu.SystemProperty),
+                       ("pressure_temperature_set", prmdgpu.PressureTemperature),
+                       ("ion_property", prmdgpu.IonProperty)):
+        assert int(lines[cname]) == C.sizeof(cls), cname
+        for key, off in lines.items():
+            if key.startswith(cname + "."):
+                assert int(off) == getattr(cls, key.split(".", 1)[1]).offset, key
+
+
+@pytest.mark.skipif(not os.path.isdir(PRMDC_PROGRAM), reason="reference checkout absent")
+def test_pr_header_layout_is_prmdc_layout(tmp_path):
+    assert _pr_layout(tmp_path, False) == _pr_layout(tmp_path, True)
+
+
+def test_pr_no_device_fails_loudly():
+    import prmdgpu
+    import prref
+    if mdgpu.device_count() > 0:
+        pytest.skip("a device is present")
+    s = prref.pr_system(n=2)
+    s.bind(prmdgpu.lib())
+    with pytest.raises(prmdgpu.MdgPrError) as e:
+        prmdgpu.call("real_space")
+    assert e.value.code == prmdgpu.E_NODEVICE
