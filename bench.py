@@ -325,6 +325,8 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-prmdc", action="store_true", help="skip pr-mdc's force routines")
+    ap.add_argument("--only-prmdc", action="store_true",
+                    help="pr-mdc's force routines alone (prints their JSON object)")
     ap.add_argument("--no-tune", action="store_true", help="built-in tiling rule only")
     ap.add_argument("--tune-file", default=None,
                     help="tuning table: loaded if it exists, else written after tuning "
@@ -344,6 +346,9 @@ def main():
     if mdgpu.device_count() < 1:
         print("bench.py: no HIP device visible", file=sys.stderr)
         sys.exit(1)
+    if args.only_prmdc:
+        print(json.dumps({"prmdc_triclinic_force_path": prmdc_leg()}), flush=True)
+        return
     probe = copy_probe()
     log(f"copy probe: {probe['GBps']:.0f} GB/s")
 
