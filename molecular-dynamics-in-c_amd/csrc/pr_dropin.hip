@@ -21,7 +21,8 @@
 //
 // Pair kernels: one wave per (home chunk of 64 ions, split of the partner
 // range).  Each lane gathers its own ion's force (no atomics); partners are
-// staged 64 at a time in LDS.  Every pair is evaluated in pr-mdc's
+// staged 16 at a time in LDS, and the partner range splits at that grain so
+// that a few thousand ions still give several waves per SIMD.  Every pair is evaluated in pr-mdc's
 // orientation, d = r_lo - r_hi with lo < hi the ion indices, so both ions of
 // a pair use the same image even where a scaled component sits exactly on
 // the +-0.5 fold boundary; the higher ion subtracts as pr-mdc's j-side does.
@@ -59,8 +60,11 @@ constexpr double kPIPI = 9.869604401089358619;
 constexpr double kSQRTPI = 1.77245385090551602731;
 
 constexpr int kW = 64;            // wavefront; pair and k-force blocks are one wave
+constexpr int kPT = 16;           // partners staged per tile (the unit of a pair split)
 constexpr int kSkThreads = 256;   // k_pr_sk block
-constexpr int kTargetWaves = 2048;
+constexpr int kPairWaves = 8192;  // pair launches: aim for this many waves
+constexpr int kKWaves = 2048;     // k-force launches
+constexpr int kMinKPerSplit = 8;  // reciprocal vectors per k-force split, at least
 constexpr int kScal = 8;          // pair-kernel scalars: pot, 6 virial entries, bad pairs
 
 enum { kPrTable = 0, kPrEwald1 = 1 };
@@ -113,12 +117,12 @@ __global__ __launch_bounds__(kW) void k_pr_pairs(PrParams p, const double* __res
                                                  const double* __restrict__ Z,
                                                  double* __restrict__ f, double* __restrict__ pf,
                                                  double* __restrict__ ps) {
-  __shared__ double sx[kW], sy[kW], sz[kW];
-  __shared__ int st[kW];
+  __shared__ double sx[kPT], sy[kPT], sz[kPT];
+  __shared__ int st[kPT];
   const int N = p.N, lane = threadIdx.x;
   const int i = blockIdx.x * kW + lane;
   const bool valid = i < N;
-  const int ntile = (N + kW - 1) / kW;
+  const int ntile = (N + kPT - 1) / kPT;
   const int t0 = (int)((long long)ntile * blockIdx.y / p.S);
   const int t1 = (int)((long long)ntile * (blockIdx.y + 1) / p.S);
   double xi = 0, yi = 0, zi = 0;
@@ -138,9 +142,9 @@ __global__ __launch_bounds__(kW) void k_pr_pairs(PrParams p, const double* __res
   const double Zi = Z[ti];
   double pot = 0, v00 = 0, v01 = 0, v02 = 0, v11 = 0, v12 = 0, v22 = 0, bad = 0;
   for (int t = t0; t < t1; ++t) {
-    const int j0 = t * kW, jl = j0 + lane;
+    const int j0 = t * kPT, jl = j0 + lane;
     __syncthreads();  // the previous tile's reads are done
-    if (jl < N) {
+    if (lane < kPT && jl < N) {
       sx[lane] = r[jl];
       sy[lane] = r[N + jl];
       sz[lane] = r[2 * N + jl];
@@ -148,7 +152,7 @@ __global__ __launch_bounds__(kW) void k_pr_pairs(PrParams p, const double* __res
     }
     __syncthreads();
     if (!valid) continue;  // every lane still reaches both barriers of each tile
-    const int nj = min(kW, N - j0);
+    const int nj = min(kPT, N - j0);
     for (int k = 0; k < nj; ++k) {
       const int j = j0 + k;
       if (j == i) continue;
@@ -230,14 +234,24 @@ __global__ __launch_bounds__(256) void k_pr_fsum(int n3, int S, const double* __
   f[idx] = a;
 }
 
-// out[c] = sum over rows w = 0..nw-1 of ps[w*kScal + c], in w order (one wave)
-__global__ __launch_bounds__(kW) void k_pr_ssum(int nw, const double* __restrict__ ps,
-                                                double* __restrict__ out) {
-  const int c = threadIdx.x;
-  if (c >= kScal) return;
+// out[c] = sum over rows w = 0..nw-1 of ps[w*kScal + c]: 64 groups of
+// threads each sum every 64th row in row order, then the 64 group sums are
+// added in group order (a fixed order for a given nw).
+constexpr int kSsumThreads = 512;
+constexpr int kSsumGroups = kSsumThreads / kScal;
+__global__ __launch_bounds__(kSsumThreads) void k_pr_ssum(int nw, const double* __restrict__ ps,
+                                                          double* __restrict__ out) {
+  __shared__ double part[kSsumGroups][kScal];
+  const int c = threadIdx.x % kScal, grp = threadIdx.x / kScal;
   double a = 0.0;
-  for (int w = 0; w < nw; ++w) a += ps[(size_t)w * kScal + c];
-  out[c] = a;
+  for (int w = grp; w < nw; w += kSsumGroups) a += ps[(size_t)w * kScal + c];
+  part[grp][c] = a;
+  __syncthreads();
+  if (threadIdx.x < kScal) {
+    double t = 0.0;
+    for (int k = 0; k < kSsumGroups; ++k) t += part[k][threadIdx.x];
+    out[threadIdx.x] = t;
+  }
 }
 
 // One block per reciprocal vector h (ewald.c:96-143).  kc[k*8 + .]:
@@ -442,9 +456,9 @@ void mat9(double** m, double* o) {
     for (int j = 0; j < 3; ++j) o[3 * i + j] = m[i][j];
 }
 
-int splits_for(int nchunk, int limit, int fixed) {
+int splits_for(int nchunk, int limit, int fixed, int target) {
   if (fixed > 0) return fixed < limit ? fixed : limit;
-  int s = (kTargetWaves + nchunk - 1) / nchunk;
+  int s = (target + nchunk - 1) / nchunk;
   if (s > limit) s = limit;
   return s < 1 ? 1 : s;
 }
@@ -541,7 +555,7 @@ int enqueue_pairs(int mode, const PrParams& p) {
                        g.pf, g.f);
     PR_HIP(hipGetLastError());
   }
-  hipLaunchKernelGGL(k_pr_ssum, dim3(1), dim3(kW), 0, g.s, nchunk * p.S, g.ps, g.out);
+  hipLaunchKernelGGL(k_pr_ssum, dim3(1), dim3(kSsumThreads), 0, g.s, nchunk * p.S, g.ps, g.out);
   PR_HIP(hipGetLastError());
   return MDG_OK;
 }
@@ -583,8 +597,8 @@ PrParams pair_params() {
   p.a1_sqrtpi = sys.a1 / kSQRTPI;
   mat9(cell.hm, p.H);
   mat9(cell.hmr, p.R);
-  const int ntile = (p.N + kW - 1) / kW;
-  p.S = splits_for(ntile, ntile, g.pair_splits);
+  const int nchunk = (p.N + kW - 1) / kW, ntile = (p.N + kPT - 1) / kPT;
+  p.S = splits_for(nchunk, ntile, g.pair_splits, kPairWaves);
   return p;
 }
 
@@ -626,7 +640,9 @@ PrK k_params() {
   p.v2 = 2.0 / cell.vol;
   p.vp = 1.0 / (kPI * cell.vol);
   const int nchunk = (p.N + kW - 1) / kW;
-  p.S = splits_for(nchunk, p.K > 0 ? p.K : 1, g.k_splits);
+  const int klimit = p.K / kMinKPerSplit > 0 ? p.K / kMinKPerSplit : 1;
+  p.S = g.k_splits > 0 ? splits_for(nchunk, p.K > 0 ? p.K : 1, g.k_splits, kKWaves)
+                       : splits_for(nchunk, klimit, 0, kKWaves);
   return p;
 }
 
