@@ -39,7 +39,6 @@
 #include <vector>
 
 #include "mdgpu.h"
-#include "mori_erfc.h"
 #include "prmdc_abi.h"
 
 // The globals of pr-mdc/program/md.h:219-227 that the routines use.  A
@@ -86,6 +85,36 @@ __device__ __forceinline__ double wsum(double v) {
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, kW);
   return v;
+}
+
+// Mori's erfc as pr-mdc's erfcc evaluates it (pr-mdc/program/ext.c:73-104,
+// the same constants as mdc's):
+//   erfc(x) ~ (x/pi) e^{-x^2} S(x^2) - 2/(e^{4 pi x} - 1),
+//   S(y) = sum_{n=0..13} w_n/(c_n + y), c_n = n^2/4, w_0 = 1/2, w_n = e^{-n^2/4}.
+// pr-mdc divides once per term; here S is carried as one fraction
+// num/den (num <- num (c_n + y) + w_n den, den <- den (c_n + y)), which is
+// the same sum with one division instead of fourteen.  The result differs
+// from the term-by-term sum by rounding only (a few 1e-16 relative); FP64
+// division is a ten-instruction sequence on gfx950, and this is the bulk of
+// the per-pair work of ewald1.
+__device__ __forceinline__ double pr_erfc(double x) {
+  const double w[13] = {
+      0.7788007830714049,    0.3678794411714423,    0.1053992245618643,
+      0.01831563888873417,   0.001930454136227709,  0.0001234098040866795,
+      4.785117392129008e-6,  1.125351747192591e-7,  1.605228055185611e-9,
+      1.388794386496401e-11, 7.287724095819693e-14, 2.319522830243569e-16,
+      4.4777324417183e-19};
+  const double y = x * x;
+  double num = 0.5, den = y;
+#pragma unroll
+  for (int n = 1; n <= 13; ++n) {
+    const double d = 0.25 * (double)(n * n) + y;
+    num = num * d + w[n - 1] * den;
+    den = den * d;
+  }
+  const double v = -2.0 / (-1.0 + exp(12.56637061435917 * x)) +
+                   (0.3183098861837907 * x * (num / den)) / exp(y);
+  return x >= 0.0 ? v : 2.0 - v;
 }
 
 // pr-mdc's cyclic_bc_dr (cell.c:305-329): to scaled coordinates with
@@ -187,7 +216,7 @@ __global__ __launch_bounds__(kW) void k_pr_pairs(PrParams p, const double* __res
         dr = sqrt(dr);
         const double ZZ = lo ? Zi * Z[st[k]] : Z[st[k]] * Zi;
         const double adr = p.a1 * dr;
-        const double e = mori_erfc(adr) / dr;
+        const double e = pr_erfc(adr) / dr;
         if (lo) pot += ZZ * e;
         fij = ZZ * (2.0 * p.a1_sqrtpi * exp(-adr * adr) + e) / (dr * dr);
       }
