@@ -253,28 +253,39 @@ __global__ __launch_bounds__(kW) void k_pr_pairs(PrParams p, const double* __res
   }
 }
 
-// f[c*N + i] += sum over splits s = 0..S-1 of pf[(s*3 + c)*N + i], in s order
+// sum_{k < n} p[k*stride] with eight independent accumulators (k mod 8)
+// combined in a fixed tree: the loads overlap instead of waiting on one add
+// chain, and the order is fixed for a given n.
+__device__ __forceinline__ double sum_strided(const double* __restrict__ p, int n, size_t stride) {
+  double a[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  int k = 0;
+  for (; k + 8 <= n; k += 8) {
+#pragma unroll
+    for (int u = 0; u < 8; ++u) a[u] += p[(size_t)(k + u) * stride];
+  }
+  for (int u = 0; k < n; ++k, ++u) a[u] += p[(size_t)k * stride];
+  return ((a[0] + a[1]) + (a[2] + a[3])) + ((a[4] + a[5]) + (a[6] + a[7]));
+}
+
+// f[c*N + i] += the sum over splits s of pf[(s*3 + c)*N + i]
 __global__ __launch_bounds__(256) void k_pr_fsum(int n3, int S, const double* __restrict__ pf,
                                                  double* __restrict__ f) {
   const int idx = blockIdx.x * 256 + threadIdx.x;
   if (idx >= n3) return;
-  double a = f[idx];
-  for (int s = 0; s < S; ++s) a += pf[(size_t)s * n3 + idx];
-  f[idx] = a;
+  f[idx] = f[idx] + sum_strided(pf + idx, S, (size_t)n3);
 }
 
 // out[c] = sum over rows w = 0..nw-1 of ps[w*kScal + c]: 64 groups of
-// threads each sum every 64th row in row order, then the 64 group sums are
-// added in group order (a fixed order for a given nw).
+// threads each sum every 64th row, then the 64 group sums are added in
+// group order (a fixed order for a given nw).
 constexpr int kSsumThreads = 512;
 constexpr int kSsumGroups = kSsumThreads / kScal;
 __global__ __launch_bounds__(kSsumThreads) void k_pr_ssum(int nw, const double* __restrict__ ps,
                                                           double* __restrict__ out) {
   __shared__ double part[kSsumGroups][kScal];
   const int c = threadIdx.x % kScal, grp = threadIdx.x / kScal;
-  double a = 0.0;
-  for (int w = grp; w < nw; w += kSsumGroups) a += ps[(size_t)w * kScal + c];
-  part[grp][c] = a;
+  const int rows = nw > grp ? (nw - grp + kSsumGroups - 1) / kSsumGroups : 0;
+  part[grp][c] = sum_strided(ps + (size_t)grp * kScal + c, rows, (size_t)kSsumGroups * kScal);
   __syncthreads();
   if (threadIdx.x < kScal) {
     double t = 0.0;
@@ -374,13 +385,20 @@ __global__ __launch_bounds__(kW) void k_pr_kforce(PrK p, const double* __restric
   o[2 * N + i] = fz;
 }
 
-// out[0] = sum over k of the vectors' energies, in k order (ewald.c:141-157)
-__global__ __launch_bounds__(kW) void k_pr_ksum(int K, const double* __restrict__ kc,
-                                                double* __restrict__ out) {
-  if (threadIdx.x != 0) return;
-  double a = 0.0;
-  for (int k = 0; k < K; ++k) a += kc[(size_t)k * 8 + 6];
-  out[0] = a;
+// out[0] = sum over k of the vectors' energies (ewald.c:141-157): thread t
+// sums every 256th vector from t, then the 256 sums are added in order.
+__global__ __launch_bounds__(256) void k_pr_ksum(int K, const double* __restrict__ kc,
+                                                 double* __restrict__ out) {
+  __shared__ double part[256];
+  const int t = threadIdx.x;
+  const int n = K > t ? (K - t + 255) / 256 : 0;
+  part[t] = sum_strided(kc + (size_t)t * 8 + 6, n, (size_t)256 * 8);
+  __syncthreads();
+  if (t == 0) {
+    double a = 0.0;
+    for (int k = 0; k < 256; ++k) a += part[k];
+    out[0] = a;
+  }
 }
 
 // out[0] = -sum_i Z_i^2 * a1/sqrt(pi) (ewald.c:162-173)
@@ -604,7 +622,7 @@ int enqueue_kspace(const PrK& p) {
                        g.pf, g.f);
     PR_HIP(hipGetLastError());
   }
-  hipLaunchKernelGGL(k_pr_ksum, dim3(1), dim3(kW), 0, g.s, p.K, g.kc, g.out + kOutK);
+  hipLaunchKernelGGL(k_pr_ksum, dim3(1), dim3(256), 0, g.s, p.K, g.kc, g.out + kOutK);
   PR_HIP(hipGetLastError());
   return MDG_OK;
 }
