@@ -7,7 +7,7 @@
  * -lmdgpu_pr; pr-mdc's rv.c (calc_foc), cell.c (calc_sigma, set_cutoff,
  * cell_force, cyclic_bc_dr), pt.c (control_press_PR) and the rest stay its
  * own.  The structs below are laid out field for field as pr-mdc's
- * (pr-mdc/program/md.h:23-191); pr-mdc's md.h defines the objects, and
+ * (pr-mdc/program/md.h:25-163); pr-mdc's md.h defines the objects, and
  * ELF interposition binds this library's references to them.
  *
  * This header and mdc_abi.h name different structs with the same names
@@ -31,7 +31,7 @@ typedef struct {
   int kinds_of_interaction;
 } calc_control;
 
-/* pr-mdc/program/md.h:45-84.  The 3x3 matrices are pr-mdc's dmat2d
+/* pr-mdc/program/md.h:45-85.  The 3x3 matrices are pr-mdc's dmat2d
  * row-pointer arrays (init.c:240-253): m[i][j]. */
 typedef struct {
   int nx, ny, nz;
@@ -51,7 +51,7 @@ typedef struct {
   double angle_ab, angle_bc, angle_ca;
 } cell_parameters;
 
-/* pr-mdc/program/md.h:86-138 */
+/* pr-mdc/program/md.h:87-140 */
 typedef struct {
   int *ion;
   int N;
@@ -72,7 +72,7 @@ typedef struct {
   int *lookup;
 } system_property;
 
-/* pr-mdc/program/md.h:140-155 */
+/* pr-mdc/program/md.h:142-158 */
 typedef struct {
   int t_control_step;
   int p_control_step;
@@ -86,13 +86,13 @@ typedef struct {
   double **kin_tensor;
 } pressure_temperature_set;
 
-/* pr-mdc/program/md.h:157-160 */
+/* pr-mdc/program/md.h:160-163 */
 typedef struct {
   double *m;
   double *Z;
 } ion_property;
 
-/* pr-mdc/program/md.h:219-227: the objects the routines below read and
+/* pr-mdc/program/md.h:224-229: the objects the routines below read and
  * write (pr-mdc's md.h defines them; so does this library, for callers
  * without pr-mdc) */
 extern calc_control ctl;

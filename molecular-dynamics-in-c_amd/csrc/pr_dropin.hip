@@ -41,7 +41,7 @@
 #include "mdgpu.h"
 #include "prmdc_abi.h"
 
-// The globals of pr-mdc/program/md.h:219-227 that the routines use.  A
+// The globals of pr-mdc/program/md.h:224-229 that the routines use.  A
 // pr-mdc executable defines its own (md.h defines them in the header); ELF
 // interposition then binds these references to the executable's objects.
 calc_control ctl;

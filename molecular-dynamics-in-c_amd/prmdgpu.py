@@ -31,7 +31,7 @@ class CalcControl(C.Structure):  # md.h:25-43
                 ("natoms_in_unit_cell", C.c_int), ("kinds_of_interaction", C.c_int)]
 
 
-class CellParameters(C.Structure):  # md.h:45-84
+class CellParameters(C.Structure):  # md.h:45-85
     _fields_ = ([("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int), ("vol", C.c_double)] +
                 [(n, PD) for n in ("rx", "ry", "rz", "sx", "sy", "sz", "vx", "vy", "vz",
                                    "fx0", "fy0", "fz0", "fx", "fy", "fz")] +
@@ -41,7 +41,7 @@ class CellParameters(C.Structure):  # md.h:45-84
                                            "angle_ca")])
 
 
-class SystemProperty(C.Structure):  # md.h:86-138
+class SystemProperty(C.Structure):  # md.h:87-140
     _fields_ = [("ion", PI), ("N", C.c_int), ("step", C.c_int), ("dt", C.c_double),
                 ("dt2", C.c_double), ("kB", C.c_double), ("kk", C.c_double), ("pot", C.c_double),
                 ("kin", C.c_double), ("perMol", C.c_double), ("e2t", C.c_double),
@@ -51,14 +51,14 @@ class SystemProperty(C.Structure):  # md.h:86-138
                 ("pe1r1", PD), ("fe1r1", PD), ("lookup", PI)]
 
 
-class PressureTemperature(C.Structure):  # md.h:140-155
+class PressureTemperature(C.Structure):  # md.h:142-158
     _fields_ = ([("t_control_step", C.c_int), ("p_control_step", C.c_int)] +
                 [(n, C.c_double) for n in ("pres_scalling_factor", "temp_set", "temp_set_initial",
                                            "pres_trace")] +
                 [(n, PPD) for n in ("pres_set", "pres_tensor", "virial", "kin_tensor")])
 
 
-class IonProperty(C.Structure):  # md.h:157-160
+class IonProperty(C.Structure):  # md.h:160-163
     _fields_ = [("m", PD), ("Z", PD)]
 
 
