@@ -92,7 +92,16 @@ typedef struct {
   double *Z;
 } ion_property;
 
-/* pr-mdc/program/md.h:224-229: the objects the routines below read and
+/* pr-mdc/program/md.h:192-198 */
+typedef struct {
+  double mass;
+  double dt;
+  double **v;
+  double **f;
+  double **f0;
+} Parrinello_Rahman_set;
+
+/* pr-mdc/program/md.h:224-234: the objects the routines below read and
  * write (pr-mdc's md.h defines them; so does this library, for callers
  * without pr-mdc) */
 extern calc_control ctl;
@@ -100,6 +109,7 @@ extern cell_parameters cell;
 extern system_property sys;
 extern pressure_temperature_set pt;
 extern ion_property ion;
+extern Parrinello_Rahman_set pr;
 
 /*
  * The force routines.  Each reads the positions cell.r*, the species
@@ -156,6 +166,27 @@ int mdg_pr_set_splits(int pair, int kspace);
  * uploads the forces again.  Returns 0 or an error code.
  */
 int mdg_pr_profile(int reps, double *ms_eval, double *ms_kernel);
+
+/*
+ * No counterpart as one routine: `nsteps` steps of pr-mdc's step loop
+ * (pr-mdc/program/main.c:100-111) with the state resident on the device --
+ * next_rv_verlet (rv.c:12-17: clear_foc, calc_sigma, set_cutoff,
+ * real_space, cell_force, ewald1-3, next_v, next_r with cyclic_bc),
+ * control_press_PR (pt.c:27-63, Parrinello-Rahman cell dynamics and
+ * calc_cell), control_temp(pt.t_control_step, pt.temp_set) (pt.c:124-150),
+ * calc_kin and calc_press -- reading pr-mdc's globals at the start
+ * (including pr, the cell's mass, time step, velocity and forces) and
+ * writing back at the end what those routines write: positions, scaled
+ * positions, velocities, forces and old forces, the cell matrices, volume,
+ * edge lengths and angles, pr's velocity and forces, the pressure, virial
+ * and kinetic tensors, pot, kin, a1, the cutoff and sys.step.  MSD and file
+ * output, which pr-mdc's loop also runs, are not part of it; the
+ * thermostat's running average starts afresh at each call.  When `record`
+ * is not NULL it receives per step {pot, kin, pres_trace, vol, hm[3][3]}
+ * (13 doubles).  Returns 0, MDG_E_RANGE (a pair inside the table's start),
+ * MDG_E_STATE (a singular cell matrix, where pr-mdc exits) or another code.
+ */
+int mdg_pr_run(int nsteps, double *record);
 
 #ifdef __cplusplus
 }

@@ -49,6 +49,7 @@ cell_parameters cell;
 system_property sys;
 pressure_temperature_set pt;
 ion_property ion;
+Parrinello_Rahman_set pr;
 
 namespace {
 
@@ -70,16 +71,45 @@ enum { kPrTable = 0, kPrEwald1 = 1 };
 
 struct PrParams {
   int N, kinds, td, S;
-  double cutoff2, a1, a1_sqrtpi;
-  double H[9];  // cell.hm[i][j] at 3i+j
-  double R[9];  // cell.hmr[i][j] at 3i+j
 };
 
 struct PrK {
   int N, K, S;
-  double g[9];  // cell.g[i][j] at 3i+j
-  double a1, v2, vp;
 };
+
+// The cell and the scalars of a pr-mdc system, resident on the device: what
+// the force kernels read (the drop-in routines upload it from pr-mdc's
+// globals per call) and what mdg_pr_run's step loop updates in place.
+// Matrices are pr-mdc's m[i][j] at 3i+j.
+struct PrCell {
+  double H[9], R[9], sig[9], G[9];         // cell.hm, hmr, sigma, g
+  double vol, cutoff, cutoff2, a1, a1_sqrtpi, v2, vp;
+  double Lx, Ly, Lz, ang_ab, ang_bc, ang_ca;
+  double prv[9], prf[9], prf0[9], prmass, prdt;  // Parrinello-Rahman cell velocity, forces
+  double HmF[9];                           // cell_force's (H^T)^-1 (dH/dt)^T
+  double pset[9], ptens[9], vir[9], kint[9];
+  double pot, kin, ptrace;
+  double dt, dt2, e2t, temp_set, oldk, vscale;
+  int step, count, t_step, skip_ewald, bad_cell, pad;
+};
+
+// what the pair kernels read of the cell
+struct Geo {
+  double H[9], R[9], cutoff2, a1, a1_sqrtpi;
+};
+
+__device__ __forceinline__ Geo load_geo(const PrCell* __restrict__ c) {
+  Geo q;
+#pragma unroll
+  for (int k = 0; k < 9; ++k) {
+    q.H[k] = c->H[k];
+    q.R[k] = c->R[k];
+  }
+  q.cutoff2 = c->cutoff2;
+  q.a1 = c->a1;
+  q.a1_sqrtpi = c->a1_sqrtpi;
+  return q;
+}
 
 __device__ __forceinline__ double wsum(double v) {
 #pragma unroll
@@ -119,7 +149,7 @@ __device__ __forceinline__ double pr_erfc(double x) {
 
 // pr-mdc's cyclic_bc_dr (cell.c:305-329): to scaled coordinates with
 // Hm^-1, fold each component into [-0.5, 0.5) ('>=' then '<'), back with Hm.
-__device__ __forceinline__ void min_image(const PrParams& p, double& dx, double& dy, double& dz) {
+__device__ __forceinline__ void min_image(const Geo& p, double& dx, double& dy, double& dz) {
   double sdx = p.R[0] * dx + p.R[3] * dy + p.R[6] * dz;
   double sdy = p.R[1] * dx + p.R[4] * dy + p.R[7] * dz;
   double sdz = p.R[2] * dx + p.R[5] * dy + p.R[8] * dz;
@@ -139,7 +169,8 @@ __device__ __forceinline__ void min_image(const PrParams& p, double& dx, double&
 // force goes to pf[(split*3 + c)*N + i] and k_pr_fsum adds them.
 // ps[(split*gridDim.x + chunk)*kScal + c]: the wave's pot, virial, bad pairs.
 template <int MODE>
-__global__ __launch_bounds__(kW) void k_pr_pairs(PrParams p, const double* __restrict__ r,
+__global__ __launch_bounds__(kW) void k_pr_pairs(PrParams p, const PrCell* __restrict__ cell_,
+                                                 const double* __restrict__ r,
                                                  const int* __restrict__ typ,
                                                  const double2* __restrict__ tab,
                                                  const int* __restrict__ lookup,
@@ -151,6 +182,7 @@ __global__ __launch_bounds__(kW) void k_pr_pairs(PrParams p, const double* __res
   const int N = p.N, lane = threadIdx.x;
   const int i = blockIdx.x * kW + lane;
   const bool valid = i < N;
+  const Geo q = load_geo(cell_);
   const int ntile = (N + kPT - 1) / kPT;
   const int t0 = (int)((long long)ntile * blockIdx.y / p.S);
   const int t1 = (int)((long long)ntile * (blockIdx.y + 1) / p.S);
@@ -196,9 +228,9 @@ __global__ __launch_bounds__(kW) void k_pr_pairs(PrParams p, const double* __res
         dy = sy[k] - yi;
         dz = sz[k] - zi;
       }
-      min_image(p, dx, dy, dz);
+      min_image(q, dx, dy, dz);
       double dr = dx * dx + dy * dy + dz * dz;
-      if (dr > p.cutoff2) continue;
+      if (dr > q.cutoff2) continue;
       double fij;
       if constexpr (MODE == kPrTable) {
         // real.c:43-56: the 0.001 A bin of r from 0.5 A
@@ -215,10 +247,10 @@ __global__ __launch_bounds__(kW) void k_pr_pairs(PrParams p, const double* __res
         // ewald.c:32-43
         dr = sqrt(dr);
         const double ZZ = lo ? Zi * Z[st[k]] : Z[st[k]] * Zi;
-        const double adr = p.a1 * dr;
+        const double adr = q.a1 * dr;
         const double e = pr_erfc(adr) / dr;
         if (lo) pot += ZZ * e;
-        fij = ZZ * (2.0 * p.a1_sqrtpi * exp(-adr * adr) + e) / (dr * dr);
+        fij = ZZ * (2.0 * q.a1_sqrtpi * exp(-adr * adr) + e) / (dr * dr);
       }
       if (lo) {
         fx += fij * dx;
@@ -297,7 +329,8 @@ __global__ __launch_bounds__(kSsumThreads) void k_pr_ssum(int nw, const double* 
 // One block per reciprocal vector h (ewald.c:96-143).  kc[k*8 + .]:
 // {2C, 2S, v2 e/|h|^2, hhx, hhy, hhz, vp e/|h|^2 ((2C)^2 + (2S)^2), 0}
 // with C, S = sum_j Z_j (cos, sin)(2 pi hh . r_j).
-__global__ __launch_bounds__(kSkThreads) void k_pr_sk(PrK p, const int4* __restrict__ hv,
+__global__ __launch_bounds__(kSkThreads) void k_pr_sk(PrK p, const PrCell* __restrict__ cl,
+                                                      const int4* __restrict__ hv,
                                                       const double* __restrict__ r,
                                                       const int* __restrict__ typ,
                                                       const double* __restrict__ Z,
@@ -306,9 +339,10 @@ __global__ __launch_bounds__(kSkThreads) void k_pr_sk(PrK p, const int4* __restr
   const int k = blockIdx.x, N = p.N, tid = threadIdx.x;
   const int4 h = hv[k];
   const double hx = (double)h.x, hy = (double)h.y, hz = (double)h.z;
-  const double hhx = hx * p.g[0] + hy * p.g[3] + hz * p.g[6];
-  const double hhy = hx * p.g[1] + hy * p.g[4] + hz * p.g[7];
-  const double hhz = hx * p.g[2] + hy * p.g[5] + hz * p.g[8];
+  const double* G = cl->G;
+  const double hhx = hx * G[0] + hy * G[3] + hz * G[6];
+  const double hhy = hx * G[1] + hy * G[4] + hz * G[7];
+  const double hhz = hx * G[2] + hy * G[5] + hz * G[8];
   const double hhx1 = kPI2 * hhx, hhy1 = kPI2 * hhy, hhz1 = kPI2 * hhz;
   double c = 0.0, s = 0.0;
   for (int j = tid; j < N; j += kSkThreads) {
@@ -334,8 +368,8 @@ __global__ __launch_bounds__(kSkThreads) void k_pr_sk(PrK p, const int4* __restr
       s += red[1][w];
     }
     const double hh2 = hhx * hhx + hhy * hhy + hhz * hhz;
-    const double ex = exp(-hh2 * (kPIPI / (p.a1 * p.a1)));
-    const double vpe = p.vp * (ex / hh2), v2e = p.v2 * (ex / hh2);
+    const double ex = exp(-hh2 * (kPIPI / (cl->a1 * cl->a1)));
+    const double vpe = cl->vp * (ex / hh2), v2e = cl->v2 * (ex / hh2);
     const double C2 = c * 2.0, S2 = s * 2.0;
     double* o = kc + (size_t)k * 8;
     o[0] = C2;
@@ -402,7 +436,7 @@ __global__ __launch_bounds__(256) void k_pr_ksum(int K, const double* __restrict
 }
 
 // out[0] = -sum_i Z_i^2 * a1/sqrt(pi) (ewald.c:162-173)
-__global__ __launch_bounds__(256) void k_pr_self(int N, double a1_sqrtpi,
+__global__ __launch_bounds__(256) void k_pr_self(int N, const PrCell* __restrict__ c,
                                                  const int* __restrict__ typ,
                                                  const double* __restrict__ Z,
                                                  double* __restrict__ out) {
@@ -418,13 +452,295 @@ __global__ __launch_bounds__(256) void k_pr_self(int N, double a1_sqrtpi,
   if (threadIdx.x == 0) {
     double z2 = 0.0;
     for (int w = 0; w < 256 / kW; ++w) z2 += red[w];
-    out[0] = -z2 * a1_sqrtpi;
+    out[0] = -z2 * c->a1_sqrtpi;
+  }
+}
+
+// ---------------------------------------------------------------- step loop
+// pr-mdc's step (main.c:100-111 without output and MSD), resident:
+// next_rv_verlet (rv.c:12-17: calc_foc, next_v, next_r), control_press_PR
+// (pt.c:27-63), control_temp (pt.c:124-150), calc_kin (rv.c:68-105),
+// calc_press (pt.c:8-24).  The cell matrices and scalars live in PrCell and
+// are updated by single-thread kernels; per-ion work runs one lane per ion.
+
+// out slots: [0, kScal) pair sums (real space, or the drop-in's pair
+// routine), kOutK reciprocal energy, kOutSelf self term, [kOutE1, +kScal)
+// ewald1's pair sums in the step loop
+enum { kOutK = kScal, kOutSelf = kScal + 1, kOutE1 = 16, kOutLen = 32 };
+constexpr int kRec = 13;  // per-step record: pot, kin, pressure, volume, H
+
+// cofactor inverse as pr-mdc writes it (cell.c:228-240, 80-90):
+// o = M^-1 with m[i][j] at 3i+j; returns det M
+__device__ double cof_inverse(const double* M, double* o) {
+  const double det = M[0] * (M[4] * M[8] - M[7] * M[5]) + M[3] * (M[7] * M[2] - M[1] * M[8]) +
+                     M[6] * (M[1] * M[5] - M[4] * M[2]);
+  o[0] = (-(M[5] * M[7]) + M[4] * M[8]) / det;
+  o[3] = (M[5] * M[6] - M[3] * M[8]) / det;
+  o[6] = (-(M[4] * M[6]) + M[3] * M[7]) / det;
+  o[1] = (M[2] * M[7] - M[1] * M[8]) / det;
+  o[4] = (-(M[2] * M[6]) + M[0] * M[8]) / det;
+  o[7] = (M[1] * M[6] - M[0] * M[7]) / det;
+  o[2] = (-(M[2] * M[4]) + M[1] * M[5]) / det;
+  o[5] = (M[2] * M[3] - M[0] * M[5]) / det;
+  o[8] = (-(M[1] * M[3]) + M[0] * M[4]) / det;
+  return det;
+}
+
+// calc_foc's prologue (rv.c:26-33): calc_sigma (cell.c:148-175), set_cutoff
+// (cell.c:27-51), and cell_force's (H^T)^-1 (dH/dt)^T (cell.c:58-95).
+__global__ void k_pr_cell_pre(PrCell* __restrict__ c) {
+  if (threadIdx.x != 0) return;
+  c->step += 1;
+  const double* H = c->H;
+  double* sg = c->sig;
+  sg[0] = (H[4] * H[8] - H[5] * H[7]);
+  sg[1] = (H[5] * H[6] - H[3] * H[8]);
+  sg[2] = (H[3] * H[7] - H[4] * H[6]);
+  sg[3] = (H[7] * H[2] - H[8] * H[1]);
+  sg[4] = (H[8] * H[0] - H[6] * H[2]);
+  sg[5] = (H[6] * H[1] - H[7] * H[0]);
+  sg[6] = (H[1] * H[5] - H[2] * H[4]);
+  sg[7] = (H[2] * H[3] - H[0] * H[5]);
+  sg[8] = (H[0] * H[4] - H[1] * H[3]);
+  for (int k = 0; k < 9; ++k) c->G[k] = sg[k] / c->vol;
+  const double* q = c->G;
+  const double h1 = 1.0 / sqrt(q[0] * q[0] + q[1] * q[1] + q[2] * q[2]);
+  const double h2 = 1.0 / sqrt(q[3] * q[3] + q[4] * q[4] + q[5] * q[5]);
+  const double h3 = 1.0 / sqrt(q[6] * q[6] + q[7] * q[7] + q[8] * q[8]);
+  double mn = 0.0;
+  if ((h1 <= h2) && (h1 <= h3)) mn = h1;
+  if ((h2 <= h1) && (h2 <= h3)) mn = h2;
+  if ((h3 <= h1) && (h3 <= h2)) mn = h3;
+  if (c->step <= 10) {
+    c->cutoff = mn / 2.0;
+    c->cutoff2 = c->cutoff * c->cutoff;
+    if (c->skip_ewald != 1) c->a1 = 3.8 / mn;
+  }
+  c->a1_sqrtpi = c->a1 / kSQRTPI;
+  c->v2 = 2.0 / c->vol;
+  c->vp = 1.0 / (kPI * c->vol);
+  double T[9], Tr[9], VT[9];
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) {
+      T[3 * i + j] = H[3 * j + i];
+      VT[3 * i + j] = c->prv[3 * j + i];
+    }
+  cof_inverse(T, Tr);
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) {
+      double a = 0.0;
+      for (int k = 0; k < 3; k++) a += Tr[3 * i + k] * VT[3 * k + j];
+      c->HmF[3 * i + j] = a;
+    }
+}
+
+// clear_foc (init.c:223-235): f0 <- f, f <- 0
+__global__ __launch_bounds__(256) void k_pr_clear(int n3, double* __restrict__ f,
+                                                  double* __restrict__ f0) {
+  const int k = blockIdx.x * 256 + threadIdx.x;
+  if (k >= n3) return;
+  f0[k] = f[k];
+  f[k] = 0.0;
+}
+
+// cell_force's per-ion part (cell.c:100-111)
+__global__ __launch_bounds__(256) void k_pr_cell_force(int N, const PrCell* __restrict__ c,
+                                                       const int* __restrict__ typ,
+                                                       const double* __restrict__ m,
+                                                       const double* __restrict__ v,
+                                                       double* __restrict__ f) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= N) return;
+  const double* F = c->HmF;
+  const double mass = m[typ[i]], vx = v[i], vy = v[N + i], vz = v[2 * N + i];
+  f[i] -= mass * (F[0] * vx + F[3] * vy + F[6] * vz);
+  f[N + i] -= mass * (F[1] * vx + F[4] * vy + F[7] * vz);
+  f[2 * N + i] -= mass * (F[2] * vx + F[5] * vy + F[8] * vz);
+}
+
+// the step's potential and virial tensor from the routines' sums
+// (real.c:20-25, 73-76; ewald.c:36, 50-67, 157, 172)
+__global__ void k_pr_scal_step(PrCell* __restrict__ c, const double* __restrict__ out, int has_k) {
+  if (threadIdx.x != 0) return;
+  double pot = out[0];
+  const bool ew = c->skip_ewald != 1;
+  if (ew) pot = ((pot + out[kOutE1]) + (has_k ? out[kOutK] : 0.0)) + out[kOutSelf];
+  c->pot = pot;
+  const int up[6] = {0, 1, 2, 4, 5, 8};
+  for (int k = 0; k < 6; ++k) c->vir[up[k]] = out[1 + k] + (ew ? out[kOutE1 + 1 + k] : 0.0);
+  c->vir[3] = c->vir[1];
+  c->vir[6] = c->vir[2];
+  c->vir[7] = c->vir[5];
+  if (out[7] != 0.0) c->bad_cell |= 2;
+}
+
+// cyclic_bc (cell.c:255-297): to scaled coordinates, into [0, 1), back
+__device__ __forceinline__ void wrap_ion(const PrCell* __restrict__ c, double& x, double& y,
+                                         double& z, double& sx, double& sy, double& sz) {
+  const double* Rm = c->R;
+  const double* H = c->H;
+  sx = Rm[0] * x + Rm[3] * y + Rm[6] * z;
+  sy = Rm[1] * x + Rm[4] * y + Rm[7] * z;
+  sz = Rm[2] * x + Rm[5] * y + Rm[8] * z;
+  if (sx >= 1.0) sx -= 1.0;
+  if (sy >= 1.0) sy -= 1.0;
+  if (sz >= 1.0) sz -= 1.0;
+  if (sx < 0.0) sx += 1.0;
+  if (sy < 0.0) sy += 1.0;
+  if (sz < 0.0) sz += 1.0;
+  x = H[0] * sx + H[3] * sy + H[6] * sz;
+  y = H[1] * sx + H[4] * sy + H[7] * sz;
+  z = H[2] * sx + H[5] * sy + H[8] * sz;
+}
+
+// next_v then next_r with cyclic_bc (rv.c:44-66)
+__global__ __launch_bounds__(256) void k_pr_next_vr(int N, const PrCell* __restrict__ c,
+                                                    const int* __restrict__ typ,
+                                                    const double* __restrict__ m,
+                                                    double* __restrict__ v,
+                                                    const double* __restrict__ f,
+                                                    const double* __restrict__ f0,
+                                                    double* __restrict__ r, double* __restrict__ sc) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= N) return;
+  const double m2 = 2.0 * m[typ[i]], dt = c->dt, dt2 = c->dt2;
+  double p[3], w[3];
+  for (int d = 0; d < 3; ++d) {
+    double vv = v[d * N + i];
+    vv += dt * (f[d * N + i] + f0[d * N + i]) / m2;
+    v[d * N + i] = vv;
+    p[d] = r[d * N + i] + (dt * vv + dt2 * f[d * N + i] / m2);
+  }
+  wrap_ion(c, p[0], p[1], p[2], w[0], w[1], w[2]);
+  for (int d = 0; d < 3; ++d) {
+    r[d * N + i] = p[d];
+    sc[d * N + i] = w[d];
+  }
+}
+
+// control_press_PR's cell update (pt.c:27-57) and calc_cell (cell.c:214-252)
+__global__ void k_pr_cell_post(PrCell* __restrict__ c) {
+  if (threadIdx.x != 0) return;
+  double ff[9];
+  for (int k = 0; k < 9; ++k) ff[k] = c->ptens[k] - c->pset[k];
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) {
+      c->prf0[3 * i + j] = c->prf[3 * i + j];
+      double a = 0.0;
+      for (int k = 0; k < 3; k++) a += ff[3 * i + k] * c->sig[3 * k + j];
+      c->prf[3 * i + j] = a;
+    }
+  for (int k = 0; k < 9; ++k) c->prv[k] += c->prdt * (c->prf[k] + c->prf0[k]) / (2.0 * c->prmass);
+  for (int k = 0; k < 9; ++k)
+    c->H[k] += c->prdt * c->prv[k] + (c->prdt * c->prdt) * c->prf[k] / (2.0 * c->prmass);
+  const double* H = c->H;
+  const double det = cof_inverse(H, c->R);
+  if (det == 0.0) c->bad_cell |= 1;  // pr-mdc prints and exits here
+  c->vol = fabs(det);
+  c->Lx = sqrt(H[0] * H[0] + H[1] * H[1] + H[2] * H[2]);
+  c->Ly = sqrt(H[3] * H[3] + H[4] * H[4] + H[5] * H[5]);
+  c->Lz = sqrt(H[6] * H[6] + H[7] * H[7] + H[8] * H[8]);
+  const double a_b = H[0] * H[3] + H[1] * H[4] + H[2] * H[5];
+  const double b_c = H[3] * H[6] + H[4] * H[7] + H[5] * H[8];
+  const double c_a = H[6] * H[0] + H[7] * H[1] + H[8] * H[2];
+  c->ang_ab = (acos(a_b / (c->Lx * c->Ly))) * 180.0 / kPI;
+  c->ang_bc = (acos(b_c / (c->Ly * c->Lz))) * 180.0 / kPI;
+  c->ang_ca = (acos(c_a / (c->Lz * c->Lx))) * 180.0 / kPI;
+}
+
+// control_press_PR's closing cyclic_bc of every ion (pt.c:59-61)
+__global__ __launch_bounds__(256) void k_pr_wrap(int N, const PrCell* __restrict__ c,
+                                                 double* __restrict__ r, double* __restrict__ sc) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= N) return;
+  double x = r[i], y = r[N + i], z = r[2 * N + i], a, b, d;
+  wrap_ion(c, x, y, z, a, b, d);
+  r[i] = x;
+  r[N + i] = y;
+  r[2 * N + i] = z;
+  sc[i] = a;
+  sc[N + i] = b;
+  sc[2 * N + i] = d;
+}
+
+// control_temp's bookkeeping (pt.c:124-140): the kinetic energy it averages
+// is the previous step's calc_kin
+__global__ void k_pr_temp(PrCell* __restrict__ c) {
+  if (threadIdx.x != 0) return;
+  c->count += 1;
+  c->oldk += c->kin;
+  c->vscale = 0.0;  // 0: no scaling this step
+  if (c->t_step > 0 && c->step % c->t_step == 0) {
+    const double averaged_K = c->oldk / (double)c->count;
+    c->oldk = 0.0;
+    c->count = 0;
+    const double tempK = c->temp_set / c->e2t;
+    c->vscale = sqrt(tempK / averaged_K);
+  }
+}
+
+// control_temp's scaling (pt.c:141-145), calc_kin (rv.c:68-105) and
+// calc_press (pt.c:8-24); writes the step's record row.  One block: each
+// thread sums every 256th ion, then a fixed tree.
+__global__ __launch_bounds__(256) void k_pr_kin(int N, PrCell* __restrict__ c,
+                                                const int* __restrict__ typ,
+                                                const double* __restrict__ m,
+                                                double* __restrict__ v, double* __restrict__ rec) {
+  __shared__ double part[6][256];
+  const int t = threadIdx.x;
+  const double sc = c->vscale;
+  double k6[6] = {0, 0, 0, 0, 0, 0};
+  for (int i = t; i < N; i += 256) {
+    double vx = v[i], vy = v[N + i], vz = v[2 * N + i];
+    if (sc != 0.0) {
+      vx *= sc;
+      vy *= sc;
+      vz *= sc;
+      v[i] = vx;
+      v[N + i] = vy;
+      v[2 * N + i] = vz;
+    }
+    const double mm = m[typ[i]];
+    k6[0] += mm * vx * vx;
+    k6[1] += mm * vx * vy;
+    k6[2] += mm * vx * vz;
+    k6[3] += mm * vy * vy;
+    k6[4] += mm * vy * vz;
+    k6[5] += mm * vz * vz;
+  }
+  for (int k = 0; k < 6; ++k) part[k][t] = k6[k];
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (t < o)
+      for (int k = 0; k < 6; ++k) part[k][t] += part[k][t + o];
+    __syncthreads();
+  }
+  if (t == 0) {
+    double* K = c->kint;
+    K[0] = part[0][0];
+    K[1] = part[1][0];
+    K[2] = part[2][0];
+    K[3] = part[1][0];
+    K[4] = part[3][0];
+    K[5] = part[4][0];
+    K[6] = part[2][0];
+    K[7] = part[4][0];
+    K[8] = part[5][0];
+    c->kin = (K[0] + K[4] + K[8]) / 2.0;
+    double* P = c->ptens;
+    for (int k = 0; k < 9; ++k) P[k] = (2.0 * K[k] + c->vir[k]) / c->vol;
+    c->ptrace = (P[0] + P[4] + P[8]) / 3.0;
+    if (rec) {
+      rec[0] = c->pot;
+      rec[1] = c->kin;
+      rec[2] = c->ptrace;
+      rec[3] = c->vol;
+      for (int k = 0; k < 9; ++k) rec[4 + k] = c->H[k];
+    }
   }
 }
 
 // ---------------------------------------------------------------- host
-// out slots: [0, kScal) pair-kernel sums, kOutK reciprocal energy, kOutSelf
-enum { kOutK = kScal, kOutSelf = kScal + 1, kOutLen = 16 };
 
 struct State {
   bool ready = false;
@@ -445,6 +761,10 @@ struct State {
   std::vector<int> h_typ, h_lookup, h_hv;
   std::vector<double> h_Z, h_pe, h_fe;
   // shapes of the last launches (for mdg_pr_profile)
+  PrCell* cell = nullptr;  // device
+  double *v = nullptr, *f0 = nullptr, *sc = nullptr, *m = nullptr, *rec = nullptr;
+  size_t capV = 0, capF0 = 0, capS = 0, capM = 0, capRec = 0;
+  PrCell h_cell{};          // host staging of cell (stable until the call's sync)
   PrParams last_pair{};
   PrK last_k{};
   bool have_pair = false, have_k = false;
@@ -493,6 +813,7 @@ int init_device() {
   int c = 0;
   if (hipGetDeviceCount(&c) != hipSuccess || c < 1) return MDG_E_NODEVICE;
   PR_HIP(hipStreamCreateWithFlags(&g.s, hipStreamNonBlocking));
+  if (hipMalloc((void**)&g.cell, sizeof(PrCell)) != hipSuccess) return MDG_E_NOMEM;
   if (hipMalloc((void**)&g.out, kOutLen * sizeof(double)) != hipSuccess) return MDG_E_NOMEM;
   g.ready = true;
   return MDG_OK;
@@ -501,6 +822,28 @@ int init_device() {
 void mat9(double** m, double* o) {
   for (int i = 0; i < 3; ++i)
     for (int j = 0; j < 3; ++j) o[3 * i + j] = m[i][j];
+}
+
+// The cell as the force routines read it, from pr-mdc's globals: H, H^-1
+// and g as pr-mdc's calc_cell / calc_sigma left them, the cutoff and alpha of
+// set_cutoff, and the prefactors ewald1 and ewald2 form (ewald.c:38-41, 90-91).
+void fill_cell(PrCell& c) {
+  mat9(cell.hm, c.H);
+  mat9(cell.hmr, c.R);
+  mat9(cell.g, c.G);
+  c.vol = cell.vol;
+  c.cutoff = sys.cutoff;
+  c.cutoff2 = sys.cutoff2;
+  c.a1 = sys.a1;
+  c.a1_sqrtpi = sys.a1 / kSQRTPI;
+  c.v2 = 2.0 / cell.vol;
+  c.vp = 1.0 / (kPI * cell.vol);
+}
+
+int upload_cell() {
+  fill_cell(g.h_cell);
+  PR_HIP(hipMemcpyAsync(g.cell, &g.h_cell, sizeof(PrCell), hipMemcpyHostToDevice, g.s));
+  return MDG_OK;
 }
 
 int splits_for(int nchunk, int limit, int fixed, int target) {
@@ -584,17 +927,17 @@ int download_forces() {
   return MDG_OK;
 }
 
-int enqueue_pairs(int mode, const PrParams& p) {
+int enqueue_pairs(int mode, const PrParams& p, int out_at = 0) {
   const int nchunk = (p.N + kW - 1) / kW;
   int rc;
   if ((rc = grow(&g.ps, &g.capPs, (size_t)nchunk * p.S * kScal)) != MDG_OK) return rc;
   if (p.S > 1 && (rc = grow(&g.pf, &g.capPf, (size_t)p.S * 3 * p.N)) != MDG_OK) return rc;
   const dim3 grid(nchunk, p.S);
   if (mode == kPrTable)
-    hipLaunchKernelGGL(k_pr_pairs<kPrTable>, grid, dim3(kW), 0, g.s, p, g.r, g.typ, g.tab,
+    hipLaunchKernelGGL(k_pr_pairs<kPrTable>, grid, dim3(kW), 0, g.s, p, g.cell, g.r, g.typ, g.tab,
                        g.lookup, g.Z, g.f, g.pf, g.ps);
   else
-    hipLaunchKernelGGL(k_pr_pairs<kPrEwald1>, grid, dim3(kW), 0, g.s, p, g.r, g.typ, g.tab,
+    hipLaunchKernelGGL(k_pr_pairs<kPrEwald1>, grid, dim3(kW), 0, g.s, p, g.cell, g.r, g.typ, g.tab,
                        g.lookup, g.Z, g.f, g.pf, g.ps);
   PR_HIP(hipGetLastError());
   if (p.S > 1) {
@@ -602,7 +945,8 @@ int enqueue_pairs(int mode, const PrParams& p) {
                        g.pf, g.f);
     PR_HIP(hipGetLastError());
   }
-  hipLaunchKernelGGL(k_pr_ssum, dim3(1), dim3(kSsumThreads), 0, g.s, nchunk * p.S, g.ps, g.out);
+  hipLaunchKernelGGL(k_pr_ssum, dim3(1), dim3(kSsumThreads), 0, g.s, nchunk * p.S, g.ps,
+                     g.out + out_at);
   PR_HIP(hipGetLastError());
   return MDG_OK;
 }
@@ -611,7 +955,8 @@ int enqueue_kspace(const PrK& p) {
   int rc;
   if ((rc = grow(&g.kc, &g.capK, (size_t)p.K * 8)) != MDG_OK) return rc;
   if (p.S > 1 && (rc = grow(&g.pf, &g.capPf, (size_t)p.S * 3 * p.N)) != MDG_OK) return rc;
-  hipLaunchKernelGGL(k_pr_sk, dim3(p.K), dim3(kSkThreads), 0, g.s, p, g.hv, g.r, g.typ, g.Z, g.kc);
+  hipLaunchKernelGGL(k_pr_sk, dim3(p.K), dim3(kSkThreads), 0, g.s, p, g.cell, g.hv, g.r, g.typ, g.Z,
+                     g.kc);
   PR_HIP(hipGetLastError());
   const int nchunk = (p.N + kW - 1) / kW;
   hipLaunchKernelGGL(k_pr_kforce, dim3(nchunk, p.S), dim3(kW), 0, g.s, p, g.r, g.typ, g.Z, g.kc,
@@ -627,8 +972,8 @@ int enqueue_kspace(const PrK& p) {
   return MDG_OK;
 }
 
-int enqueue_self(int N, double a1_sqrtpi) {
-  hipLaunchKernelGGL(k_pr_self, dim3(1), dim3(256), 0, g.s, N, a1_sqrtpi, g.typ, g.Z,
+int enqueue_self(int N) {
+  hipLaunchKernelGGL(k_pr_self, dim3(1), dim3(256), 0, g.s, N, g.cell, g.typ, g.Z,
                      g.out + kOutSelf);
   PR_HIP(hipGetLastError());
   return MDG_OK;
@@ -639,11 +984,6 @@ PrParams pair_params() {
   p.N = sys.N;
   p.kinds = ctl.kinds_of_ions;
   p.td = sys.table_division;
-  p.cutoff2 = sys.cutoff2;
-  p.a1 = sys.a1;
-  p.a1_sqrtpi = sys.a1 / kSQRTPI;
-  mat9(cell.hm, p.H);
-  mat9(cell.hmr, p.R);
   const int nchunk = (p.N + kW - 1) / kW, ntile = (p.N + kPT - 1) / kPT;
   p.S = splits_for(nchunk, ntile, g.pair_splits, kPairWaves);
   return p;
@@ -682,10 +1022,6 @@ PrK k_params() {
   PrK p{};
   p.N = sys.N;
   p.K = (int)(g.h_hv.size() / 4);
-  mat9(cell.g, p.g);
-  p.a1 = sys.a1;
-  p.v2 = 2.0 / cell.vol;
-  p.vp = 1.0 / (kPI * cell.vol);
   const int nchunk = (p.N + kW - 1) / kW;
   const int klimit = p.K / kMinKPerSplit > 0 ? p.K / kMinKPerSplit : 1;
   p.S = g.k_splits > 0 ? splits_for(nchunk, p.K > 0 ? p.K : 1, g.k_splits, kKWaves)
@@ -721,6 +1057,7 @@ void pair_routine(int mode, const char* what) {
       return;
     }
     int rc = upload_state(mode == kPrTable);
+    if (rc == MDG_OK) rc = upload_cell();
     const PrParams p = pair_params();
     if (rc == MDG_OK) rc = enqueue_pairs(mode, p);
     if (rc == MDG_OK) rc = download_forces();
@@ -759,6 +1096,7 @@ void ewald2(void) {
   if (!enter("ewald2")) return;
   int rc = kvectors();
   if (rc == MDG_OK) rc = upload_state(false);
+  if (rc == MDG_OK) rc = upload_cell();
   const PrK p = k_params();
   if (rc == MDG_OK && p.K > 0) {
     rc = enqueue_kspace(p);
@@ -783,7 +1121,8 @@ void ewald3(void) {
   }
   if (!enter("ewald3")) return;
   int rc = upload_state(false);
-  if (rc == MDG_OK) rc = enqueue_self(sys.N, sys.a1 / kSQRTPI);
+  if (rc == MDG_OK) rc = upload_cell();
+  if (rc == MDG_OK) rc = enqueue_self(sys.N);
   if (rc == MDG_OK) {
     if (hipMemcpyAsync(g.h_out, g.out, kOutLen * sizeof(double), hipMemcpyDeviceToHost, g.s) !=
             hipSuccess ||
@@ -809,6 +1148,140 @@ void calc_alpha(void) {
     sys.a1 += 0.01;
   }
   exit(0);
+}
+
+// main.c:100-111 (pr-mdc's step loop without its output and MSD), resident
+int mdg_pr_run(int nsteps, double* record) {
+  if (nsteps < 0) return MDG_E_ARG;
+  if (!enter("mdg_pr_run")) return g.status;
+  const int N = sys.N;
+  if (N < 1 || !pt.pres_set || !pt.pres_tensor || !pt.virial || !pt.kin_tensor || !pr.v ||
+      !pr.f || !pr.f0 || !cell.sigma || !cell.vx || !cell.fx0 || !ion.m || pr.mass == 0.0)
+    return MDG_E_ARG;
+  if (sqrt(sys.cutoff2) > 0.5 + 0.001 * sys.table_division) return MDG_E_RANGE;
+  int rc = kvectors();
+  if (rc == MDG_OK) rc = upload_state(true);
+  const int kinds = ctl.kinds_of_ions;
+  const size_t row = (size_t)N * sizeof(double);
+  if (rc == MDG_OK) rc = grow(&g.v, &g.capV, (size_t)3 * N);
+  if (rc == MDG_OK) rc = grow(&g.f0, &g.capF0, (size_t)3 * N);
+  if (rc == MDG_OK) rc = grow(&g.sc, &g.capS, (size_t)3 * N);
+  if (rc == MDG_OK) rc = grow(&g.m, &g.capM, (size_t)kinds);
+  if (rc == MDG_OK) rc = grow(&g.rec, &g.capRec, (size_t)(nsteps > 0 ? nsteps : 1) * kRec);
+  if (rc != MDG_OK) return rc;
+  PR_HIP(hipMemcpyAsync(g.v, cell.vx, row, hipMemcpyHostToDevice, g.s));
+  PR_HIP(hipMemcpyAsync(g.v + N, cell.vy, row, hipMemcpyHostToDevice, g.s));
+  PR_HIP(hipMemcpyAsync(g.v + 2 * N, cell.vz, row, hipMemcpyHostToDevice, g.s));
+  PR_HIP(hipMemcpyAsync(g.m, ion.m, (size_t)kinds * sizeof(double), hipMemcpyHostToDevice, g.s));
+  PrCell& c = g.h_cell;
+  c = PrCell{};
+  fill_cell(c);
+  mat9(cell.sigma, c.sig);
+  mat9(pr.v, c.prv);
+  mat9(pr.f, c.prf);
+  mat9(pr.f0, c.prf0);
+  c.prmass = pr.mass;
+  c.prdt = pr.dt;
+  mat9(pt.pres_set, c.pset);
+  mat9(pt.pres_tensor, c.ptens);
+  mat9(pt.virial, c.vir);
+  mat9(pt.kin_tensor, c.kint);
+  c.pot = sys.pot;
+  c.kin = sys.kin;
+  c.ptrace = pt.pres_trace;
+  c.Lx = cell.Lx;
+  c.Ly = cell.Ly;
+  c.Lz = cell.Lz;
+  c.ang_ab = cell.angle_ab;
+  c.ang_bc = cell.angle_bc;
+  c.ang_ca = cell.angle_ca;
+  c.dt = sys.dt;
+  c.dt2 = sys.dt2;
+  c.e2t = sys.e2t;
+  c.temp_set = pt.temp_set;
+  c.t_step = pt.t_control_step;
+  c.skip_ewald = ctl.skip_ewald;
+  c.step = sys.step;
+  PR_HIP(hipMemcpyAsync(g.cell, &c, sizeof(PrCell), hipMemcpyHostToDevice, g.s));
+  const PrParams pp = pair_params();
+  const PrK pk = k_params();
+  const int nb = (N + 255) / 256, n3b = (3 * N + 255) / 256;
+  for (int st = 0; st < nsteps && rc == MDG_OK; ++st) {
+    hipLaunchKernelGGL(k_pr_cell_pre, dim3(1), dim3(kW), 0, g.s, g.cell);
+    hipLaunchKernelGGL(k_pr_clear, dim3(n3b), dim3(256), 0, g.s, 3 * N, g.f, g.f0);
+    rc = enqueue_pairs(kPrTable, pp, 0);
+    if (rc != MDG_OK) break;
+    hipLaunchKernelGGL(k_pr_cell_force, dim3(nb), dim3(256), 0, g.s, N, g.cell, g.typ, g.m, g.v,
+                       g.f);
+    if (ctl.skip_ewald != 1) {
+      rc = enqueue_pairs(kPrEwald1, pp, kOutE1);
+      if (rc == MDG_OK && pk.K > 0) rc = enqueue_kspace(pk);
+      if (rc == MDG_OK) rc = enqueue_self(N);
+      if (rc != MDG_OK) break;
+    }
+    hipLaunchKernelGGL(k_pr_scal_step, dim3(1), dim3(kW), 0, g.s, g.cell, g.out, pk.K > 0);
+    hipLaunchKernelGGL(k_pr_next_vr, dim3(nb), dim3(256), 0, g.s, N, g.cell, g.typ, g.m, g.v, g.f,
+                       g.f0, g.r, g.sc);
+    hipLaunchKernelGGL(k_pr_cell_post, dim3(1), dim3(kW), 0, g.s, g.cell);
+    hipLaunchKernelGGL(k_pr_wrap, dim3(nb), dim3(256), 0, g.s, N, g.cell, g.r, g.sc);
+    hipLaunchKernelGGL(k_pr_temp, dim3(1), dim3(kW), 0, g.s, g.cell);
+    hipLaunchKernelGGL(k_pr_kin, dim3(1), dim3(256), 0, g.s, N, g.cell, g.typ, g.m, g.v,
+                       g.rec + (size_t)st * kRec);
+    if (hipGetLastError() != hipSuccess) rc = MDG_E_HIP;
+  }
+  if (rc != MDG_OK) return rc;
+  // write back what pr-mdc's routines write
+  double* hr[3] = {cell.rx, cell.ry, cell.rz};
+  double* hv[3] = {cell.vx, cell.vy, cell.vz};
+  double* hf[3] = {cell.fx, cell.fy, cell.fz};
+  double* hf0[3] = {cell.fx0, cell.fy0, cell.fz0};
+  double* hs[3] = {cell.sx, cell.sy, cell.sz};
+  for (int d = 0; d < 3; ++d) {
+    PR_HIP(hipMemcpyAsync(hr[d], g.r + (size_t)d * N, row, hipMemcpyDeviceToHost, g.s));
+    PR_HIP(hipMemcpyAsync(hv[d], g.v + (size_t)d * N, row, hipMemcpyDeviceToHost, g.s));
+    PR_HIP(hipMemcpyAsync(hf[d], g.f + (size_t)d * N, row, hipMemcpyDeviceToHost, g.s));
+    PR_HIP(hipMemcpyAsync(hf0[d], g.f0 + (size_t)d * N, row, hipMemcpyDeviceToHost, g.s));
+    if (nsteps > 0 && hs[d])
+      PR_HIP(hipMemcpyAsync(hs[d], g.sc + (size_t)d * N, row, hipMemcpyDeviceToHost, g.s));
+  }
+  PR_HIP(hipMemcpyAsync(&c, g.cell, sizeof(PrCell), hipMemcpyDeviceToHost, g.s));
+  if (record && nsteps > 0)
+    PR_HIP(hipMemcpyAsync(record, g.rec, (size_t)nsteps * kRec * sizeof(double),
+                          hipMemcpyDeviceToHost, g.s));
+  PR_HIP(hipStreamSynchronize(g.s));
+  auto put9 = [](const double* a, double** m) {
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) m[i][j] = a[3 * i + j];
+  };
+  put9(c.H, cell.hm);
+  put9(c.R, cell.hmr);
+  put9(c.sig, cell.sigma);
+  put9(c.G, cell.g);
+  put9(c.prv, pr.v);
+  put9(c.prf, pr.f);
+  put9(c.prf0, pr.f0);
+  put9(c.ptens, pt.pres_tensor);
+  put9(c.vir, pt.virial);
+  put9(c.kint, pt.kin_tensor);
+  cell.vol = c.vol;
+  cell.Lx = c.Lx;
+  cell.Ly = c.Ly;
+  cell.Lz = c.Lz;
+  cell.angle_ab = c.ang_ab;
+  cell.angle_bc = c.ang_bc;
+  cell.angle_ca = c.ang_ca;
+  sys.pot = c.pot;
+  sys.kin = c.kin;
+  sys.a1 = c.a1;
+  sys.cutoff = c.cutoff;
+  sys.cutoff2 = c.cutoff2;
+  sys.step = c.step;
+  pt.pres_trace = c.ptrace;
+  // the device's f now differs from what the drop-in's next upload brings; it
+  // re-uploads per call, so nothing is stale
+  if (c.bad_cell & 1) return MDG_E_STATE;
+  if (c.bad_cell & 2) return MDG_E_RANGE;
+  return MDG_OK;
 }
 
 int mdg_pr_status(void) { return g.status; }
@@ -843,7 +1316,7 @@ int mdg_pr_profile(int reps, double* ms_eval, double* ms_kernel) {
     (void)hipEventRecord(ev[2], g.s);
     if (rc == MDG_OK && kspace) rc = enqueue_kspace(g.last_k);
     (void)hipEventRecord(ev[3], g.s);
-    if (rc == MDG_OK) rc = enqueue_self(p.N, p.a1_sqrtpi);
+    if (rc == MDG_OK) rc = enqueue_self(p.N);
     (void)hipEventRecord(ev[4], g.s);
     if (rc == MDG_OK && hipEventSynchronize(ev[4]) != hipSuccess) rc = MDG_E_HIP;
     if (rc == MDG_OK && it > 0)

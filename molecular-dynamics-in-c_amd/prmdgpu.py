@@ -62,6 +62,22 @@ class IonProperty(C.Structure):  # md.h:160-163
     _fields_ = [("m", PD), ("Z", PD)]
 
 
+class PrSet(C.Structure):  # md.h:192-198 (Parrinello-Rahman cell mass, dt, velocity, forces)
+    _fields_ = [("mass", C.c_double), ("dt", C.c_double), ("v", PPD), ("f", PPD), ("f0", PPD)]
+
+
+GLOBALS = (("ctl", CalcControl), ("cell", CellParameters), ("sys", SystemProperty),
+           ("pt", PressureTemperature), ("ion", IonProperty), ("pr", PrSet))
+
+
+def copy_globals(src, dst):
+    """Points library dst's pr-mdc globals at what library src's hold (the
+    same arrays, not copies)."""
+    for name, cls in GLOBALS:
+        C.memmove(C.addressof(cls.in_dll(dst, name)), C.addressof(cls.in_dll(src, name)),
+                  C.sizeof(cls))
+
+
 class MdgPrError(RuntimeError):
     def __init__(self, code, what):
         super().__init__(f"{what}: libmdgpu_pr error {code}")
@@ -87,6 +103,8 @@ def lib():
         L.mdg_pr_set_splits.restype = C.c_int
         L.mdg_pr_profile.argtypes = [C.c_int, PD, PD]
         L.mdg_pr_profile.restype = C.c_int
+        L.mdg_pr_run.argtypes = [C.c_int, PD]
+        L.mdg_pr_run.restype = C.c_int
         _lib = L
     return _lib
 
@@ -203,3 +221,15 @@ def profile(reps):
     if rc != 0:
         raise MdgPrError(rc, "mdg_pr_profile")
     return ms.value, list(per)
+
+
+def run(nsteps):
+    """mdg_pr_run: nsteps of pr-mdc's step loop resident on the device, on
+    the system in the library's globals; returns the per-step records
+    (nsteps, 13): pot, kin, pres_trace, vol, hm[3][3]."""
+    rec = np.zeros((max(nsteps, 1), 13))
+    rc = lib().mdg_pr_run(nsteps, rec.ctypes.data_as(PD))
+    if rc != 0:
+        lib().mdg_pr_clear_status()
+        raise MdgPrError(rc, "mdg_pr_run")
+    return rec[:nsteps]

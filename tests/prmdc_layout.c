@@ -32,5 +32,7 @@ int main(void)
   F(pressure_temperature_set, kin_tensor);
   S(ion_property);
   F(ion_property, m); F(ion_property, Z);
+  S(Parrinello_Rahman_set);
+  F(Parrinello_Rahman_set, dt); F(Parrinello_Rahman_set, v); F(Parrinello_Rahman_set, f0);
   return 0;
 }

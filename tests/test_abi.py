@@ -222,7 +222,8 @@ def test_pr_ctypes_mirror_matches_header_layout(tmp_path):
                        ("cell_parameters", prmdgpu.CellParameters),
                        ("system_property", prmdgpu.SystemProperty),
                        ("pressure_temperature_set", prmdgpu.PressureTemperature),
-                       ("ion_property", prmdgpu.IonProperty)):
+                       ("ion_property", prmdgpu.IonProperty),
+                       ("Parrinello_Rahman_set", prmdgpu.PrSet)):
         assert int(lines[cname]) == C.sizeof(cls), cname
         for key, off in lines.items():
             if key.startswith(cname + "."):

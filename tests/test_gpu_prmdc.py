@@ -158,3 +158,31 @@ def test_prmdc_program_linked_to_libmdgpu_pr_matches_prmdc(gpu, n, steps, shear)
     assert rel(fb, fa) < 1e-10
     # the cell did move (constant pressure), and it stayed triclinic if it started so
     assert not np.array_equal(a[0, 4:], a[-1, 4:])
+
+
+@pytest.mark.skipif(not (os.path.exists(prref.PREF_SO) and
+                         os.path.exists(os.path.join(REF, "prmdc_cpu"))),
+                    reason="oracle/_ref pr-mdc harness and program not built")
+@pytest.mark.parametrize("n,steps,shear", [(2, 25, (0.1, 0.05, -0.08)), (3, 20, (-0.2, 0.15, 0.1)),
+                                           (3, 12, (0.0, 0.0, 0.0))])
+def test_resident_npt_run_matches_prmdc_program(gpu, n, steps, shear):
+    """mdg_pr_run: pr-mdc's whole step loop (forces, velocity Verlet with the
+    scaled-coordinate wrap, Parrinello-Rahman cell dynamics, the thermostat
+    at step 10 and 20, kinetic and pressure tensors) resident on the
+    device, against pr-mdc's own program from the same start."""
+    print(f"prmdc resident run n={n} steps={steps} shear={shear}", flush=True)
+    R = prref.ref()
+    R.prref_setup(n, 5.63, *shear, 17, 0.2, 0.05)
+    L = prmdgpu.lib()
+    prmdgpu.copy_globals(R, L)
+    rec = prmdgpu.run(steps)
+    a, fa, _ = run("prmdc_cpu", n, steps, 17, *shear)
+    assert rec.shape == a.shape
+    for c in range(a.shape[1]):
+        assert rel(rec[:, c], a[:, c]) < 1e-10, c
+    ctl, cell, sys_, pt, ion = prmdgpu.globals_of(L)
+    N = sys_.N
+    sums = [np.ctypeslib.as_array(p, shape=(N,)).sum() for p in
+            (cell.rx, cell.ry, cell.rz, cell.vx, cell.vy, cell.vz)]
+    assert rel(np.array(sums), fa) < 1e-9
+    assert sys_.step == steps
