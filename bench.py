@@ -254,6 +254,46 @@ def pr_pairs_within_cutoff(s):
     return cnt
 
 
+def prmdc_resident(prref, prmdgpu, n, steps=200, cpu_steps=3):
+    """pr-mdc's whole step loop (mdg_pr_run: forces, velocity Verlet,
+    Parrinello-Rahman cell, thermostat, tensors) resident on the device,
+    against pr-mdc's own step loop compiled from its sources
+    (oracle/_ref/libprmdcref.so) on one host core, from the same start
+    (pr-mdc's NaCl, n^3 cells sheared as the force leg, seed 17)."""
+    if not os.path.exists(prref.PREF_SO):
+        return None
+    import ctypes as C
+    R = prref.ref()
+    shear = (0.1, 0.05, -0.08)
+    for nm in ("next_rv_verlet", "control_press_PR", "calc_kin", "calc_press"):
+        getattr(R, nm).argtypes, getattr(R, nm).restype = [], None
+    R.control_temp.argtypes, R.control_temp.restype = [C.c_int, C.c_double], None
+    R.prref_setup(n, 5.63, *shear, 17, 0.2, 0.05)
+    _, _, sysr, ptr, _ = prmdgpu.globals_of(R)
+    t0 = time.perf_counter()
+    for k in range(1, cpu_steps + 1):
+        sysr.step = k
+        R.next_rv_verlet()
+        R.control_press_PR()
+        R.control_temp(ptr.t_control_step, ptr.temp_set)
+        R.calc_kin()
+        R.calc_press()
+    cpu_ms = (time.perf_counter() - t0) * 1e3 / cpu_steps
+    R.prref_setup(n, 5.63, *shear, 17, 0.2, 0.05)
+    L = prmdgpu.lib()
+    prmdgpu.copy_globals(R, L)
+    prmdgpu.run(20)
+    t0 = time.perf_counter()
+    rec = prmdgpu.run(steps)
+    gpu_ms = (time.perf_counter() - t0) * 1e3 / steps
+    log(f"    resident NPT N={8 * n ** 3}: {gpu_ms:.3f} ms/step, pr-mdc on 1 core "
+        f"{cpu_ms:.1f} ms/step")
+    return {"ms_per_step": gpu_ms, "steps": steps, "ion_steps_per_s": 8 * n ** 3 / (gpu_ms * 1e-3),
+            "cpu_reference_ms_per_step": cpu_ms, "cpu_steps_timed": cpu_steps, "cpu_cores": 1,
+            "cpu_kind": "reference (pr-mdc sources)", "speedup_vs_cpu": cpu_ms / gpu_ms,
+            "volume_first_last": [float(rec[0, 3]), float(rec[-1, 3])]}
+
+
 def prmdc_leg():
     """pr-mdc's force routines (libmdgpu_pr: real_space, ewald1, ewald2,
     ewald3, FP64) on NaCl in a sheared (triclinic) cell at pr-mdc's own
@@ -298,6 +338,7 @@ def prmdc_leg():
             "max_rel_force_err_vs_oracle": err})
         log(f"  pr-mdc N={s.N}: device {ms:.3f} ms/eval, drop-in {dropin_ms:.3f} ms, "
             f"oracle {cpu_ms:.1f} ms, err {err:.1e}")
+        rows[-1]["resident_npt_step"] = prmdc_resident(prref, prmdgpu, n)
     return {"workload": "pr-mdc NaCl (5.63 A cells), cell sheared b += 0.1 a, "
                         "c += 0.05 a - 0.08 b, pr-mdc's cutoff/alpha, hm = 27, FP64",
             "sizes": rows}
