@@ -90,7 +90,7 @@ struct PrCell {
   double pset[9], ptens[9], vir[9], kint[9];
   double pot, kin, ptrace;
   double dt, dt2, e2t, temp_set, oldk, vscale;
-  int step, count, t_step, skip_ewald, bad_cell, pad;
+  int step, count, t_step, skip_ewald, bad_cell, rec_base, rec_cap, pad;
 };
 
 // what the pair kernels read of the cell
@@ -686,6 +686,7 @@ __global__ __launch_bounds__(256) void k_pr_kin(int N, PrCell* __restrict__ c,
                                                 const int* __restrict__ typ,
                                                 const double* __restrict__ m,
                                                 double* __restrict__ v, double* __restrict__ rec) {
+  // rec: the run's record array; this step's row is step - rec_base - 1
   __shared__ double part[6][256];
   const int t = threadIdx.x;
   const double sc = c->vscale;
@@ -730,12 +731,14 @@ __global__ __launch_bounds__(256) void k_pr_kin(int N, PrCell* __restrict__ c,
     double* P = c->ptens;
     for (int k = 0; k < 9; ++k) P[k] = (2.0 * K[k] + c->vir[k]) / c->vol;
     c->ptrace = (P[0] + P[4] + P[8]) / 3.0;
-    if (rec) {
-      rec[0] = c->pot;
-      rec[1] = c->kin;
-      rec[2] = c->ptrace;
-      rec[3] = c->vol;
-      for (int k = 0; k < 9; ++k) rec[4 + k] = c->H[k];
+    const int row = c->step - c->rec_base - 1;
+    if (rec && row >= 0 && row < c->rec_cap) {
+      double* o = rec + (size_t)row * kRec;
+      o[0] = c->pot;
+      o[1] = c->kin;
+      o[2] = c->ptrace;
+      o[3] = c->vol;
+      for (int k = 0; k < 9; ++k) o[4 + k] = c->H[k];
     }
   }
 }
@@ -1202,22 +1205,26 @@ int mdg_pr_run(int nsteps, double* record) {
   c.t_step = pt.t_control_step;
   c.skip_ewald = ctl.skip_ewald;
   c.step = sys.step;
+  c.rec_base = sys.step;
+  c.rec_cap = nsteps;
   PR_HIP(hipMemcpyAsync(g.cell, &c, sizeof(PrCell), hipMemcpyHostToDevice, g.s));
   const PrParams pp = pair_params();
   const PrK pk = k_params();
   const int nb = (N + 255) / 256, n3b = (3 * N + 255) / 256;
-  for (int st = 0; st < nsteps && rc == MDG_OK; ++st) {
+  // one step's launches (every buffer they use is allocated by the first,
+  // eager step, so that a captured block allocates nothing)
+  auto step = [&]() -> int {
     hipLaunchKernelGGL(k_pr_cell_pre, dim3(1), dim3(kW), 0, g.s, g.cell);
     hipLaunchKernelGGL(k_pr_clear, dim3(n3b), dim3(256), 0, g.s, 3 * N, g.f, g.f0);
-    rc = enqueue_pairs(kPrTable, pp, 0);
-    if (rc != MDG_OK) break;
+    int e = enqueue_pairs(kPrTable, pp, 0);
+    if (e != MDG_OK) return e;
     hipLaunchKernelGGL(k_pr_cell_force, dim3(nb), dim3(256), 0, g.s, N, g.cell, g.typ, g.m, g.v,
                        g.f);
     if (ctl.skip_ewald != 1) {
-      rc = enqueue_pairs(kPrEwald1, pp, kOutE1);
-      if (rc == MDG_OK && pk.K > 0) rc = enqueue_kspace(pk);
-      if (rc == MDG_OK) rc = enqueue_self(N);
-      if (rc != MDG_OK) break;
+      e = enqueue_pairs(kPrEwald1, pp, kOutE1);
+      if (e == MDG_OK && pk.K > 0) e = enqueue_kspace(pk);
+      if (e == MDG_OK) e = enqueue_self(N);
+      if (e != MDG_OK) return e;
     }
     hipLaunchKernelGGL(k_pr_scal_step, dim3(1), dim3(kW), 0, g.s, g.cell, g.out, pk.K > 0);
     hipLaunchKernelGGL(k_pr_next_vr, dim3(nb), dim3(256), 0, g.s, N, g.cell, g.typ, g.m, g.v, g.f,
@@ -1225,10 +1232,38 @@ int mdg_pr_run(int nsteps, double* record) {
     hipLaunchKernelGGL(k_pr_cell_post, dim3(1), dim3(kW), 0, g.s, g.cell);
     hipLaunchKernelGGL(k_pr_wrap, dim3(nb), dim3(256), 0, g.s, N, g.cell, g.r, g.sc);
     hipLaunchKernelGGL(k_pr_temp, dim3(1), dim3(kW), 0, g.s, g.cell);
-    hipLaunchKernelGGL(k_pr_kin, dim3(1), dim3(256), 0, g.s, N, g.cell, g.typ, g.m, g.v,
-                       g.rec + (size_t)st * kRec);
-    if (hipGetLastError() != hipSuccess) rc = MDG_E_HIP;
+    hipLaunchKernelGGL(k_pr_kin, dim3(1), dim3(256), 0, g.s, N, g.cell, g.typ, g.m, g.v, g.rec);
+    return hipGetLastError() == hipSuccess ? MDG_OK : MDG_E_HIP;
+  };
+  int done = 0;
+  if (nsteps > 0) {
+    rc = step();
+    done = 1;
   }
+  // blocks of kGraphSteps steps as one hipGraph (about 25 launches a step)
+  constexpr int kGraphSteps = 16;
+  if (rc == MDG_OK && nsteps - done >= 2 * kGraphSteps) {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    bool ok = hipStreamBeginCapture(g.s, hipStreamCaptureModeThreadLocal) == hipSuccess;
+    int e = MDG_OK;
+    if (ok) {
+      for (int k = 0; k < kGraphSteps && e == MDG_OK; ++k) e = step();
+      ok = hipStreamEndCapture(g.s, &graph) == hipSuccess && e == MDG_OK;
+    }
+    if (ok) ok = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) == hipSuccess;
+    if (ok) {
+      while (rc == MDG_OK && nsteps - done >= kGraphSteps) {
+        if (hipGraphLaunch(exec, g.s) != hipSuccess) rc = MDG_E_HIP;
+        done += kGraphSteps;
+      }
+    } else {
+      (void)hipGetLastError();  // capture refused: the same launches, one by one
+    }
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+  }
+  for (; done < nsteps && rc == MDG_OK; ++done) rc = step();
   if (rc != MDG_OK) return rc;
   // write back what pr-mdc's routines write
   double* hr[3] = {cell.rx, cell.ry, cell.rz};

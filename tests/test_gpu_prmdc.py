@@ -164,7 +164,7 @@ def test_prmdc_program_linked_to_libmdgpu_pr_matches_prmdc(gpu, n, steps, shear)
                          os.path.exists(os.path.join(REF, "prmdc_cpu"))),
                     reason="oracle/_ref pr-mdc harness and program not built")
 @pytest.mark.parametrize("n,steps,shear", [(2, 25, (0.1, 0.05, -0.08)), (3, 20, (-0.2, 0.15, 0.1)),
-                                           (3, 12, (0.0, 0.0, 0.0))])
+                                           (3, 12, (0.0, 0.0, 0.0)), (2, 45, (0.1, 0.05, -0.08))])
 def test_resident_npt_run_matches_prmdc_program(gpu, n, steps, shear):
     """mdg_pr_run: pr-mdc's whole step loop (forces, velocity Verlet with the
     scaled-coordinate wrap, Parrinello-Rahman cell dynamics, the thermostat
@@ -179,10 +179,35 @@ def test_resident_npt_run_matches_prmdc_program(gpu, n, steps, shear):
     a, fa, _ = run("prmdc_cpu", n, steps, 17, *shear)
     assert rec.shape == a.shape
     for c in range(a.shape[1]):
-        assert rel(rec[:, c], a[:, c]) < 1e-10, c
+        assert rel(rec[:, c], a[:, c]) < (1e-10 if steps <= 25 else 1e-9), c
     ctl, cell, sys_, pt, ion = prmdgpu.globals_of(L)
     N = sys_.N
     sums = [np.ctypeslib.as_array(p, shape=(N,)).sum() for p in
             (cell.rx, cell.ry, cell.rz, cell.vx, cell.vy, cell.vz)]
     assert rel(np.array(sums), fa) < 1e-9
     assert sys_.step == steps
+
+
+@pytest.mark.skipif(not os.path.exists(prref.PREF_SO), reason="pr-mdc harness not built")
+def test_resident_graph_blocks_equal_single_steps(gpu):
+    """A 60-step run replays a 16-step hipGraph three times between eager
+    steps; 60 one-step calls launch every kernel eagerly, with the state
+    round-tripping through pr-mdc's globals in between.  Thermostat off (its
+    running average restarts per call): the two are bit-identical."""
+    R = prref.ref()
+    L = prmdgpu.lib()
+    out = []
+    for mode in ("graph", "eager"):
+        R.prref_setup(2, 5.63, 0.1, 0.05, -0.08, 23, 0.2, 0.05)
+        prmdgpu.globals_of(R)[3].t_control_step = 0
+        prmdgpu.copy_globals(R, L)
+        if mode == "graph":
+            rec = prmdgpu.run(60)
+        else:
+            rec = np.concatenate([prmdgpu.run(1) for _ in range(60)])
+        ctl, cell, sys_, pt, ion = prmdgpu.globals_of(L)
+        r = np.stack([np.ctypeslib.as_array(p, shape=(sys_.N,)).copy()
+                      for p in (cell.rx, cell.ry, cell.rz, cell.vx, cell.vy, cell.vz)])
+        out.append((rec, r))
+    assert np.array_equal(out[0][0], out[1][0])
+    assert np.array_equal(out[0][1], out[1][1])
