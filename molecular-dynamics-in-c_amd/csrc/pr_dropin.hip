@@ -67,7 +67,7 @@ constexpr int kKWaves = 2048;     // k-force launches
 constexpr int kMinKPerSplit = 8;  // reciprocal vectors per k-force split, at least
 constexpr int kScal = 8;          // pair-kernel scalars: pot, 6 virial entries, bad pairs
 
-enum { kPrTable = 0, kPrEwald1 = 1 };
+enum { kPrTable = 0, kPrEwald1 = 1, kPrBoth = 2 };  // kPrBoth: the resident loop's fused pass
 
 struct PrParams {
   int N, kinds, td, S;
@@ -231,27 +231,35 @@ __global__ __launch_bounds__(kW) void k_pr_pairs(PrParams p, const PrCell* __res
       min_image(q, dx, dy, dz);
       double dr = dx * dx + dy * dy + dz * dz;
       if (dr > q.cutoff2) continue;
-      double fij;
-      if constexpr (MODE == kPrTable) {
+      double fij = 0.0, pe = 0.0;
+      const double rr = sqrt(dr);
+      if constexpr (MODE != kPrEwald1) {
         // real.c:43-56: the 0.001 A bin of r from 0.5 A
         const int tl = lo ? ti : st[k], th = lo ? st[k] : ti;
-        const int ddr = (int)((sqrt(dr) - 0.5) * 1000.0);
+        const int ddr = (int)((rr - 0.5) * 1000.0);
         if (ddr < 0 || ddr > p.td) {  // inside the table's first bin: pr-mdc reads out of bounds
           bad += 1.0;
           continue;
         }
         const double2 e = tab[(size_t)lookup[tl * p.kinds + th] * (p.td + 1) + ddr];
         fij = e.y;
-        if (lo) pot += e.x;
-      } else {
-        // ewald.c:32-43
-        dr = sqrt(dr);
-        const double ZZ = lo ? Zi * Z[st[k]] : Z[st[k]] * Zi;
-        const double adr = q.a1 * dr;
-        const double e = pr_erfc(adr) / dr;
-        if (lo) pot += ZZ * e;
-        fij = ZZ * (2.0 * q.a1_sqrtpi * exp(-adr * adr) + e) / (dr * dr);
+        pe = e.x;
       }
+      if constexpr (MODE != kPrTable) {
+        // ewald.c:32-43
+        const double ZZ = lo ? Zi * Z[st[k]] : Z[st[k]] * Zi;
+        const double adr = q.a1 * rr;
+        const double e = pr_erfc(adr) / rr;
+        const double fe = ZZ * (2.0 * q.a1_sqrtpi * exp(-adr * adr) + e) / (rr * rr);
+        if constexpr (MODE == kPrBoth) {
+          pe += ZZ * e;
+          fij += fe;
+        } else {
+          pe = ZZ * e;
+          fij = fe;
+        }
+      }
+      if (lo) pot += pe;
       if (lo) {
         fx += fij * dx;
         fy += fij * dy;
@@ -463,10 +471,9 @@ __global__ __launch_bounds__(256) void k_pr_self(int N, const PrCell* __restrict
 // calc_press (pt.c:8-24).  The cell matrices and scalars live in PrCell and
 // are updated by single-thread kernels; per-ion work runs one lane per ion.
 
-// out slots: [0, kScal) pair sums (real space, or the drop-in's pair
-// routine), kOutK reciprocal energy, kOutSelf self term, [kOutE1, +kScal)
-// ewald1's pair sums in the step loop
-enum { kOutK = kScal, kOutSelf = kScal + 1, kOutE1 = 16, kOutLen = 32 };
+// out slots: [0, kScal) pair sums (the drop-in's pair routine, or the step
+// loop's fused pass), kOutK reciprocal energy, kOutSelf self term
+enum { kOutK = kScal, kOutSelf = kScal + 1, kOutLen = 16 };
 constexpr int kRec = 13;  // per-step record: pot, kin, pressure, volume, H
 
 // cofactor inverse as pr-mdc writes it (cell.c:228-240, 80-90):
@@ -488,8 +495,7 @@ __device__ double cof_inverse(const double* M, double* o) {
 
 // calc_foc's prologue (rv.c:26-33): calc_sigma (cell.c:148-175), set_cutoff
 // (cell.c:27-51), and cell_force's (H^T)^-1 (dH/dt)^T (cell.c:58-95).
-__global__ void k_pr_cell_pre(PrCell* __restrict__ c) {
-  if (threadIdx.x != 0) return;
+__device__ void cell_pre(PrCell* __restrict__ c) {
   c->step += 1;
   const double* H = c->H;
   double* sg = c->sig;
@@ -534,9 +540,12 @@ __global__ void k_pr_cell_pre(PrCell* __restrict__ c) {
     }
 }
 
+// the step's first kernel: the prologue above (block 0, thread 0) and
 // clear_foc (init.c:223-235): f0 <- f, f <- 0
-__global__ __launch_bounds__(256) void k_pr_clear(int n3, double* __restrict__ f,
+__global__ __launch_bounds__(256) void k_pr_begin(PrCell* __restrict__ c, int n3,
+                                                  double* __restrict__ f,
                                                   double* __restrict__ f0) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) cell_pre(c);
   const int k = blockIdx.x * 256 + threadIdx.x;
   if (k >= n3) return;
   f0[k] = f[k];
@@ -560,14 +569,15 @@ __global__ __launch_bounds__(256) void k_pr_cell_force(int N, const PrCell* __re
 
 // the step's potential and virial tensor from the routines' sums
 // (real.c:20-25, 73-76; ewald.c:36, 50-67, 157, 172)
-__global__ void k_pr_scal_step(PrCell* __restrict__ c, const double* __restrict__ out, int has_k) {
-  if (threadIdx.x != 0) return;
+// out[0, kScal): the fused pair pass (table + ewald1), or the table pass
+// alone when Ewald is skipped
+__device__ void scal_step(PrCell* __restrict__ c, const double* __restrict__ out, int has_k) {
   double pot = out[0];
   const bool ew = c->skip_ewald != 1;
-  if (ew) pot = ((pot + out[kOutE1]) + (has_k ? out[kOutK] : 0.0)) + out[kOutSelf];
+  if (ew) pot = (pot + (has_k ? out[kOutK] : 0.0)) + out[kOutSelf];
   c->pot = pot;
   const int up[6] = {0, 1, 2, 4, 5, 8};
-  for (int k = 0; k < 6; ++k) c->vir[up[k]] = out[1 + k] + (ew ? out[kOutE1 + 1 + k] : 0.0);
+  for (int k = 0; k < 6; ++k) c->vir[up[k]] = out[1 + k];
   c->vir[3] = c->vir[1];
   c->vir[6] = c->vir[2];
   c->vir[7] = c->vir[5];
@@ -594,13 +604,17 @@ __device__ __forceinline__ void wrap_ion(const PrCell* __restrict__ c, double& x
 }
 
 // next_v then next_r with cyclic_bc (rv.c:44-66)
-__global__ __launch_bounds__(256) void k_pr_next_vr(int N, const PrCell* __restrict__ c,
+__global__ __launch_bounds__(256) void k_pr_next_vr(int N, PrCell* __restrict__ c,
+                                                    const double* __restrict__ out, int has_k,
                                                     const int* __restrict__ typ,
                                                     const double* __restrict__ m,
                                                     double* __restrict__ v,
                                                     const double* __restrict__ f,
                                                     const double* __restrict__ f0,
                                                     double* __restrict__ r, double* __restrict__ sc) {
+  // the step's potential and virial (block 0, thread 0); this kernel reads
+  // none of what that writes
+  if (blockIdx.x == 0 && threadIdx.x == 0) scal_step(c, out, has_k);
   const int i = blockIdx.x * 256 + threadIdx.x;
   if (i >= N) return;
   const double m2 = 2.0 * m[typ[i]], dt = c->dt, dt2 = c->dt2;
@@ -665,8 +679,7 @@ __global__ __launch_bounds__(256) void k_pr_wrap(int N, const PrCell* __restrict
 
 // control_temp's bookkeeping (pt.c:124-140): the kinetic energy it averages
 // is the previous step's calc_kin
-__global__ void k_pr_temp(PrCell* __restrict__ c) {
-  if (threadIdx.x != 0) return;
+__device__ void temp_step(PrCell* __restrict__ c) {
   c->count += 1;
   c->oldk += c->kin;
   c->vscale = 0.0;  // 0: no scaling this step
@@ -688,8 +701,14 @@ __global__ __launch_bounds__(256) void k_pr_kin(int N, PrCell* __restrict__ c,
                                                 double* __restrict__ v, double* __restrict__ rec) {
   // rec: the run's record array; this step's row is step - rec_base - 1
   __shared__ double part[6][256];
+  __shared__ double sc_sh;
   const int t = threadIdx.x;
-  const double sc = c->vscale;
+  if (t == 0) {
+    temp_step(c);
+    sc_sh = c->vscale;
+  }
+  __syncthreads();
+  const double sc = sc_sh;
   double k6[6] = {0, 0, 0, 0, 0, 0};
   for (int i = t; i < N; i += 256) {
     double vx = v[i], vy = v[N + i], vz = v[2 * N + i];
@@ -939,8 +958,11 @@ int enqueue_pairs(int mode, const PrParams& p, int out_at = 0) {
   if (mode == kPrTable)
     hipLaunchKernelGGL(k_pr_pairs<kPrTable>, grid, dim3(kW), 0, g.s, p, g.cell, g.r, g.typ, g.tab,
                        g.lookup, g.Z, g.f, g.pf, g.ps);
-  else
+  else if (mode == kPrEwald1)
     hipLaunchKernelGGL(k_pr_pairs<kPrEwald1>, grid, dim3(kW), 0, g.s, p, g.cell, g.r, g.typ, g.tab,
+                       g.lookup, g.Z, g.f, g.pf, g.ps);
+  else
+    hipLaunchKernelGGL(k_pr_pairs<kPrBoth>, grid, dim3(kW), 0, g.s, p, g.cell, g.r, g.typ, g.tab,
                        g.lookup, g.Z, g.f, g.pf, g.ps);
   PR_HIP(hipGetLastError());
   if (p.S > 1) {
@@ -1214,24 +1236,21 @@ int mdg_pr_run(int nsteps, double* record) {
   // one step's launches (every buffer they use is allocated by the first,
   // eager step, so that a captured block allocates nothing)
   auto step = [&]() -> int {
-    hipLaunchKernelGGL(k_pr_cell_pre, dim3(1), dim3(kW), 0, g.s, g.cell);
-    hipLaunchKernelGGL(k_pr_clear, dim3(n3b), dim3(256), 0, g.s, 3 * N, g.f, g.f0);
-    int e = enqueue_pairs(kPrTable, pp, 0);
+    hipLaunchKernelGGL(k_pr_begin, dim3(n3b), dim3(256), 0, g.s, g.cell, 3 * N, g.f, g.f0);
+    // real_space and ewald1 in one pair pass (cell_force then follows both)
+    int e = enqueue_pairs(ctl.skip_ewald != 1 ? kPrBoth : kPrTable, pp, 0);
     if (e != MDG_OK) return e;
     hipLaunchKernelGGL(k_pr_cell_force, dim3(nb), dim3(256), 0, g.s, N, g.cell, g.typ, g.m, g.v,
                        g.f);
     if (ctl.skip_ewald != 1) {
-      e = enqueue_pairs(kPrEwald1, pp, kOutE1);
-      if (e == MDG_OK && pk.K > 0) e = enqueue_kspace(pk);
+      if (pk.K > 0) e = enqueue_kspace(pk);
       if (e == MDG_OK) e = enqueue_self(N);
       if (e != MDG_OK) return e;
     }
-    hipLaunchKernelGGL(k_pr_scal_step, dim3(1), dim3(kW), 0, g.s, g.cell, g.out, pk.K > 0);
-    hipLaunchKernelGGL(k_pr_next_vr, dim3(nb), dim3(256), 0, g.s, N, g.cell, g.typ, g.m, g.v, g.f,
-                       g.f0, g.r, g.sc);
+    hipLaunchKernelGGL(k_pr_next_vr, dim3(nb), dim3(256), 0, g.s, N, g.cell, g.out, pk.K > 0,
+                       g.typ, g.m, g.v, g.f, g.f0, g.r, g.sc);
     hipLaunchKernelGGL(k_pr_cell_post, dim3(1), dim3(kW), 0, g.s, g.cell);
     hipLaunchKernelGGL(k_pr_wrap, dim3(nb), dim3(256), 0, g.s, N, g.cell, g.r, g.sc);
-    hipLaunchKernelGGL(k_pr_temp, dim3(1), dim3(kW), 0, g.s, g.cell);
     hipLaunchKernelGGL(k_pr_kin, dim3(1), dim3(256), 0, g.s, N, g.cell, g.typ, g.m, g.v, g.rec);
     return hipGetLastError() == hipSuccess ? MDG_OK : MDG_E_HIP;
   };
