@@ -211,3 +211,30 @@ def test_resident_graph_blocks_equal_single_steps(gpu):
         out.append((rec, r))
     assert np.array_equal(out[0][0], out[1][0])
     assert np.array_equal(out[0][1], out[1][1])
+
+
+def test_cutoff_past_table_end_is_an_error(gpu):
+    """pr-mdc's cutoff is half the smallest face distance; past 10 unit
+    cells it passes the table's 20 A end, where pr-mdc reads past its
+    table.  The library refuses instead."""
+    s = prref.pr_system(n=8, jitter=0.1)
+    assert s.cutoff > 20.0
+    with pytest.raises(prmdgpu.MdgPrError) as e:
+        gpu_run(s, ("real_space",))
+    assert e.value.code == prmdgpu.E_RANGE
+    gpu_run(prref.pr_system(n=2), ("real_space",))
+
+
+def test_run_without_cell_dynamics_state_is_an_argument_error(gpu):
+    s = prref.pr_system(n=2)
+    L = prmdgpu.lib()
+    s.bind(L)
+    pr = prmdgpu.PrSet.in_dll(L, "pr")
+    keep = (pr.mass, pr.v)
+    pr.mass, pr.v = 0.0, None
+    try:
+        with pytest.raises(prmdgpu.MdgPrError) as e:
+            prmdgpu.run(5)
+        assert e.value.code == prmdgpu.E_ARG
+    finally:
+        pr.mass, pr.v = keep
