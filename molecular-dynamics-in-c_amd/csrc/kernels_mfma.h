@@ -106,14 +106,24 @@ __global__ __launch_bounds__(kIonThreads) void k_mixed_tables(const BoxDesc* __r
     zq = (float)B.Z[B.typ[i]];
     float2* X = reinterpret_cast<float2*>(B.tabx);
     float2* Y = reinterpret_cast<float2*>(B.taby);
+    // e^{2 pi i h s} by FP64 angle addition from h = 1: the rounding drift
+    // after h steps is ~h 1e-16, far below the FP32 entries stored
+    double s1x, c1x, s1y, c1y;
+    sincospi(2.0 * sx, &s1x, &c1x);
+    sincospi(2.0 * sy, &s1y, &c1y);
+    double cx = 1.0, snx = 0.0, cy = 1.0, sny = 0.0;
     for (int h = 0; h <= B.hmax; ++h) {
-      double sn, cs;
-      sincospi(2.0 * (double)h * sx, &sn, &cs);
-      X[(size_t)h * N + i] = make_float2((float)cs, (float)sn);
-      sincospi(2.0 * (double)h * sy, &sn, &cs);
-      Y[(size_t)h * N + i] = make_float2((float)cs, (float)sn);
+      X[(size_t)h * N + i] = make_float2((float)cx, (float)snx);
+      Y[(size_t)h * N + i] = make_float2((float)cy, (float)sny);
+      const double tx = cx * c1x - snx * s1x, ty = cy * c1y - sny * s1y;
+      snx = cx * s1x + snx * c1x;
+      sny = cy * s1y + sny * c1y;
+      cx = tx;
+      cy = ty;
     }
   }
+  double s1z = 0.0, c1z = 1.0, cz = 1.0, snz = 0.0;
+  if (ion) sincospi(2.0 * sz, &s1z, &c1z);
   u4_t* wb = reinterpret_cast<u4_t*>(B.wb);
   u4_t* sb = reinterpret_cast<u4_t*>(B.sb);
   const int HP = 16 * B.sknt, HB = 8 * B.kbk;
@@ -125,11 +135,12 @@ __global__ __launch_bounds__(kIonThreads) void k_mixed_tables(const BoxDesc* __r
       const int hz = 4 * q4 + e;
       c[e] = s[e] = 0.f;
       if (ion && hz <= B.hmax) {
-        double sn, cs;
-        sincospi(2.0 * (double)hz * sz, &sn, &cs);
-        c[e] = (float)cs;
-        s[e] = (float)sn;
+        c[e] = (float)cz;
+        s[e] = (float)snz;
       }
+      const double t = cz * c1z - snz * s1z;  // advance to hz + 1
+      snz = cz * s1z + snz * c1z;
+      cz = t;
     }
     if (ion && 4 * q4 < HB) {
       _Float16 hv[8], lv[8];
