@@ -7,7 +7,7 @@
  * potential.h:28).  libmdgpu exports the force-path routines with the same
  * names and signatures, computing on the GPU and leaving the results in
  * the same global objects.  A program built from mdc's sources links
- * libmdgpu in place of real.o, rcprcl.o and rv.o (recipe in INTEGRATION.md).
+ * libmdgpu in place of real.o, rcprcl.o, rv.o, pt.o and msd.o.
  *
  * The struct layouts below are field-for-field the layouts of md.h and
  * potential.h, so that the globals of an mdc executable (which interpose
