@@ -254,6 +254,25 @@ def pr_pairs_within_cutoff(s):
     return cnt
 
 
+# ewald1's pair pass, FP64 FLOP per ordered pair evaluation (add/sub/mul/div
+# = 1; exp and sqrt counted apart): difference, scaled-coordinate minimum
+# image and r^2 38 for every evaluation; for one inside the cutoff a further
+# 100: Mori's erfc as one fraction 74, the force factor 7, the pair's
+# charge product and energy 4, the force 6 and half the virial tensor 9.
+PR_FLOP_EVAL = 38
+PR_FLOP_ACCEPTED = 100
+
+
+def pr_roofline(n, npairs, ms):
+    """ewald1's routine (its pair kernel and two fixed-order sums) against the
+    FP64 peak: every ion's pair from both ions, 2 npairs of them inside."""
+    flop = n * (n - 1) * PR_FLOP_EVAL + 2 * npairs * PR_FLOP_ACCEPTED
+    achieved = flop / (ms * 1e-3) / 1e12
+    return {"kernel": "ewald1 pair pass (k_pr_pairs<1> + sums)", "bound": "fp64",
+            "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+            "frac": achieved / PEAK_FP64_TFLOPS, "flop_per_launch": flop}
+
+
 def prmdc_resident(prref, prmdgpu, n, steps=200, cpu_steps=3):
     """pr-mdc's whole step loop (mdg_pr_run: forces, velocity Verlet,
     Parrinello-Rahman cell, thermostat, tensors) resident on the device,
@@ -335,7 +354,8 @@ def prmdc_leg():
             "device_speedup_vs_cpu_oracle": cpu_ms / ms,
             "pairs_within_cutoff": npairs,
             "pair_terms_per_s_device": 2 * npairs / ((per[0] + per[1]) * 1e-3),
-            "max_rel_force_err_vs_oracle": err})
+            "max_rel_force_err_vs_oracle": err,
+            "roofline": pr_roofline(s.N, npairs, per[1])})
         log(f"  pr-mdc N={s.N}: device {ms:.3f} ms/eval, drop-in {dropin_ms:.3f} ms, "
             f"oracle {cpu_ms:.1f} ms, err {err:.1e}")
         rows[-1]["resident_npt_step"] = prmdc_resident(prref, prmdgpu, n)
