@@ -19,8 +19,14 @@
  * copies the host arrays it reads to the device, computes there, and
  * copies the arrays and scalars it writes back before returning, so host
  * code between calls (thermostats, I/O, analysis) sees current values.
- * Errors never exit the process (mdc exits on fatal errors, files.c:9-24):
- * they are printed to stderr once and kept in mdg_dropin_status().
+ * Errors follow mdc's convention for fatal errors (files.c:9-24): the
+ * first failure is printed to stderr and the process exits with
+ * EXIT_FAILURE, because mdc's unmodified main.c checks no status and would
+ * otherwise step on with stale arrays.  A host that checks status opts out
+ * with mdg_dropin_set_error_mode(MDG_DROPIN_STATUS) or by running with
+ * MDGPU_DROPIN_ERRORS=status in the environment: then a failed routine
+ * leaves its outputs as they were and the error stays in
+ * mdg_dropin_status() until cleared.
  */
 #ifndef MDC_ABI_H
 #define MDC_ABI_H
@@ -138,6 +144,12 @@ void   calc_msd(void);          /* prototypes.h:62  msd.c:20-81       */
 void   control_temp(int d_step, double temp);  /* prototypes.h:42  pt.c:85-112 */
 
 /* ---- additions of the drop-in (no counterpart in mdc) ---- */
+#ifndef MDG_DROPIN_EXIT
+#define MDG_DROPIN_EXIT   0   /* print, then exit(EXIT_FAILURE) (default) */
+#define MDG_DROPIN_STATUS 1   /* print once, keep a sticky status          */
+#endif
+/* Chooses what a failing routine does; returns 0 or MDG_E_ARG (3). */
+int    mdg_dropin_set_error_mode(int mode);
 /* 0 = ok, otherwise an MDG_E* code of mdgpu.h; sticky until cleared */
 int    mdg_dropin_status(void);
 void   mdg_dropin_clear_status(void);

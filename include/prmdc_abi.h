@@ -132,7 +132,11 @@ extern Parrinello_Rahman_set pr;
  *   ewald1-3 from alpha = 0.01 in steps of 0.01, printing alpha and the
  *   potential, then exit(0) as pr-mdc does.
  *
- * A routine that fails (no device, an allocation) prints once to stderr,
+ * A routine that fails (no device, an allocation, a pair inside the
+ * table's start) prints to stderr and exits with EXIT_FAILURE, as pr-mdc
+ * does on its fatal errors (cell.c:218-221), since pr-mdc's main.c checks
+ * no status.  After mdg_pr_set_error_mode(MDG_DROPIN_STATUS), or with
+ * MDGPU_DROPIN_ERRORS=status in the environment, it instead prints once,
  * leaves the outputs unchanged and sets the status mdg_pr_status returns.
  */
 void real_space(void);
@@ -140,6 +144,15 @@ void ewald1(void);
 void ewald2(void);
 void ewald3(void);
 void calc_alpha(void);
+
+/* No counterpart in pr-mdc: what a failing routine does, MDG_DROPIN_EXIT
+ * (0, the default: print, then exit(EXIT_FAILURE)) or MDG_DROPIN_STATUS
+ * (1: print once, keep a sticky status).  Returns 0 or MDG_E_ARG. */
+#ifndef MDG_DROPIN_EXIT
+#define MDG_DROPIN_EXIT   0
+#define MDG_DROPIN_STATUS 1
+#endif
+int mdg_pr_set_error_mode(int mode);
 
 /* No counterpart in pr-mdc: 0 while every routine succeeded, else the
  * first error code (the MDG_E_* codes of mdgpu.h: 1 no device, 2 HIP

@@ -400,7 +400,7 @@ int mdg_batch_work(mdg_batch* b, int box, long long* pairs, int* nk, int* n_ions
   int rc = box_read_scalars(b->boxes[box], S, b->stream);
   if (rc != MDG_OK) return rc;
   if (pairs) *pairs = (long long)S[kScalPairs];
-  if (nk) *nk = b->boxes[box]->plan.ewald ? b->boxes[box]->plan.nk : 0;
+  if (nk) *nk = b->boxes[box]->plan.kspace ? b->boxes[box]->plan.nk : 0;
   if (n_ions) *n_ions = b->boxes[box]->desc.N;
   return MDG_OK;
 }

@@ -9,6 +9,7 @@
 // reproduces mdc's force accumulation order bit for bit.
 #include <math.h>
 #include <stdio.h>
+#include <stdlib.h>
 #include <string.h>
 
 #include <string>
@@ -50,11 +51,29 @@ struct DropIn {
 
 DropIn g;
 
+// The error mode: mdc's convention is exit(EXIT_FAILURE) on a fatal error
+// (files.c:9-24), and an unmodified mdc main.c checks no status, so the
+// default is to exit; MDG_DROPIN_STATUS (or MDGPU_DROPIN_ERRORS=status in
+// the environment) keeps the process alive with a sticky status instead.
+int error_mode = -1;  // -1: not chosen yet (the environment decides)
+
+int current_error_mode() {
+  if (error_mode < 0) {
+    const char* e = getenv("MDGPU_DROPIN_ERRORS");
+    error_mode = (e && strcmp(e, "status") == 0) ? MDG_DROPIN_STATUS : MDG_DROPIN_EXIT;
+  }
+  return error_mode;
+}
+
 void fail(int code, const char* what) {
   if (g.status == MDG_OK) g.status = code;
   if (!g.reported) {
     fprintf(stderr, "libmdgpu drop-in: %s failed: %s\n", what, mdg_error_string(code));
     g.reported = true;
+  }
+  if (current_error_mode() == MDG_DROPIN_EXIT) {
+    fflush(stdout);
+    exit(EXIT_FAILURE);
   }
 }
 
@@ -223,6 +242,11 @@ bool ewald_on() { return !(sys.a1 == 0.0 || sys.a1 > 10.0); }  // rcprcl.c:23
 
 extern "C" {
 
+int mdg_dropin_set_error_mode(int mode) {
+  if (mode != MDG_DROPIN_EXIT && mode != MDG_DROPIN_STATUS) return MDG_E_ARG;
+  error_mode = mode;
+  return MDG_OK;
+}
 int mdg_dropin_status(void) { return g.status; }
 void mdg_dropin_clear_status(void) {
   g.status = MDG_OK;

@@ -94,7 +94,7 @@ Tiling default_tiling(const BoxPlan& P, bool can_cells) {
       t.rsplit = std::max(1, std::min(ceil_div(64, nhome), N / 128));
     }
   }
-  if (P.ewald) {
+  if (P.kspace) {
     // structure factor: ion chunks of >= 128 ions, at most 64 chunks and
     // 64 MB of partial sums
     int nch = std::max(1, std::min(ceil_div(N, 128), 64));
@@ -138,7 +138,7 @@ static Tiling finish_tiling(const BoxPlan& P, bool can_cells, Tiling t) {
     t.rsplit = std::max(1, std::min(t.rsplit, std::max(1, N / 32)));
   }
   if (sp.ordered) t.rsplit = 1;
-  if (P.ewald) {
+  if (P.kspace) {
     int nch = std::max(1, std::min(t.nchunk, 64));
     nch = std::max(1, std::min(nch, ceil_div(N, kSkTile)));
     nch = std::max(1, std::min(nch, (int)((64l << 20) / (16l * std::max(1, P.nk)))));
@@ -255,7 +255,9 @@ int make_plan(const BoxSpec& sp, int precision, BoxPlan* P, std::string* why) {
       }
     P->ncol = (int)P->cols.size();
     P->nseg = (int)P->segs.size();
-    if (P->nk == 0) P->ewald = false;
+    // with no k vector there is no k-space work, but the self term of
+    // reciprocal_space3 (rcprcl.c:115-133) still follows the alpha rule
+    P->kspace = P->nk > 0;
     // quads of four consecutive columns over zero-padded weight rows
     P->kslot.assign(P->nk, 0);
     int row = 0;
@@ -331,7 +333,7 @@ int make_plan(const BoxSpec& sp, int precision, BoxPlan* P, std::string* why) {
   else P->nc[0] = P->nc[1] = P->nc[2] = 0;
   const int homes = P->all_pairs ? ceil_div(sp.N, kRealThreads) : P->ncell;
   P->nreal = homes * P->tile.rsplit;
-  if (P->ewald)
+  if (P->kspace)
     P->kf_group_start = precision == MDG_MIXED ? partition_tiles(P->kftiles, P->tile.ngroup)
                                                : partition_quads(P->quads, P->tile.ngroup);
   return MDG_OK;
@@ -358,7 +360,7 @@ static Layout layout(const BoxPlan& P, int precision) {
   const long N = sp.N;
   const bool mixed = precision == MDG_MIXED;
   const size_t cplx = mixed ? sizeof(float2) : sizeof(double2);
-  const int ngroup = P.ewald ? (int)P.kf_group_start.size() - 1 : 0;
+  const int ngroup = P.kspace ? (int)P.kf_group_start.size() - 1 : 0;
   size_t off = 0;
   for (int k = 0; k < 12; ++k) L.ion[k] = take(off, sizeof(double) * N);
   L.typ = take(off, sizeof(int) * N);
@@ -372,7 +374,7 @@ static Layout layout(const BoxPlan& P, int precision) {
     for (int d = 0; d < 3; ++d) L.s[d] = take(off, sizeof(double) * N);
     L.styp = take(off, sizeof(int) * N);
   }
-  if (P.ewald) {
+  if (P.kspace) {
     const size_t H = P.hmax + 1;
     L.cols = take(off, sizeof(KCol) * P.ncol);
     L.segs = take(off, sizeof(KSeg) * P.nseg);
@@ -454,6 +456,7 @@ int box_create(const BoxSpec& spec, int precision, Box** out, std::string* why) 
   D.typ = (int*)(base + L.typ);
   D.N = N;
   D.ewald = P.ewald ? 1 : 0;
+  D.kspace = P.kspace ? 1 : 0;
   D.real_mode = spec.real_mode;
   D.all_pairs = P.all_pairs ? 1 : 0;
   D.kinds = spec.kinds;
@@ -504,7 +507,7 @@ int box_create(const BoxSpec& spec, int precision, Box** out, std::string* why) 
     if (n && err == MDG_OK && hipMemcpy(base + o, src, n, hipMemcpyHostToDevice) != hipSuccess)
       err = MDG_E_HIP;
   };
-  if (P.ewald) {
+  if (P.kspace) {
     D.nk = P.nk;
     D.ncol = P.ncol;
     D.nseg = P.nseg;
@@ -822,7 +825,7 @@ int BoxSet::set_boxes(const std::vector<Box*>& boxes) {
     for (int bi : order) {
       const BoxPlan& P = boxes_[bi]->plan;
       const double N = (double)boxes_[bi]->desc.N;
-      const double w = N * ((P.ewald ? (double)P.nk : 0.0) + 2.0 * std::min(N, 200.0));
+      const double w = N * ((P.kspace ? (double)P.nk : 0.0) + 2.0 * std::min(N, 200.0));
       int g = 0;
       for (int h = 1; h < G; ++h)
         if (load[h] / share[h] < load[g] / share[g]) g = h;
@@ -840,7 +843,7 @@ int BoxSet::set_boxes(const std::vector<Box*>& boxes) {
   // the widest box fixes the back-projection's K blocks, hence its ion blocks
   int hmax_all = 0;
   for (const Box* B : boxes_)
-    if (B->plan.ewald) hmax_all = std::max(hmax_all, B->plan.hmax);
+    if (B->plan.kspace) hmax_all = std::max(hmax_all, B->plan.hmax);
   const int kf_ions = kforce_mfma_ions(kforce_hz_for(hmax_all) / 8);
   for (int bi : order) {
     const Box* B = boxes_[bi];
@@ -863,7 +866,7 @@ int BoxSet::set_boxes(const std::vector<Box*>& boxes) {
         for (int s = 0; s < rs; ++s) real.push_back({bi, c, slot++, s, 0});
     }
     for (int i0 = 0; i0 < N; i0 += kIonThreads) L[LIon][g].push_back({bi, i0, 0, 0, 0});
-    if (P.ewald) {
+    if (P.kspace) {
       hmax = std::max(hmax, P.hmax);
       for (int i0 = 0; i0 < N; i0 += kIonThreads) L[LTab][g].push_back({bi, i0, 0, 0, 0});
       if (mixed) {

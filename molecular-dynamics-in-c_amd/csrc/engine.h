@@ -65,7 +65,8 @@ std::vector<std::pair<TuneKey, Tiling>> tune_entries();
 
 struct BoxPlan {
   BoxSpec spec;
-  bool ewald = false;
+  bool ewald = false;               // alpha rule of rcprcl.c:23,115 (gates the self term)
+  bool kspace = false;              // ewald and at least one k vector (k-space work)
   int hm_sqrt = 0, hmax = 0, nk = 0, ncol = 0, nseg = 0;
   std::vector<KCol> cols;
   std::vector<KSeg> segs;

@@ -240,7 +240,7 @@ __global__ __launch_bounds__(kIonThreads) void k_force_vv(const BoxDesc* __restr
   } else {
     fx = B.fx[i]; fy = B.fy[i]; fz = B.fz[i];
   }
-  if (B.ewald && (terms & 2)) {
+  if (B.kspace && (terms & 2)) {
     for (int g = 0; g < B.ngroup; ++g) {
       const double* fp = B.fpart + (size_t)g * 3 * N;
       fx += fp[i];
@@ -333,7 +333,7 @@ __global__ __launch_bounds__(kScalThreads) void k_box_scalars(const BoxDesc* __r
     v[3] += B.rpart[4 * s + 3];
     v[8] += (double)B.rpairs[s];
   }
-  if (B.ewald)
+  if (B.kspace)
     for (int k = tid; k < (B.nk + kIonThreads - 1) / kIonThreads; k += kScalThreads)
       v[4] += B.kpot[k];
   for (int i = tid; i < B.N; i += kScalThreads) {
@@ -346,7 +346,7 @@ __global__ __launch_bounds__(kScalThreads) void k_box_scalars(const BoxDesc* __r
   if (tid == 0) {
     double* S = B.scal;
     const double pself = B.ewald ? B.self_pot : 0.0;
-    const double precip = B.ewald ? v[4] : 0.0;
+    const double precip = B.kspace ? v[4] : 0.0;
     S[kScalPotReal] = v[0];
     S[kScalPotRecip] = precip;
     S[kScalPotSelf] = pself;

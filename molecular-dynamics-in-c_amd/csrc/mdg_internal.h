@@ -83,6 +83,7 @@ struct BoxDesc {
   int *typ;
   int N;
   int ewald;          // 0: Ewald skipped (alpha == 0 or alpha > 10)
+  int kspace;         // 1: ewald and at least one k vector (reciprocal work)
   int real_mode;      // 0 table, 1 analytic
   int all_pairs;      // 1: O(N^2) pair loop in mdc's order; 0: cell list
   int kinds;          // species count (table lookup stride)

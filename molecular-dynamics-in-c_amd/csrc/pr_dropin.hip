@@ -806,11 +806,29 @@ const char* err_name(int code) {
   }
 }
 
+// pr-mdc exits on a fatal error (cell.c:218-221, files.c) and its main.c
+// checks no status, so a failing routine exits by default; a host that
+// checks mdg_pr_status opts out (mdg_pr_set_error_mode, or
+// MDGPU_DROPIN_ERRORS=status in the environment).
+int error_mode = -1;  // -1: not chosen yet (the environment decides)
+
+int current_error_mode() {
+  if (error_mode < 0) {
+    const char* e = getenv("MDGPU_DROPIN_ERRORS");
+    error_mode = (e && strcmp(e, "status") == 0) ? MDG_DROPIN_STATUS : MDG_DROPIN_EXIT;
+  }
+  return error_mode;
+}
+
 void fail(int code, const char* what) {
   if (g.status == MDG_OK) g.status = code;
   if (!g.reported) {
     fprintf(stderr, "libmdgpu_pr: %s failed: %s\n", what, err_name(code));
     g.reported = true;
+  }
+  if (current_error_mode() == MDG_DROPIN_EXIT) {
+    fflush(stdout);
+    exit(EXIT_FAILURE);
   }
 }
 
@@ -1178,7 +1196,10 @@ void calc_alpha(void) {
 // main.c:100-111 (pr-mdc's step loop without its output and MSD), resident
 int mdg_pr_run(int nsteps, double* record) {
   if (nsteps < 0) return MDG_E_ARG;
-  if (!enter("mdg_pr_run")) return g.status;
+  // an entry point with a return code: it reports, it never exits
+  if (g.status != MDG_OK) return g.status;
+  if (const int rc0 = init_device(); rc0 != MDG_OK) return rc0;
+  if (sys.N < 0 || ctl.kinds_of_ions < 1 || ctl.kinds_of_ions > 16) return MDG_E_ARG;
   const int N = sys.N;
   if (N < 1 || !pt.pres_set || !pt.pres_tensor || !pt.virial || !pt.kin_tensor || !pr.v ||
       !pr.f || !pr.f0 || !cell.sigma || !cell.vx || !cell.fx0 || !ion.m || pr.mass == 0.0)
@@ -1335,6 +1356,12 @@ int mdg_pr_run(int nsteps, double* record) {
   // re-uploads per call, so nothing is stale
   if (c.bad_cell & 1) return MDG_E_STATE;
   if (c.bad_cell & 2) return MDG_E_RANGE;
+  return MDG_OK;
+}
+
+int mdg_pr_set_error_mode(int mode) {
+  if (mode != MDG_DROPIN_EXIT && mode != MDG_DROPIN_STATUS) return MDG_E_ARG;
+  error_mode = mode;
   return MDG_OK;
 }
 

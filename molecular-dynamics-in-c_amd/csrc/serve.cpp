@@ -220,8 +220,13 @@ extern "C" int mdg_serve(const mdg_job* jobs, int njobs, const mdg_serve_config*
       }
       if (x.left == 0) {
         std::vector<double> r(3 * (size_t)x.box->desc.N);
-        box_download_state(x.box, r.data(), nullptr, nullptr, nullptr, stream);
-        R.final_checksum = checksum(r);
+        const int drc = box_download_state(x.box, r.data(), nullptr, nullptr, nullptr, stream);
+        if (drc != MDG_OK) {
+          R.status = drc;
+          rc_all = drc;
+        } else {
+          R.final_checksum = checksum(r);
+        }
         R.finish_s = now_s(t0);
         R.latency_s = R.finish_s - (cfg->realtime ? jobs[x.job].arrival_s : 0.0);
         resident -= x.box->bytes;

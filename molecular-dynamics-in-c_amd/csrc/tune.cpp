@@ -140,7 +140,7 @@ int tune_class(const BoxSpec& spec, const mdg_box_params& prm, int precision, lo
   std::vector<int> ch_list, gr_list;
   ch_list.push_back(def.nchunk);
   gr_list.push_back(def.ngroup);
-  if (P.ewald) {
+  if (P.kspace) {
     for (int c : {1, 2, 4, 8, 16, 32, 64})
       if (c * kSkTile <= N + kSkTile - 1) push_unique(ch_list, c);
     for (int g : {1, 2, 4, 8, 16, 32, 64})

@@ -15,6 +15,7 @@ LIB_PATH = os.path.join(PKG_DIR, "lib", "libmdgpu.so")
 OK, E_NODEVICE, E_HIP, E_ARG, E_NOMEM, E_BUDGET, E_STATE, E_RANGE, E_TIMEOUT = range(9)
 FP64, MIXED = 0, 1
 TABLE, ANALYTIC = 0, 1
+DROPIN_EXIT, DROPIN_STATUS = 0, 1   # mdg_dropin_set_error_mode (mdc_abi.h)
 CELLS_AUTO, ALL_PAIRS, CELL_LIST = -1, 0, 1
 TERM_REAL, TERM_RECIP = 1, 2
 
@@ -259,6 +260,7 @@ def lib():
             "mdg_tune_save": ([C.c_char_p], C.c_int),
             "mdg_tune_load": ([C.c_char_p], C.c_int),
             "mdg_box_tiling": ([C.POINTER(BoxParams), C.c_int, C.POINTER(Tiling)], C.c_int),
+            "mdg_dropin_set_error_mode": ([C.c_int], C.c_int),
             "mdg_dropin_status": ([], C.c_int),
             "mdg_dropin_clear_status": ([], None),
             "mdg_dropin_release": ([], None),
@@ -284,6 +286,9 @@ def lib():
             f = getattr(L, name)
             f.argtypes = []
             f.restype = None
+        # this mirror raises MdgError from the drop-in's status (dropin_call)
+        # instead of letting a failing routine exit the interpreter
+        L.mdg_dropin_set_error_mode(DROPIN_STATUS)
         _lib = L
     return _lib
 

@@ -97,6 +97,10 @@ def lib():
         for n in ("real_space", "ewald1", "ewald2", "ewald3", "calc_alpha"):
             getattr(L, n).restype = None
             getattr(L, n).argtypes = []
+        L.mdg_pr_set_error_mode.argtypes = [C.c_int]
+        L.mdg_pr_set_error_mode.restype = C.c_int
+        # errors surface as MdgPrError from the status, not as an exit
+        L.mdg_pr_set_error_mode(1)
         L.mdg_pr_status.restype = C.c_int
         L.mdg_pr_clear_status.restype = None
         L.mdg_pr_set_splits.argtypes = [C.c_int, C.c_int]

@@ -91,3 +91,25 @@ def test_other_tabulated_materials_match_mdc(gpu, mat):
         assert np.abs(dev[:, c] - cpu[:, c]).max() <= 1e-12 * scale, (mat, c)
     assert np.allclose(fdev, fcpu, rtol=1e-12, atol=0)
 
+
+@pytest.mark.skipif(not os.path.exists(PROGS["mdc_gpu"]),
+                    reason="oracle/_ref drop-in programs not built (no reference checkout)")
+def test_linked_program_exits_on_a_pair_inside_the_table_start(gpu):
+    """A 0.9 A lattice puts nearest neighbours at about 0.45 A, inside the
+    table's 0.5 A start, where mdc would index before its table
+    (real.c:62).  The device skips such pairs and flags them; the drop-in
+    then prints MDG_E_RANGE and exits with EXIT_FAILURE, mdc's convention
+    for fatal errors, instead of letting mdc step on.  With
+    MDGPU_DROPIN_ERRORS=status the program keeps running and the error is
+    reported once."""
+    args = [PROGS["mdc_gpu"], "3", "2", "11", "0", "0", "0", "0", "0.9"]
+    env = dict(os.environ)
+    env.pop("MDGPU_DROPIN_ERRORS", None)
+    p = subprocess.run(args, capture_output=True, text=True, timeout=120, env=env)
+    assert p.returncode == 1, (p.returncode, p.stderr[-2000:])
+    assert "libmdgpu drop-in: next_rv_verlet failed" in p.stderr, p.stderr[-2000:]
+    assert not any(ln.split()[:1] == ["1"] for ln in p.stdout.splitlines())
+    env["MDGPU_DROPIN_ERRORS"] = "status"
+    p = subprocess.run(args, capture_output=True, text=True, timeout=120, env=env)
+    assert p.returncode == 0, (p.returncode, p.stderr[-2000:])
+    assert p.stderr.count("libmdgpu drop-in") == 1, p.stderr[-2000:]

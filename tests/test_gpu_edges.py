@@ -37,6 +37,44 @@ def test_ewald_sentinel_turns_reciprocal_space_off(gpu, alpha):
         b.close()
 
 
+def test_empty_half_sphere_keeps_the_self_term(gpu):
+    """hm = 0 with a valid alpha: no k vector (rcprcl.c:32-38), yet
+    reciprocal_space3 adds -a3 sum Z^2 (rcprcl.c:108-133).  The handle API's
+    calc_foc, in both precisions, and the drop-in's calc_foc and its
+    routine-by-routine sequence all agree with the oracle, self term
+    included."""
+    n, a = 3, H.A_NACL
+    L = n * a
+    o = H.cfg1_oracle(hm=0)
+    r = mdref.jitter_positions(o.r, o.L, 0.2, 31)
+    o.set_state(r=r, f=np.zeros_like(r))
+    o.calc_foc()
+    assert o.b.pot_self != 0.0 and o.b.pot_recip == 0.0
+    p = mdgpu.box_params(n, n, n, a, 5.6 / L, 0, L / 2, 2.0, real_mode=mdgpu.TABLE)
+    for prec, tol in ((mdgpu.FP64, 1e-12), (mdgpu.MIXED, 1e-5)):
+        b = mdgpu.Batch([p], prec)
+        b.set_state(0, r=r)
+        b.forces()
+        s = b.scalars(0)
+        assert b.work(0)["nk"] == 0
+        assert s["pot_recip"] == 0.0
+        assert s["pot_self"] == pytest.approx(o.b.pot_self, rel=1e-14)
+        assert s["pot"] == pytest.approx(o.b.pot, rel=tol)
+        assert H.rel_err(b.state(0)["f"], o.f) <= tol
+        b.close()
+    d = H.DropIn(o)
+    try:
+        for seq in (("calc_foc",), ("clear_foc", "real_space", "reciprocal_space",
+                                    "reciprocal_space3")):
+            d.set_state(r=r, f=np.zeros_like(r), f0=np.zeros_like(r))
+            for name in seq:
+                d.call(name)
+            assert H.rel_err(d.f, o.f) <= 1e-12, seq
+            assert d.sys.pot == pytest.approx(o.b.pot, rel=1e-12), seq
+    finally:
+        d.release()
+
+
 def test_pair_inside_table_start_is_reported(gpu):
     """Table mode covers r >= 0.5 A (mdc's bins, init.c:65); a closer pair
     is an error mdc would index out of its table for.  Both precisions

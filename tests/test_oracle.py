@@ -259,3 +259,27 @@ def test_oracle_matches_golden_fixture():
     o.b.step = 1
     o.nve(int(g["traj_steps"]))
     assert np.array_equal(o.r, g["traj_r"]) and np.array_equal(o.v, g["traj_v"])
+
+
+@pytest.mark.skipif(not mdref.ref_available(), reason="oracle/_ref not built (no reference checkout)")
+def test_empty_half_sphere_keeps_the_self_term():
+    """hm = 0 leaves no k vector in the half sphere (rcprcl.c:32-38), but
+    alpha is valid, so reciprocal_space adds nothing while
+    reciprocal_space3 still adds -a3 sum Z^2 (rcprcl.c:108-133): the
+    oracle's calc_foc equals mdc's, and its pot is the real-space energy
+    plus that self term."""
+    L = 3 * A
+    o = cfg1_box(hm=0)
+    R = mdref.Reference(3, 3, 3, A, 5.6 / L, 0, L / 2, 2.0)
+    assert mdref.lib().orc_count_kvectors(0) == 0
+    r = mdref.jitter_positions(o.r, o.L, 0.2, 31)
+    o.set_state(r=r, f=np.zeros_like(r))
+    R.set_state(r=r, f=np.zeros_like(r))
+    o.calc_foc()
+    R.call("calc_foc")
+    assert np.array_equal(R.state()["f"], o.f)
+    assert R.scalars()["pot"] == o.scalars()["pot"]
+    z2 = float((o.charges() ** 2).sum())
+    assert o.b.pot_recip == 0.0
+    assert o.b.pot_self == pytest.approx(-o.b.a3 * z2, rel=1e-14)
+    assert o.b.pot == pytest.approx(o.b.pot_real + o.b.pot_self, rel=1e-14)
