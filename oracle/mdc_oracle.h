@@ -40,9 +40,14 @@
  *   erfcc            compiled mdc (bit-equal), math.erfc (|err| < 1e-14)
  *   unit_converter   compiled mdc (tables, trajectory and scalars bit-equal)
  *   potential_nacl   compiled mdc tables bit-equal; closed-form table entries
- *   potential_kcl    PARITY UNPINNED: mdc has no KCl; the constants are the
- *                    published Tosi-Fumi (1964) KCl set, checked only for
- *                    consistency of the host copy with this one
+ *   potential_kcl    mdc has no KCl (tests/test_kcl_constants.py): closed
+ *                    forms (Pauling factors of closed 8-electron shells,
+ *                    sigma_ij = sigma_i + sigma_j, Cl- radius, b and the
+ *                    C/D unit conversion shared with mdc's NaCl set); the
+ *                    static crystal's lattice constant, lattice energy and
+ *                    bulk modulus against measured data, NaCl as control,
+ *                    each plausible constant mistake failing; the crystal
+ *                    energy equal to a closed-form lattice sum to 1e-9
  *   set_loc          compiled mdc (positions and species bit-equal)
  *   mk_table         compiled mdc (bit-equal); closed form; -U'/r by
  *                    centred differences
