@@ -257,6 +257,31 @@ const char *mdg_error_string(int code);
  * yardstick.  *ms_avg: average time of one copy. */
 int  mdg_copy_probe(size_t bytes, int reps, double *ms_avg);
 
+/* ------------------------------------------------------ allocation-free */
+/* No counterpart in mdc.  For callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc): mdc's calc_foc (rv.c:12-18) for one box, as mdg_batch_forces
+ * computes it for a batch of that box alone, on the caller's device
+ * buffers, allocating nothing.
+ *   r:        3N positions [A], SoA (x block, y, z), N = 8 nx ny nz;
+ *   species:  N ints (0 = cation, 1 = anion);
+ *   f:        3N forces, SoA (written);
+ *   energies: 8 doubles (written): pot, pot_real, pot_recip, pot_self,
+ *             pot_real + pot_recip / 4 + pot_self, vir[3], as mdg_scalars;
+ *   stream:   a hipStream_t, or NULL for the null stream.
+ * Every other device buffer sits in one static device array of
+ * MDG_NOALLOC_SCRATCH_BYTES per device, held until the process ends; a box
+ * that needs more is refused with MDG_E_BUDGET (a 4096-ion box at
+ * Ewald 1e-6, rc = 10 A needs about 18 MiB).  All calls on a device share
+ * that array and it keeps the last call's box: calls with the same
+ * parameters and precision reuse its uploaded plan.  Calls that overlap
+ * on one device race (calls are serialised within a process).  Returns
+ * after the results are written: MDG_OK, MDG_E_ARG (a species out of
+ * range), MDG_E_STATE (a non-finite coordinate), MDG_E_RANGE, MDG_E_BUDGET. */
+#define MDG_NOALLOC_SCRATCH_BYTES (32u << 20)
+int  mdg_forces_noalloc(const mdg_box_params *p, int precision, const double *r,
+                        const int *species, double *f, double *energies, void *stream);
+
 /* -------------------------------------------------------------- serving */
 typedef struct {
   double arrival_s;     /* seconds after the start of the stream          */

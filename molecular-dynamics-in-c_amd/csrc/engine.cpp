@@ -423,6 +423,10 @@ size_t box_bytes_for(const BoxSpec& spec, int precision) {
 }
 
 // ------------------------------------------------------------------ boxes
+// Points the box's descriptor at its arrays in the zeroed device block
+// `base` (layout(P).total bytes) and uploads the box's constants there.
+static int bind_box(Box* b, char* base, std::string* why);
+
 int box_create(const BoxSpec& spec, int precision, Box** out, std::string* why) {
   Box* b = new Box();
   b->precision = precision;
@@ -431,23 +435,64 @@ int box_create(const BoxSpec& spec, int precision, Box** out, std::string* why) 
     delete b;
     return rc;
   }
-  const BoxPlan& P = b->plan;
-  const int N = spec.N;
-  const bool mixed = precision == MDG_MIXED;
-  const Layout L = layout(P, precision);
-  b->bytes = L.total;
+  b->bytes = layout(b->plan, precision).total;
   if (hipMalloc(&b->dmem, b->bytes) != hipSuccess) {
     (void)hipGetLastError();
     if (why) *why = "device allocation failed";
     delete b;
     return MDG_E_NOMEM;
   }
-  char* base = (char*)b->dmem;
-  if (hipMemset(base, 0, b->bytes) != hipSuccess) {
+  if (hipMemset(b->dmem, 0, b->bytes) != hipSuccess) {
     (void)hipFree(b->dmem);
     delete b;
     return MDG_E_HIP;
   }
+  rc = bind_box(b, (char*)b->dmem, why);
+  if (rc != MDG_OK) {
+    box_destroy(b);
+    return rc;
+  }
+  *out = b;
+  return MDG_OK;
+}
+
+int box_create_at(const BoxSpec& spec, int precision, void* base, size_t cap, hipStream_t s,
+                  Box** out, std::string* why) {
+  Box* b = new Box();
+  b->precision = precision;
+  int rc = make_plan(spec, precision, &b->plan, why);
+  if (rc != MDG_OK) {
+    delete b;
+    return rc;
+  }
+  b->bytes = layout(b->plan, precision).total;
+  if (b->bytes > cap) {
+    if (why) *why = "box exceeds the device memory it was given";
+    delete b;
+    return MDG_E_BUDGET;
+  }
+  if (hipMemsetAsync(base, 0, b->bytes, s) != hipSuccess ||
+      hipStreamSynchronize(s) != hipSuccess) {
+    (void)hipGetLastError();
+    delete b;
+    return MDG_E_HIP;
+  }
+  rc = bind_box(b, (char*)base, why);  // b->dmem stays null: the caller owns base
+  if (rc != MDG_OK) {
+    delete b;
+    return rc;
+  }
+  *out = b;
+  return MDG_OK;
+}
+
+static int bind_box(Box* b, char* base, std::string* why) {
+  const BoxPlan& P = b->plan;
+  const BoxSpec& spec = P.spec;
+  const int precision = b->precision;
+  const int N = spec.N;
+  const bool mixed = precision == MDG_MIXED;
+  const Layout L = layout(P, precision);
   BoxDesc& D = b->desc;
   memset(&D, 0, sizeof(D));
   double** ion_ptrs[12] = {&D.rx, &D.ry, &D.rz, &D.vx, &D.vy, &D.vz,
@@ -576,12 +621,10 @@ int box_create(const BoxSpec& spec, int precision, Box** out, std::string* why) 
   D.obs = (double*)(base + L.obs);  // observables off until set_observe()
   if (err != MDG_OK) {
     if (why) *why = "uploading box constants failed";
-    box_destroy(b);
     return err;
   }
   b->species.assign(N, 0);
   b->desc_dirty = true;
-  *out = b;
   return MDG_OK;
 }
 
@@ -706,9 +749,16 @@ BoxSet::~BoxSet() {
   }
   for (hipEvent_t ev : ev_join_) (void)hipEventDestroy(ev);
   if (ev_fork_) (void)hipEventDestroy(ev_fork_);
+  if (arena_) return;  // the caller owns the descriptors and work lists
   if (d_desc_) (void)hipFree(d_desc_);
   if (d_items_) (void)hipFree(d_items_);
   if (d_ids_) (void)hipFree(d_ids_);
+}
+
+void BoxSet::set_arena(void* base, size_t bytes) {
+  arena_ = (char*)base;
+  arena_bytes_ = bytes;
+  serial_ = true;  // no side streams: creating them is what such callers avoid
 }
 
 // Side streams of a step's concurrent branches and their fork/join events,
@@ -922,6 +972,20 @@ int BoxSet::set_boxes(const std::vector<Box*>& boxes) {
       ids.insert(ids.end(), I[l][g].begin(), I[l][g].end());
     }
     all_.in[l] = (int)ids.size() - all_.ioff[l];
+  }
+  if (arena_) {
+    // descriptors, items and box ids back to back in the caller's memory
+    size_t off = 0;
+    const size_t o_desc = take(off, sizeof(BoxDesc) * std::max(nb, 1));
+    const size_t o_items = take(off, sizeof(Item) * std::max<size_t>(items.size(), 1));
+    const size_t o_ids = take(off, sizeof(int) * std::max<size_t>(ids.size(), 1));
+    if (off > arena_bytes_) return MDG_E_BUDGET;
+    d_desc_ = (BoxDesc*)(arena_ + o_desc);
+    d_items_ = (Item*)(arena_ + o_items);
+    d_ids_ = (int*)(arena_ + o_ids);
+    desc_cap_ = std::max(nb, 1);
+    items_cap_ = std::max<size_t>(items.size(), 1);
+    ids_cap_ = std::max<size_t>(ids.size(), 1);
   }
   if (items.size() > items_cap_) {
     if (d_items_) (void)hipFree(d_items_);
@@ -1161,10 +1225,11 @@ int BoxSet::profile(int nsteps, hipStream_t s, bool as_step, std::vector<std::st
   if (rc != MDG_OK) return rc;
   LaunchTimer t;
   timer_ = &t;
-  serial_ = !as_step;
+  const bool was_serial = serial_;
+  serial_ = was_serial || !as_step;
   for (int st = 0; st < nsteps; ++st) enqueue_step(s, true);
   timer_ = nullptr;
-  serial_ = false;
+  serial_ = was_serial;
   hipError_t e = hipGetLastError();
   if (e == hipSuccess) e = hipStreamSynchronize(s);
   std::vector<double> tot(kKernelNames, 0.0);

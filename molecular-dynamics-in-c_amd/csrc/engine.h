@@ -103,6 +103,11 @@ struct Box {
 // Device bytes a box of this spec would hold (for admission control).
 size_t box_bytes_for(const BoxSpec& spec, int precision);
 int box_create(const BoxSpec& spec, int precision, Box** out, std::string* why);
+// The same box in `cap` bytes of device memory at `base` that the caller
+// owns (nothing is allocated; box_destroy frees nothing of it).  The block
+// is zeroed on stream s.  MDG_E_BUDGET when the box needs more than cap.
+int box_create_at(const BoxSpec& spec, int precision, void* base, size_t cap, hipStream_t s,
+                  Box** out, std::string* why);
 void box_destroy(Box* b);
 int box_upload_state(Box* b, const double* r, const double* v, const int* species,
                      hipStream_t s);
@@ -137,7 +142,13 @@ class BoxSet {
   explicit BoxSet(int precision, int real_mode);
   ~BoxSet();
   int set_boxes(const std::vector<Box*>& boxes);  // rebuild work lists
+  // Device memory for the set's descriptors and work lists, owned by the
+  // caller: set_boxes then allocates nothing (MDG_E_BUDGET when they do
+  // not fit), and steps run on the caller's stream alone (no side streams
+  // or events are created).
+  void set_arena(void* base, size_t bytes);
   const std::vector<Box*>& boxes() const { return boxes_; }
+  BoxDesc* device_desc() const { return d_desc_; }  // the boxes' descriptors on the device
   // mdc's calc_foc on every box (f0 <- f, f <- forces); vv: also next_v and
   // next_r and count the step.
   void enqueue_step(hipStream_t s, bool vv);
@@ -173,6 +184,8 @@ class BoxSet {
   Item* d_items_ = nullptr;
   int* d_ids_ = nullptr;
   size_t items_cap_ = 0, ids_cap_ = 0, desc_cap_ = 0;
+  char* arena_ = nullptr;   // caller-owned device memory (set_arena)
+  size_t arena_bytes_ = 0;
   // offsets into d_items_ / d_ids_ and counts
   // Work lists: item ranges in d_items_ per kernel, box-id ranges in d_ids_.
   enum { LReal, LTab, LSk, LFin, LKf, LIon, LPack, kLists };

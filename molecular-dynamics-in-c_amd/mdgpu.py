@@ -260,6 +260,8 @@ def lib():
             "mdg_tune_save": ([C.c_char_p], C.c_int),
             "mdg_tune_load": ([C.c_char_p], C.c_int),
             "mdg_box_tiling": ([C.POINTER(BoxParams), C.c_int, C.POINTER(Tiling)], C.c_int),
+            "mdg_forces_noalloc": ([C.POINTER(BoxParams), C.c_int, C.c_void_p, C.c_void_p,
+                                    C.c_void_p, C.c_void_p, C.c_void_p], C.c_int),
             "mdg_dropin_set_error_mode": ([C.c_int], C.c_int),
             "mdg_dropin_status": ([], C.c_int),
             "mdg_dropin_clear_status": ([], None),
