@@ -2,8 +2,8 @@
 
   nacl216      NaCl, 216 ions (3x3x3 cells, a = 5.64 A), alpha = 5.6/L,
                cutoff L/2, hm = 27, 1000 K, dt = 2 fs, FP64
-  nacl4096     NaCl, 4096 ions (8x8x8), melted at 1500 K, cutoff 10 A,
-               alpha/hm for a 1e-6 Ewald error, FP64, cell lists
+  nacl4096     NaCl, 4096 ions (8x8x8), melted at 1500 K (melt_snapshot),
+               cutoff 10 A, alpha/hm for a 1e-6 Ewald error, FP64, cell lists
   kcl32768     KCl, 32768 ions (16x16x16, a = 6.29 A), 1300 K, mixed
   batch256     256 alkali-halide boxes (NaCl or KCl by seed), 64..4096 ions
                log-uniform in whole conventional cells, 300..2000 K, mixed
@@ -82,6 +82,34 @@ def initial_state(p, temp_k, seed, jitter=0.0):
         r = np.mod(r + rng.uniform(-jitter, jitter, r.shape), L)
     v = mdgpu.maxwell(seed, temp_k, sp, p.pot)
     return r, v, sp
+
+
+def melt_snapshot(p, seed=42, t_hot=3000.0, hot_steps=3000, t_melt=1500.0, settle_steps=1000):
+    """A melt of box p at t_melt, prepared on the device: mdc's lattice
+    with Maxwell velocities at t_hot, stepped hot_steps steps (mixed
+    precision, analytic erfc + BMH, the box's own Ewald split) under mdc's
+    velocity rescaling (control_temp every 10 steps, pt.c:85-112) at t_hot,
+    well above the crystal's superheating limit at this density, then
+    settle_steps steps rescaled to t_melt.  Returns (r, v, species, msd):
+    the state after the last step and the mean-squared displacement per
+    species over the settle stage (about 2 ps at dt = 2 fs; a liquid's
+    grows to several A^2 there, a crystal's stays below about 0.5 A^2)."""
+    q = mdgpu.BoxParams.from_buffer_copy(p)
+    q.real_mode = mdgpu.ANALYTIC
+    r, v, sp = initial_state(q, t_hot, seed)
+    b = mdgpu.Batch([q], mdgpu.MIXED)
+    try:
+        b.set_state(0, r=r, v=v, species=sp)
+        b.set_observe(0, t_control_step=10, temp_K=t_hot, t_control_until=hot_steps)
+        b.step(hot_steps)
+        b.set_observe(0, msd=True, t_control_step=10, temp_K=t_melt,
+                      t_control_until=hot_steps + settle_steps + 1)
+        b.step(settle_steps)
+        st = b.state(0)
+        msd = b.observed(0)["msd"][:2]
+    finally:
+        b.close()
+    return st["r"], st["v"], sp, msd
 
 
 def stream(seed=7, njobs=64, rate_hz=50.0, n_min=216, n_max=32768, steps_min=100,
