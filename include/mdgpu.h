@@ -154,6 +154,11 @@ int  mdg_batch_force_terms(mdg_batch *b, int terms);
 /* nsteps velocity-Verlet steps (next_rv_verlet + calc_kin + calc_press) on
  * every box; the step sequence is captured once in a hipGraph and replayed */
 int  mdg_batch_step(mdg_batch *b, int nsteps);
+/* No counterpart in mdc: captures, instantiates and uploads the 8-step
+ * graph that mdg_batch_step replays, executing no step, so that a timed
+ * or latency-bound stretch of steps that follows contains no capture.
+ * Membership or state changes that rebuild the work lists discard it. */
+int  mdg_batch_prepare(mdg_batch *b);
 int  mdg_batch_scalars(mdg_batch *b, int box, mdg_scalars *out);
 int  mdg_batch_sync(mdg_batch *b);
 

@@ -322,6 +322,11 @@ int mdg_batch_step(mdg_batch* b, int nsteps) {
   return b->set->step(nsteps, b->stream);
 }
 
+int mdg_batch_prepare(mdg_batch* b) {
+  if (!b) return MDG_E_ARG;
+  return b->set->prepare(b->stream);
+}
+
 int mdg_batch_set_observe(mdg_batch* b, int box, const mdg_observe* o) {
   if (!b || !o || box < 0 || box >= (int)b->boxes.size()) return MDG_E_ARG;
   ObserveOpts x;

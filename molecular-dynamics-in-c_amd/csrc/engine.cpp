@@ -1187,6 +1187,24 @@ void BoxSet::enqueue_scale_v(hipStream_t s, double scale) {
   launch_scale_v(d_desc_, d_items_ + all_.off[LIon], all_.n[LIon], scale, s);
 }
 
+// Captures the kGraphSteps-step graph, instantiates it and uploads it to
+// the device; no step is executed (the kernels only run when it is launched).
+int BoxSet::prepare(hipStream_t s) {
+  if (boxes_.empty()) return MDG_OK;
+  int rc = sync_desc(s);
+  if (rc != MDG_OK) return rc;
+  if (graph_exec_ && graph_steps_ == kGraphSteps) return MDG_OK;
+  invalidate_graph();
+  MDG_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+  for (int k = 0; k < kGraphSteps; ++k) enqueue_step(s, true);
+  MDG_HIP(hipStreamEndCapture(s, &graph_));
+  MDG_HIP(hipGraphInstantiate(&graph_exec_, graph_, nullptr, nullptr, 0));
+  graph_steps_ = kGraphSteps;
+  MDG_HIP(hipGraphUpload(graph_exec_, s));
+  MDG_HIP(hipStreamSynchronize(s));
+  return MDG_OK;
+}
+
 int BoxSet::step(int nsteps, hipStream_t s) {
   if (nsteps <= 0 || boxes_.empty()) return MDG_OK;
   int rc = sync_desc(s);
@@ -1194,19 +1212,15 @@ int BoxSet::step(int nsteps, hipStream_t s) {
   // Whole groups of 8 steps replay one captured graph; a shorter remainder
   // (or a call of fewer than 8 steps) is launched directly, so callers that
   // step by varying amounts do not re-capture.
-  const int G = 8;
+  const int G = kGraphSteps;
   if (nsteps < G) {
     for (int k = 0; k < nsteps; ++k) enqueue_step(s, true);
     MDG_HIP(hipGetLastError());
     return MDG_OK;
   }
   if (!graph_exec_ || graph_steps_ != G) {
-    invalidate_graph();
-    MDG_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-    for (int k = 0; k < G; ++k) enqueue_step(s, true);
-    MDG_HIP(hipStreamEndCapture(s, &graph_));
-    MDG_HIP(hipGraphInstantiate(&graph_exec_, graph_, nullptr, nullptr, 0));
-    graph_steps_ = G;
+    rc = prepare(s);
+    if (rc != MDG_OK) return rc;
   }
   int left = nsteps;
   while (left >= G) {

@@ -136,6 +136,9 @@ int box_write_observe(Box* b, const double* obs, const double* old, const double
 constexpr int kStepGroups = 2;
 constexpr double kGroup0Share = 0.5;  // share of the work in the group that starts k-space first
 
+// Steps per captured step graph (BoxSet::step replays it).
+constexpr int kGraphSteps = 8;
+
 // A set of boxes stepped together (one launch per kernel per step).
 class BoxSet {
  public:
@@ -167,6 +170,8 @@ class BoxSet {
   // launch_snapshot over every box (off: device array, one entry per box)
   void enqueue_snapshot(hipStream_t s, const long long* off, double* dst);
   int step(int nsteps, hipStream_t s);          // graph replay
+  // Captures and uploads the step graph that step() replays, running nothing.
+  int prepare(hipStream_t s);
   // Kernel timing over `nsteps` eager steps: every launch carries a start
   // and a stop hipEvent that its own dispatch stamps.  as_step: the step's own schedule
   // (box groups, side streams, kernels overlapping); else every kernel once

@@ -242,6 +242,7 @@ def lib():
             "mdg_batch_forces": ([vp], C.c_int),
             "mdg_batch_force_terms": ([vp, C.c_int], C.c_int),
             "mdg_batch_step": ([vp, C.c_int], C.c_int),
+            "mdg_batch_prepare": ([vp], C.c_int),
             "mdg_batch_scalars": ([vp, C.c_int, C.POINTER(Scalars)], C.c_int),
             "mdg_batch_sync": ([vp], C.c_int),
             "mdg_batch_profile": ([vp, C.c_int, C.c_int, C.c_void_p, _dp,
@@ -428,6 +429,10 @@ class Batch:
 
     def step(self, n):
         check(lib().mdg_batch_step(self.h, n), "step")
+
+    def prepare(self):
+        """Capture and upload the step graph without stepping."""
+        check(lib().mdg_batch_prepare(self.h), "prepare")
 
     def sync(self):
         check(lib().mdg_batch_sync(self.h), "sync")

@@ -14,12 +14,15 @@
  * control_temp(t_step, temp) while step < 10000 and moment_correction
  * every `moment` steps.
  *
- * usage: mdc_{cpu,gpu}[_material] n steps seed [t_step temp msd moment [a]]
+ * usage: mdc_{cpu,gpu}[_material] n steps seed [t_step temp msd moment [a [vtemp]]]
  *        (n x n x n cells of the material that control.c sets up, lattice
  *        constant a (default 5.64 A), alpha = 5.6 / L, hm = 27, cutoff
  *        L / 2, dt = 2 fs; t_step 0 = no thermostat, msd 0/1, moment 0 =
- *        no moment correction)
+ *        no moment correction; vtemp > 0: Maxwell-Boltzmann velocities at
+ *        vtemp K from the seed (Box-Muller on the seeded generator) in place
+ *        of the small uniform ones)
  */
+#include <math.h>
 #include <stdio.h>
 #include <stdlib.h>
 
@@ -43,17 +46,25 @@ static double uniform(void) /* in [0, 1) */
   return (double)(lcg_state >> 11) * (1.0 / 9007199254740992.0);
 }
 
+static double gauss(void) /* standard normal, Box-Muller */
+{
+  double u1 = uniform(), u2 = uniform();
+  if (u1 < 1e-300) u1 = 1e-300;
+  return sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2);
+}
+
 int main(int argc, char **argv)
 {
   int n, steps, i, t_step = 0, msd_on = 0, moment = 0, k;
-  double temp = 0.0;
+  double temp = 0.0, vtemp = 0.0;
   double a = 5.64, L, sx = 0, sy = 0, sz = 0;
 
-  if (argc != 4 && argc != 8 && argc != 9) {
-    fprintf(stderr, "usage: %s n steps seed [t_step temp msd moment [a]]\n", argv[0]);
+  if (argc != 4 && argc != 8 && argc != 9 && argc != 10) {
+    fprintf(stderr, "usage: %s n steps seed [t_step temp msd moment [a [vtemp]]]\n", argv[0]);
     return 2;
   }
-  if (argc == 9) a = atof(argv[8]);
+  if (argc >= 9) a = atof(argv[8]);
+  if (argc == 10) vtemp = atof(argv[9]);
   if (argc >= 8) {
     t_step = atoi(argv[4]);
     temp = atof(argv[5]);
@@ -74,9 +85,16 @@ int main(int argc, char **argv)
     if (sys.rx[i] < 0.0) sys.rx[i] += L;
     if (sys.ry[i] < 0.0) sys.ry[i] += L;
     if (sys.rz[i] < 0.0) sys.rz[i] += L;
-    sys.vx[i] = 0.05 * (uniform() - 0.5);
-    sys.vy[i] = 0.05 * (uniform() - 0.5);
-    sys.vz[i] = 0.05 * (uniform() - 0.5);
+    if (vtemp > 0.0) { /* sqrt(kB T / m) per component, kB as init.c:66 */
+      double s = sqrt(1.380658e-5 * vtemp / ion.m[sys.ion[i]]);
+      sys.vx[i] = s * gauss();
+      sys.vy[i] = s * gauss();
+      sys.vz[i] = s * gauss();
+    } else {
+      sys.vx[i] = 0.05 * (uniform() - 0.5);
+      sys.vy[i] = 0.05 * (uniform() - 0.5);
+      sys.vz[i] = 0.05 * (uniform() - 0.5);
+    }
   }
   moment_correction();
 
