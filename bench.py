@@ -386,6 +386,9 @@ def leg_nacl4096(steps=5000, cpu_steps=10):
     t0 = time.perf_counter()
     r, v, sp, msd = W.melt_snapshot(p)
     melt_s = time.perf_counter() - t0
+    # the autotuner's tilings for this box alone on the chip
+    tune = mdgpu.tune([p], mdgpu.FP64, context_ions=1)
+    tiling = mdgpu.box_tiling(p, mdgpu.FP64)
     b = mdgpu.Batch([p], mdgpu.FP64)
     b.set_state(0, r=r, v=v, species=sp)
     b.forces()
@@ -430,6 +433,7 @@ def leg_nacl4096(steps=5000, cpu_steps=10):
                         "mixed precision), cell lists rc=10 A, Ewald 1e-6 "
                         f"(alpha={p.alpha:.4f}, hm={p.hm}), FP64 table, NVE",
             "melt": {"seconds": melt_s, "msd_last_2ps_A2": [float(x) for x in msd]},
+            "autotune": {"seconds": tune["seconds"], "tiling": tiling},
             "steps": steps, "ms_per_step": ms, "ion_steps_per_s": 4096 / (ms * 1e-3),
             "max_rel_force_err_vs_cpu_at_snapshot": err,
             "energy_change_per_ion": (e1["etot"] - e0["etot"]) / 4096,

@@ -246,7 +246,9 @@ int  mdg_format_velocities(char *buf, size_t cap, int n, const int *species, con
  * (hipExtLaunchKernel).  Per kernel name:
  * ms_step = ms per step summed over its launches, launches = launches per
  * step (ms_step / launches is its average launch duration).  Up to `cap`
- * entries; the count is returned in *n. */
+ * entries; the count is returned in *n.  A negative nsteps times -nsteps
+ * steps with every kernel launched alone over all boxes, one after the
+ * other, which gives each kernel's duration without the others beside it. */
 int  mdg_batch_profile(mdg_batch *b, int nsteps, int cap, char names[][32],
                        double *ms_step, int *launches, int *n);
 /* Algorithmic work of one step of box `box`: accepted real-space pairs

@@ -382,11 +382,13 @@ int mdg_batch_scalars(mdg_batch* b, int box, mdg_scalars* out) {
 
 int mdg_batch_profile(mdg_batch* b, int nsteps, int cap, char names[][32], double* ms_step,
                       int* launches, int* n) {
-  if (!b || nsteps <= 0 || !names || !ms_step || !launches || !n) return MDG_E_ARG;
+  if (!b || nsteps == 0 || !names || !ms_step || !launches || !n) return MDG_E_ARG;
   std::vector<std::string> nm;
   std::vector<double> t;
   std::vector<int> c;
-  int rc = b->set->profile(nsteps, b->stream, true, &nm, &t, &c);
+  // nsteps < 0: -nsteps steps with every kernel alone over all boxes
+  const bool as_step = nsteps > 0;
+  int rc = b->set->profile(as_step ? nsteps : -nsteps, b->stream, as_step, &nm, &t, &c);
   if (rc != MDG_OK) return rc;
   int k = 0;
   for (; k < (int)nm.size() && k < cap; ++k) {

@@ -268,6 +268,54 @@ __device__ __forceinline__ void real_tile(const BoxDesc& B, const BoxCtx& c, con
       if (dr <= c.rc2m) m |= 1ull << u;
     }
   }
+  if constexpr (KIND == kRealTable64) {
+    // Table mode: each round a lane takes up to kTabBatch of its candidates
+    // in increasing u and issues their table reads together, then adds
+    // them in that same order, so an ion's force is summed exactly as in a
+    // one-pair-at-a-time loop while several reads are in flight.
+    constexpr int kTabBatch = 4;
+    while (__ballot(m != 0ull)) {
+      double2 pf[kTabBatch];
+      double ex[kTabBatch], ey[kTabBatch], ez[kTabBatch];
+      bool ok[kTabBatch], upper[kTabBatch];
+#pragma unroll
+      for (int q = 0; q < kTabBatch; ++q) {
+        ok[q] = false;
+        upper[q] = false;
+        ex[q] = ey[q] = ez[q] = 0.0;
+        pf[q] = make_double2(0.0, 0.0);
+        if (m == 0ull) continue;
+        const int u = __builtin_ctzll(m);
+        m &= m - 1ull;
+        if (jid[u] == h.i) continue;
+        double dx = h.xi - jx[u], dy = h.yi - jy[u], dz = h.zi - jz[u];
+        fold(c, dx, dy, dz);
+        const double dr = dx * dx + dy * dy + dz * dz;  // unfused, as mdc
+        if (!(dr <= c.rc2)) continue;
+        const int bin = (int)((sqrt(dr) - 0.5) * 1000.0);  // truncating, real.c:62
+        if (bin < 0 || bin >= c.nb) { err |= kErrRange; continue; }
+        const int s = jt[u];
+        const double2* row =
+            s == 0 ? h.t64[0] : (s == 1 ? h.t64[1] : (s == 2 ? h.t64[2] : h.t64[3]));
+        pf[q] = row[bin];
+        ok[q] = true;
+        upper[q] = jid[u] > h.i;
+        ex[q] = dx; ey[q] = dy; ez[q] = dz;
+      }
+#pragma unroll
+      for (int q = 0; q < kTabBatch; ++q) {
+        if (!ok[q]) continue;
+        const double dx = ex[q], dy = ey[q], dz = ez[q];
+        a.fx += pf[q].y * dx; a.fy += pf[q].y * dy; a.fz += pf[q].y * dz;
+        if (upper[q]) {
+          a.pot += pf[q].x;
+          a.vx += pf[q].y * dx * dx; a.vy += pf[q].y * dy * dy; a.vz += pf[q].y * dz * dz;
+          ++a.np;
+        }
+      }
+    }
+    return;
+  }
   while (__ballot(m != 0ull)) {
     if (m == 0ull) continue;
     const int u = __builtin_ctzll(m);
@@ -275,21 +323,7 @@ __device__ __forceinline__ void real_tile(const BoxDesc& B, const BoxCtx& c, con
     if (jid[u] == h.i) continue;
     double dx = h.xi - jx[u], dy = h.yi - jy[u], dz = h.zi - jz[u];
     fold(c, dx, dy, dz);
-    if constexpr (KIND == kRealTable64) {
-      const double dr = dx * dx + dy * dy + dz * dz;  // unfused, as mdc
-      if (!(dr <= c.rc2)) continue;
-      const int bin = (int)((sqrt(dr) - 0.5) * 1000.0);  // truncating, real.c:62
-      if (bin < 0 || bin >= c.nb) { err |= kErrRange; continue; }
-      const int s = jt[u];
-      const double2* row = s == 0 ? h.t64[0] : (s == 1 ? h.t64[1] : (s == 2 ? h.t64[2] : h.t64[3]));
-      const double2 pf = row[bin];
-      a.fx += pf.y * dx; a.fy += pf.y * dy; a.fz += pf.y * dz;
-      if (jid[u] > h.i) {
-        a.pot += pf.x;
-        a.vx += pf.y * dx * dx; a.vy += pf.y * dy * dy; a.vz += pf.y * dz * dz;
-        ++a.np;
-      }
-    } else if constexpr (KIND == kRealAnalytic64) {
+    if constexpr (KIND == kRealAnalytic64) {
       const double dr = dx * dx + dy * dy + dz * dz;
       if (!(dr <= c.rc2)) continue;
       const int s = jt[u];

@@ -490,9 +490,12 @@ class Batch:
         check(lib().mdg_batch_work(self.h, b, C.byref(p), C.byref(k), C.byref(n)), "work")
         return dict(pairs=p.value, nk=k.value, n_ions=n.value)
 
-    def profile(self, nsteps):
+    def profile(self, nsteps, alone=False):
         """Per kernel name: (ms per step over all its launches, launches per
-        step), timed in the step's own schedule."""
+        step), timed in the step's own schedule, or with every kernel
+        launched alone over all boxes (alone=True)."""
+        if alone:
+            nsteps = -nsteps
         cap = 16
         names = (C.c_char * 32 * cap)()
         ms = np.zeros(cap)
