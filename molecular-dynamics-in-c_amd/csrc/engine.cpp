@@ -133,7 +133,9 @@ static Tiling finish_tiling(const BoxPlan& P, bool can_cells, Tiling t) {
   if (t.cells) {
     for (int d = 0; d < 3; ++d)
       if ((int)floor(sp.L[d] / (sp.rc * t.cell_w * (1.0 + 1e-9))) < 3) t.cell_w = 1.0;
-    t.rsplit = std::max(1, std::min(t.rsplit, 27));
+    // FP64: the 27 cells around a home cell; mixed: the home cell and the
+    // 13 of its half shell (each pair once)
+    t.rsplit = std::max(1, std::min(t.rsplit, P.mixed ? 14 : 27));
   } else {
     t.rsplit = std::max(1, std::min(t.rsplit, std::max(1, N / 32)));
   }
@@ -194,6 +196,7 @@ static std::vector<int> partition_tiles(const std::vector<KfTile>& tiles, int gr
 int make_plan(const BoxSpec& sp, int precision, BoxPlan* P, std::string* why) {
   *P = BoxPlan();
   P->spec = sp;
+  P->mixed = precision == MDG_MIXED;
   if (sp.N <= 0 || !(sp.L[0] > 0) || !(sp.L[1] > 0) || !(sp.L[2] > 0) || !(sp.rc > 0)) {
     if (why) *why = "box needs N > 0, L > 0 and rc > 0";
     return MDG_E_ARG;
@@ -341,7 +344,7 @@ int make_plan(const BoxSpec& sp, int precision, BoxPlan* P, std::string* why) {
 
 // ------------------------------------------------------------------ layout
 struct Layout {
-  size_t ion[12], typ, rpf, cell_of, cnt, cwave, start, sorted, s[3], styp;
+  size_t ion[12], typ, rpf, fix, cell_of, cnt, cwave, start, sorted, s[3], styp;
   size_t cols, segs, quads, khz, kslot, kce, kcf, kab, kab32, kpot, tx, ty, tw, skp, fp;
   size_t wb, kfa, colw, kft, sb;
   size_t tab, tab32, rpart, rpairs, scal, err, msd_old, msd_d, obs, total;
@@ -365,6 +368,7 @@ static Layout layout(const BoxPlan& P, int precision) {
   for (int k = 0; k < 12; ++k) L.ion[k] = take(off, sizeof(double) * N);
   L.typ = take(off, sizeof(int) * N);
   if (P.tile.rsplit > 1) L.rpf = take(off, sizeof(double) * 3 * (size_t)P.tile.rsplit * N);
+  if (mixed) L.fix = take(off, sizeof(unsigned long long) * 3 * (size_t)N);
   if (!P.all_pairs) {
     L.cell_of = take(off, sizeof(int) * N);
     L.cnt = take(off, sizeof(int) * P.ncell);
@@ -531,6 +535,7 @@ static int bind_box(Box* b, char* base, std::string* why) {
   D.table_nb = spec.table_nb;
   D.rsplit = P.tile.rsplit;
   D.rpf = P.tile.rsplit > 1 ? (double*)(base + L.rpf) : nullptr;
+  D.fix = mixed ? (unsigned long long*)(base + L.fix) : nullptr;
   if (!P.all_pairs) {
     for (int d = 0; d < 3; ++d) {
       D.nc[d] = P.nc[d];
@@ -907,8 +912,17 @@ int BoxSet::set_boxes(const std::vector<Box*>& boxes) {
     if (D.msd_on || D.t_control_step > 0 || D.moment_every > 0) I[IObs][g].push_back(bi);
     if (P.all_pairs) {
       int slot = 0;
-      for (int i0 = 0; i0 < N; i0 += kRealThreads)
-        for (int s = 0; s < rs; ++s) real.push_back({bi, i0, slot++, s, 0});
+      if (mixed) {
+        // each pair once: a home block meets the ions from its own first
+        // on, in chunks of ceil(N / rs) (k_real_win), so every item has
+        // the same partner count at most
+        const int chunk = ceil_div(N, rs);
+        for (int i0 = 0; i0 < N; i0 += kRealThreads)
+          for (int s = 0; s < rs && i0 + s * chunk < N; ++s) real.push_back({bi, i0, slot++, s, 0});
+      } else {
+        for (int i0 = 0; i0 < N; i0 += kRealThreads)
+          for (int s = 0; s < rs; ++s) real.push_back({bi, i0, slot++, s, 0});
+      }
     } else {
       I[ICell][g].push_back(bi);
       int slot = 0;

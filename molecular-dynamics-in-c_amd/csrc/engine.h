@@ -67,6 +67,7 @@ struct BoxPlan {
   BoxSpec spec;
   bool ewald = false;               // alpha rule of rcprcl.c:23,115 (gates the self term)
   bool kspace = false;              // ewald and at least one k vector (k-space work)
+  bool mixed = false;               // planned for the mixed precision path
   int hm_sqrt = 0, hmax = 0, nk = 0, ncol = 0, nseg = 0;
   std::vector<KCol> cols;
   std::vector<KSeg> segs;

@@ -463,9 +463,20 @@ __global__ __launch_bounds__(kRealThreads) void k_real(const BoxDesc* __restrict
 }
 
 // ======================================================================
-// Real space, mixed precision: screened windows of partners.
+// Real space, mixed precision: screened windows of partners, each pair
+// evaluated once (Newton's third law, as mdc's i < j loop applies it,
+// mdc/program/real.c:70-75).
 //
-// The same pairs, pair function and per-ion FP64 sums as k_real's mixed
+// A home unit meets only the partners that follow it: all pairs, the ions
+// from its own first index on (j > i within it); cell lists, its own cell
+// (j > i) and the 13 cells of the half shell.  The home ion's share of a
+// pair force is summed in its lane in FP64; the partner's share goes into
+// 64-bit fixed-point accumulators (x 2^32) in LDS per window slot, then
+// into the box's fixed-point array with integer atomics.  Integer sums are
+// exact and order-free, so the result has the same bits whatever order the
+// atomics land in; k_force_vv adds the array to the home sums.
+//
+// The pairs, pair function and per-ion FP64 home sums are k_real's mixed
 // kinds, organised for issue rate:
 //  * Partners are staged kRealWin at a time (a window), so one lane's pop
 //    loop spans four tiles and the lanes' candidate counts even out.
@@ -479,12 +490,13 @@ __global__ __launch_bounds__(kRealThreads) void k_real(const BoxDesc* __restrict
 //  * A lane's candidates are bits of 32-bit words kept in LDS; the lane
 //    pops them in increasing partner order and applies the FP64
 //    minimum-image fold and the FP64 cutoff test, then the FP32 pair
-//    function.  Forces, energy and virial accumulate in FP64.
+//    function.  The pair force f d is formed once in FP32; the home sums,
+//    energy and virial accumulate it in FP64.
 // The fold is x - L rint(x / L), which equals mdc's strict-comparison
 // fold (mdc/program/real.c:43-49) except for a separation of exactly L/2
 // on an axis, where both images are at the same distance.
 // ======================================================================
-constexpr int kRealWin = 256;                 // partners per window
+constexpr int kRealWin = 128;                 // partners per window
 constexpr int kRealWinWords = kRealWin / 32;  // candidate words per lane
 
 struct alignas(16) WinRec {
@@ -519,6 +531,16 @@ __device__ __forceinline__ unsigned screen32(uint2 hq, const uint2* pq, int u0, 
   return m;
 }
 
+// x * 2^32 as a 64-bit two's complement integer (truncated below 2^-32):
+// floor(x) in the upper word, the fraction (exact in FP32) scaled into the
+// lower one.  |x| is clamped to 1e9 (a force far beyond any pair's).
+__device__ __forceinline__ unsigned long long to_fix(float x) {
+  x = fminf(fmaxf(x, -1.0e9f), 1.0e9f);
+  const float hi = floorf(x);
+  const float lo = (x - hi) * 4294967296.0f;  // in [0, 2^32)
+  return ((unsigned long long)(long long)(int)hi << 32) + (unsigned long long)(unsigned)lo;
+}
+
 template <int KIND>
 __global__ __launch_bounds__(kRealThreads) void k_real_win(const BoxDesc* __restrict__ boxes,
                                                            const Item* __restrict__ items,
@@ -527,6 +549,7 @@ __global__ __launch_bounds__(kRealThreads) void k_real_win(const BoxDesc* __rest
   __shared__ WinRec pr[kRealWin];
   __shared__ uint2 pq[kRealWin];
   __shared__ unsigned pm[kRealWinWords][kRealThreads];
+  __shared__ unsigned long long pacc[3][kRealWin];  // partner shares, fixed point
   __shared__ int ncum[28], nst[27];
   const Item it = items[blockIdx.x];
   const BoxDesc& B = boxes[it.box];
@@ -554,18 +577,26 @@ __global__ __launch_bounds__(kRealThreads) void k_real_win(const BoxDesc* __rest
     thr1 = screen_all ? 0 : (int)(s * s) + 1;
   }
   RealAcc tot;
-  // partner source: all pairs -> index range; cells -> run of neighbour cells
-  int hs, he, pbeg, pend;
+  // partner source: all pairs -> index range; cells -> run of neighbour cells.
+  // Partners before self_end (a run index) pair with the home unit's own
+  // ions, so only those of higher index are taken.
+  int hs, he, pbeg, pend, self_end = 0x7fffffff;
   int cx = 0, cy = 0, cz = 0;
   if (B.all_pairs) {
+    // partners from the unit's first ion on, in chunks of ceil(N / nsplit)
+    // (a block gets as many items as its run needs; the home-side partial
+    // slots it does not reach stay zero)
     hs = it.a; he = min(N, it.a + kRealThreads);
-    pbeg = (int)((long)split * N / nsplit); pend = (int)((long)(split + 1) * N / nsplit);
+    const int chunk = (N + nsplit - 1) / nsplit;
+    pbeg = hs + split * chunk; pend = min(N, pbeg + chunk);
   } else {
     const int cell = it.a;
     const int ncx = B.nc[0], ncy = B.nc[1], ncz = B.nc[2];
     cx = cell % ncx; cy = (cell / ncx) % ncy; cz = cell / (ncx * ncy);
     hs = B.cell_start[cell]; he = B.cell_start[cell + 1];
-    const int n0 = split * 27 / nsplit, n1 = (split + 1) * 27 / nsplit;
+    // stencil cells 13 (the home cell) .. 26: the half shell, split into runs
+    const int n0 = 13 + split * 14 / nsplit, n1 = 13 + (split + 1) * 14 / nsplit;
+    self_end = n0 == 13 ? he - hs : 0;
     const int n = n0 + lane;
     int cnt = 0, st = 0;
     if (n < n1) {
@@ -627,6 +658,7 @@ __global__ __launch_bounds__(kRealThreads) void k_real_win(const BoxDesc* __rest
         r.x = x; r.y = y; r.z = z; r.id = j; r.t = t;
         pr[s] = r;
         pq[s] = quantize(x, y, z, sx, sy, sz);
+        pacc[0][s] = 0ull; pacc[1][s] = 0ull; pacc[2][s] = 0ull;
       }
       __syncthreads();
       // screen: one candidate word per 32 partners
@@ -642,9 +674,9 @@ __global__ __launch_bounds__(kRealThreads) void k_real_win(const BoxDesc* __rest
         nz |= (m != 0u ? 1u : 0u) << w;
       }
       // Evaluate: each lane pops its own candidates in increasing order.
-      // The body is branch-free; a lane out of candidates evaluates a
-      // discarded pair.  Energy and virial are summed over ordered pairs
-      // and halved once per item.
+      // The body is branch-free but for the partner's atomics; a lane out of
+      // candidates evaluates a discarded pair.  Energy and virial count each
+      // evaluated (unordered) pair once.
       int w = nz ? __builtin_ctz(nz) : 0;
       unsigned cur = nz ? pm[w][lane] : 0u;
       if (__ballot(cur != 0u)) do {
@@ -655,7 +687,8 @@ __global__ __launch_bounds__(kRealThreads) void k_real_win(const BoxDesc* __rest
         // its discarded pair is weighted by zero, and a slot the window did
         // not stage may hold another kernel's LDS bits (a NaN as FP64 would
         // survive the multiplication by zero).
-        const WinRec P = pr[live ? w * 32 + b : 0];
+        const int slot = live ? w * 32 + b : 0;
+        const WinRec P = pr[slot];
         if (live && cur == 0u) {
           nz &= ~(1u << w);
           if (nz) { w = __builtin_ctz(nz); cur = pm[w][lane]; }
@@ -664,7 +697,9 @@ __global__ __launch_bounds__(kRealThreads) void k_real_win(const BoxDesc* __rest
         dx = fma(-c.Lx, rint(dx * ilx), dx);
         dy = fma(-c.Ly, rint(dy * ily), dy);
         dz = fma(-c.Lz, rint(dz * ilz), dz);
-        bool ok = live && P.id != h.i;
+        // each unordered pair once: by index within the home unit, all of
+        // the half shell's
+        bool ok = live && (P.id > h.i || w0 + slot >= self_end);
         float p, f;
         if constexpr (KIND == kRealAnalytic32) {
           const double dr = fma(dx, dx, fma(dy, dy, dz * dz));
@@ -683,16 +718,35 @@ __global__ __launch_bounds__(kRealThreads) void k_real_win(const BoxDesc* __rest
           const float2 pf = row[bin];
           p = pf.x; f = pf.y;
         }
-        const double fd = ok ? (double)f : 0.0;
-        a.fx = fma(fd, dx, a.fx); a.fy = fma(fd, dy, a.fy); a.fz = fma(fd, dz, a.fz);
+        // the pair force f d once, in FP32; +f d on the home ion, -f d on
+        // the partner (exact fixed point)
+        const float fs = ok ? f : 0.f;
+        const float gx = fs * (float)dx, gy = fs * (float)dy, gz = fs * (float)dz;
+        a.fx += (double)gx; a.fy += (double)gy; a.fz += (double)gz;
         a.pot += ok ? (double)p : 0.0;
-        a.vx = fma(fd * dx, dx, a.vx); a.vy = fma(fd * dy, dy, a.vy); a.vz = fma(fd * dz, dz, a.vz);
-        a.np += (ok && P.id > h.i) ? 1u : 0u;
+        a.vx = fma((double)gx, dx, a.vx); a.vy = fma((double)gy, dy, a.vy);
+        a.vz = fma((double)gz, dz, a.vz);
+        a.np += ok ? 1u : 0u;
+        if (ok) {
+          atomicAdd(&pacc[0][slot], to_fix(-gx));
+          atomicAdd(&pacc[1][slot], to_fix(-gy));
+          atomicAdd(&pacc[2][slot], to_fix(-gz));
+        }
       } while (__ballot(cur != 0u));
+      __syncthreads();
+      // the window's partner shares into the box's fixed-point sums
+      for (int s = lane; s < cnt; s += kRealThreads) {
+        const size_t id = (size_t)pr[s].id;
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+          const unsigned long long v = pacc[q][s];
+          if (v) atomicAdd(&B.fix[q * (size_t)N + id], v);
+        }
+      }
       __syncthreads();
     }
     real_store(B, h, split, mode, a);
-    tot.pot += 0.5 * a.pot; tot.vx += 0.5 * a.vx; tot.vy += 0.5 * a.vy; tot.vz += 0.5 * a.vz;
+    tot.pot += a.pot; tot.vx += a.vx; tot.vy += a.vy; tot.vz += a.vz;
     tot.np += a.np;
   }
   const double pot = wave_sum(tot.pot), vx = wave_sum(tot.vx), vy = wave_sum(tot.vy),

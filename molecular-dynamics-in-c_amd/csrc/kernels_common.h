@@ -240,6 +240,15 @@ __global__ __launch_bounds__(kIonThreads) void k_force_vv(const BoxDesc* __restr
   } else {
     fx = B.fx[i]; fy = B.fy[i]; fz = B.fz[i];
   }
+  if (B.fix && (terms & 1)) {
+    // the partner shares of the mixed real space, exact in fixed point;
+    // cleared for the next evaluation
+    const double s = 1.0 / kFixScale;
+    fx += (double)(long long)B.fix[i] * s;
+    fy += (double)(long long)B.fix[N + i] * s;
+    fz += (double)(long long)B.fix[2 * N + i] * s;
+    B.fix[i] = 0ull; B.fix[N + i] = 0ull; B.fix[2 * N + i] = 0ull;
+  }
   if (B.kspace && (terms & 2)) {
     for (int g = 0; g < B.ngroup; ++g) {
       const double* fp = B.fpart + (size_t)g * 3 * N;

@@ -31,6 +31,8 @@ constexpr int kScalThreads = 256;   // per-box scalar reduction
 // error bits written by kernels into BoxDesc::err
 constexpr int kErrState = 1;        // non-finite / out-of-box coordinate
 constexpr int kErrRange = 2;        // pair closer than the table start
+// scale of the fixed-point real-space partner sums (BoxDesc::fix)
+constexpr double kFixScale = 4294967296.0;  // 2^32
 
 // One column of the half-sphere k set: fixed (hx, hy), hz = hz0 .. hz0+len-1.
 // Columns are ordered by hx, then hy, as mdc's loops visit them
@@ -105,6 +107,10 @@ struct BoxDesc {
   // ---- real-space split: partners divided over rsplit items per home unit ----
   int rsplit;         // 1: forces written in place (mdc's order); >1: partials
   double *rpf;        // [rsplit][3][N] partial real-space forces (rsplit > 1)
+  // mixed path: each pair is evaluated once; the partner's share of its
+  // force is summed here as 64-bit fixed point (x 2^32, two's complement),
+  // [3][N], exactly and in any order; k_force_vv adds it and clears it
+  unsigned long long *fix;
   // ---- cell list ----
   int nc[3], ncell;
   double cellscale[3];  // nc / L
