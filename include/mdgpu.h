@@ -102,6 +102,12 @@ int  mdg_maxwell(unsigned long long seed, double temp_k, int n, const int *speci
 /* Ewald split for a target relative error eps: alpha = s/rc and
  * hm = floor((s alpha L_max / pi)^2) with s = sqrt(-ln eps). */
 int  mdg_ewald_params(double rc, double l_max, double eps, double *alpha, int *hm);
+/* No counterpart in mdc (mdc fixes rc per run): a real-space cutoff for a
+ * box of n_ions rock-salt ions whose Ewald split mdg_ewald_params then
+ * makes, chosen by measurement on MI355X: rc = 16.5 A (n_ions / 32768)^(1/6)
+ * (the pair sum and the k sum balance as N^(1/6) at fixed density), at
+ * least 10 A, at most rc_cap and L_min / 2. */
+int  mdg_ewald_rc_rule(int n_ions, double l_min, double rc_cap, double *rc);
 
 /* ---------------------------------------------------------------- batch */
 /* nbox independent boxes in one precision mode.  Each box's arithmetic
@@ -309,6 +315,8 @@ typedef struct {
   int    one_at_a_time; /* 1: run jobs alone in arrival order (baseline)   */
   int    realtime;      /* 1: jobs become visible at their arrival times    */
   int    steps_per_tick;/* steps per batched launch between admissions     */
+  int    auto_split;    /* 1: rc from mdg_ewald_rc_rule(N, L_min, rc_max)
+                           instead of min(rc_max, L_min / 2)               */
 } mdg_serve_config;
 
 typedef struct {
@@ -330,7 +338,8 @@ typedef struct {
 int  mdg_serve(const mdg_job *jobs, int njobs, const mdg_serve_config *cfg,
                mdg_job_result *results, mdg_serve_stats *stats);
 /* the box parameters mdg_serve gives a job (NaCl, a = 5.64 A, Ewald split
- * for cfg->eps with rc = min(cfg->rc_max, L_min / 2)), e.g. to tune them */
+ * for cfg->eps with rc = min(cfg->rc_max, L_min / 2), or by
+ * mdg_ewald_rc_rule when cfg->auto_split), e.g. to tune them */
 int  mdg_serve_job_params(const mdg_job *job, const mdg_serve_config *cfg,
                           mdg_box_params *out);
 

@@ -334,7 +334,12 @@ int make_plan(const BoxSpec& sp, int precision, BoxPlan* P, std::string* why) {
   P->all_pairs = !P->tile.cells;
   if (!P->all_pairs) P->ncell = P->nc[0] * P->nc[1] * P->nc[2];
   else P->nc[0] = P->nc[1] = P->nc[2] = 0;
-  const int homes = P->all_pairs ? ceil_div(sp.N, kRealThreads) : P->ncell;
+  // a cell's home ions in blocks of 64 spread over cell_hb items (block b
+  // goes to item b mod cell_hb), from the mean population: large cells
+  // (large cutoffs) do not serialise one item
+  P->cell_hb = P->all_pairs ? 1 : std::max(1, ceil_div(ceil_div(sp.N, std::max(1, P->ncell)),
+                                                       kRealThreads));
+  const int homes = P->all_pairs ? ceil_div(sp.N, kRealThreads) : P->ncell * P->cell_hb;
   P->nreal = homes * P->tile.rsplit;
   if (P->kspace)
     P->kf_group_start = precision == MDG_MIXED ? partition_tiles(P->kftiles, P->tile.ngroup)
@@ -542,6 +547,7 @@ static int bind_box(Box* b, char* base, std::string* why) {
       D.cellscale[d] = (double)P.nc[d] / spec.L[d];
     }
     D.ncell = P.ncell;
+    D.cell_hb = P.cell_hb;
     D.cell_of = (int*)(base + L.cell_of);
     D.cell_count = (int*)(base + L.cnt);
     D.cell_wave = (int*)(base + L.cwave);
@@ -927,7 +933,8 @@ int BoxSet::set_boxes(const std::vector<Box*>& boxes) {
       I[ICell][g].push_back(bi);
       int slot = 0;
       for (int c = 0; c < P.ncell; ++c)
-        for (int s = 0; s < rs; ++s) real.push_back({bi, c, slot++, s, 0});
+        for (int hb = 0; hb < P.cell_hb; ++hb)
+          for (int s = 0; s < rs; ++s) real.push_back({bi, c, slot++, s, hb});
     }
     for (int i0 = 0; i0 < N; i0 += kIonThreads) L[LIon][g].push_back({bi, i0, 0, 0, 0});
     if (P.kspace) {
@@ -1219,15 +1226,16 @@ int BoxSet::prepare(hipStream_t s) {
   return MDG_OK;
 }
 
-int BoxSet::step(int nsteps, hipStream_t s) {
+int BoxSet::step(int nsteps, hipStream_t s, bool use_graph) {
   if (nsteps <= 0 || boxes_.empty()) return MDG_OK;
   int rc = sync_desc(s);
   if (rc != MDG_OK) return rc;
   // Whole groups of 8 steps replay one captured graph; a shorter remainder
   // (or a call of fewer than 8 steps) is launched directly, so callers that
-  // step by varying amounts do not re-capture.
+  // step by varying amounts do not re-capture.  use_graph = false launches
+  // every step directly (no capture for a set that is about to change).
   const int G = kGraphSteps;
-  if (nsteps < G) {
+  if (nsteps < G || (!use_graph && !graph_exec_)) {
     for (int k = 0; k < nsteps; ++k) enqueue_step(s, true);
     MDG_HIP(hipGetLastError());
     return MDG_OK;

@@ -78,6 +78,7 @@ struct BoxPlan {
   bool all_pairs = true;
   bool cells_ok = false;            // >= 3 cells of width rc per axis
   int nc[3] = {0, 0, 0}, ncell = 0;
+  int cell_hb = 1;                  // home blocks of 64 per cell, one item each (strided)
   int nreal = 0;
   Tiling tile;
   // back-projection groups: first quad (FP64) or first MFMA tile (mixed) of
@@ -170,7 +171,7 @@ class BoxSet {
   void enqueue_scale_v(hipStream_t s, double scale);
   // launch_snapshot over every box (off: device array, one entry per box)
   void enqueue_snapshot(hipStream_t s, const long long* off, double* dst);
-  int step(int nsteps, hipStream_t s);          // graph replay
+  int step(int nsteps, hipStream_t s, bool use_graph = true);  // graph replay
   // Captures and uploads the step graph that step() replays, running nothing.
   int prepare(hipStream_t s);
   // Kernel timing over `nsteps` eager steps: every launch carries a start

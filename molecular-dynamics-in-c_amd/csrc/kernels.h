@@ -413,7 +413,8 @@ __global__ __launch_bounds__(kRealThreads) void k_real(const BoxDesc* __restrict
     const int cx = cell % ncx, cy = (cell / ncx) % ncy, cz = cell / (ncx * ncy);
     const int hs = B.cell_start[cell], he = B.cell_start[cell + 1];
     const int n0 = split * 27 / nsplit, n1 = (split + 1) * 27 / nsplit;
-    for (int h0 = hs; h0 < he; h0 += kRealThreads) {
+    // this item's home blocks: it.d, it.d + cell_hb, ...
+    for (int h0 = hs + it.d * kRealThreads; h0 < he; h0 += B.cell_hb * kRealThreads) {
       const int hp = h0 + lane;
       const bool valid = hp < he;
       const int hq = valid ? hp : hs;
@@ -616,7 +617,9 @@ __global__ __launch_bounds__(kRealThreads) void k_real_win(const BoxDesc* __rest
     pbeg = 0; pend = __shfl(inc, 26, 64);
     __syncthreads();
   }
-  for (int h0 = hs; h0 < he; h0 += kRealThreads) {
+  // cells: this item's home blocks it.d, it.d + cell_hb, ... of its cell
+  const int hstep = B.all_pairs ? kRealThreads : B.cell_hb * kRealThreads;
+  for (int h0 = hs + (B.all_pairs ? 0 : it.d * kRealThreads); h0 < he; h0 += hstep) {
     const int hp = h0 + lane;
     const bool valid = hp < he;
     const int hq = valid ? hp : hs;

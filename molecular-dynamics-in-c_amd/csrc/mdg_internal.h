@@ -113,6 +113,7 @@ struct BoxDesc {
   unsigned long long *fix;
   // ---- cell list ----
   int nc[3], ncell;
+  int cell_hb;        // home blocks per cell item stride (item d: first block)
   double cellscale[3];  // nc / L
   int *cell_of;       // [N]
   int *cell_count;    // [ncell]

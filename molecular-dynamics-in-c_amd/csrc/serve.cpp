@@ -33,6 +33,10 @@ int spec_from_params(const mdg_box_params& p, BoxSpec* s, std::vector<double>* p
 
 namespace {
 
+// A membership that lasts at least this many steps runs from a captured
+// step graph; shorter ones launch their steps directly.
+constexpr int kGraphHorizon = 32;
+
 struct Running {
   int job;
   Box* box;
@@ -46,7 +50,7 @@ double now_s(std::chrono::steady_clock::time_point t0) {
 }
 
 // NaCl box of the job (a = 5.64 A), Ewald split for cfg->eps with
-// rc = min(cfg->rc_max, L_min/2).
+// rc = min(cfg->rc_max, L_min/2), or mdg_ewald_rc_rule's with auto_split.
 int job_params(const mdg_job& j, const mdg_serve_config& c, mdg_box_params* p) {
   memset(p, 0, sizeof(*p));
   if (j.nx <= 0 || j.ny <= 0 || j.nz <= 0 || j.steps <= 0 || !(j.temp_k >= 0)) return MDG_E_ARG;
@@ -55,6 +59,10 @@ int job_params(const mdg_job& j, const mdg_serve_config& c, mdg_box_params* p) {
   const double Lmin = 5.64 * std::min(j.nx, std::min(j.ny, j.nz));
   const double Lmax = 5.64 * std::max(j.nx, std::max(j.ny, j.nz));
   p->rc = std::min(c.rc_max, 0.5 * Lmin);
+  if (c.auto_split) {
+    const int rc0 = mdg_ewald_rc_rule(8 * j.nx * j.ny * j.nz, Lmin, c.rc_max, &p->rc);
+    if (rc0 != MDG_OK) return rc0;
+  }
   int rc = mdg_ewald_params(p->rc, Lmax, c.eps, &p->alpha, &p->hm);
   if (rc != MDG_OK) return rc;
   p->dt_fs = c.dt_fs;
@@ -113,6 +121,16 @@ extern "C" int mdg_serve(const mdg_job* jobs, int njobs, const mdg_serve_config*
 
   hipStream_t stream = nullptr;
   if (hipStreamCreateWithFlags(&stream, hipStreamNonBlocking) != hipSuccess) return MDG_E_HIP;
+  // the end of the last two ticks (the host runs at most two ahead)
+  hipEvent_t ev[2] = {nullptr, nullptr};
+  for (auto& e : ev)
+    if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) {
+      for (auto& f : ev)
+        if (f) (void)hipEventDestroy(f);
+      (void)hipStreamDestroy(stream);
+      return MDG_E_HIP;
+    }
+  long long ticks_enq = 0;
   BoxSet set(cfg->precision, cfg->real_mode);
   std::vector<Running> run;
   std::deque<int> waiting;
@@ -140,8 +158,16 @@ extern "C" int mdg_serve(const mdg_job* jobs, int njobs, const mdg_serve_config*
       }
       waiting.push_back(j);
     }
-    // admission in arrival order while a slot and the budget allow
+    // admission in arrival order while a slot and the budget allow; it
+    // rewrites the work lists, so the device must have finished the ticks
+    // in flight first
     bool changed = false;
+    if (!waiting.empty() && (int)run.size() < slots &&
+        resident + need[waiting.front()] <= cfg->hbm_budget &&
+        hipStreamSynchronize(stream) != hipSuccess) {
+      rc_all = MDG_E_HIP;
+      break;
+    }
     while (!waiting.empty() && (int)run.size() < slots &&
            resident + need[waiting.front()] <= cfg->hbm_budget) {
       const int j = waiting.front();
@@ -193,17 +219,37 @@ extern "C" int mdg_serve(const mdg_job* jobs, int njobs, const mdg_serve_config*
     // a tick right after an admission is one step long, so every job's first
     // energy is taken after exactly one step whatever the mix
     int K = changed ? 1 : per_tick;
-    for (auto& x : run) K = std::min(K, x.left);
-    int rc = set.step(K, stream);
-    if (rc == MDG_OK && hipStreamSynchronize(stream) != hipSuccess) rc = MDG_E_HIP;
+    int horizon = 1 << 30;  // steps until the first resident job leaves
+    for (auto& x : run) {
+      K = std::min(K, x.left);
+      horizon = std::min(horizon, x.left);
+    }
+    // a captured step graph pays for its capture when the membership lasts;
+    // shorter stretches run eagerly
+    int rc = set.step(K, stream, horizon >= kGraphHorizon);
+    if (rc == MDG_OK && hipEventRecord(ev[ticks_enq % 2], stream) != hipSuccess) rc = MDG_E_HIP;
     if (rc != MDG_OK) { rc_all = rc; break; }
+    ++ticks_enq;
     ++stats->ticks;
+    bool need_results = false;
+    for (auto& x : run) {
+      x.left -= K;
+      ion_steps += (double)x.ions * K;
+      need_results |= !x.first_done || x.left == 0;
+    }
+    if (!need_results) {
+      // nothing to read back: keep at most two ticks in flight
+      if (ticks_enq >= 2 && hipEventSynchronize(ev[ticks_enq % 2]) != hipSuccess) {
+        rc_all = MDG_E_HIP;
+        break;
+      }
+      continue;
+    }
+    if (hipStreamSynchronize(stream) != hipSuccess) { rc_all = MDG_E_HIP; break; }
     // bookkeeping: first energies, finished jobs leave
     std::vector<Running> keep;
     bool left_any = false;
     for (auto& x : run) {
-      x.left -= K;
-      ion_steps += (double)x.ions * K;
       double S[kScalCount];
       mdg_job_result& R = results[x.job];
       if (!x.first_done || x.left == 0) {
@@ -247,7 +293,9 @@ extern "C" int mdg_serve(const mdg_job* jobs, int njobs, const mdg_serve_config*
       set.set_boxes({});
     }
   }
+  (void)hipStreamSynchronize(stream);
   for (auto& x : run) box_destroy(x.box);
+  for (auto& e : ev) (void)hipEventDestroy(e);
   (void)hipStreamDestroy(stream);
 
   stats->wall_s = now_s(t0);

@@ -65,7 +65,7 @@ class ServeConfig(C.Structure):
     _fields_ = [("hbm_budget", C.c_size_t), ("max_slots", C.c_int), ("precision", C.c_int),
                 ("real_mode", C.c_int), ("eps", C.c_double), ("rc_max", C.c_double),
                 ("dt_fs", C.c_double), ("one_at_a_time", C.c_int), ("realtime", C.c_int),
-                ("steps_per_tick", C.c_int)]
+                ("steps_per_tick", C.c_int), ("auto_split", C.c_int)]
 
 
 class JobResult(C.Structure):
@@ -256,6 +256,8 @@ def lib():
                           C.POINTER(TuneReport)], C.c_int),
             "mdg_tune_clear": ([], C.c_int),
             "mdg_copy_probe": ([C.c_size_t, C.c_int, C.POINTER(C.c_double)], C.c_int),
+            "mdg_ewald_rc_rule": ([C.c_int, C.c_double, C.c_double, C.POINTER(C.c_double)],
+                                  C.c_int),
             "mdg_serve_job_params": ([C.POINTER(Job), C.POINTER(ServeConfig),
                                       C.POINTER(BoxParams)], C.c_int),
             "mdg_tune_save": ([C.c_char_p], C.c_int),
@@ -325,6 +327,13 @@ def ewald_params(rc, l_max, eps):
     a, h = C.c_double(), C.c_int()
     check(lib().mdg_ewald_params(rc, l_max, eps, C.byref(a), C.byref(h)), "ewald_params")
     return a.value, h.value
+
+
+def ewald_rc_rule(n_ions, l_min, rc_cap):
+    """The library's real-space cutoff rule (mdg_ewald_rc_rule)."""
+    rc = C.c_double()
+    check(lib().mdg_ewald_rc_rule(n_ions, l_min, rc_cap, C.byref(rc)), "ewald_rc_rule")
+    return rc.value
 
 
 def rocksalt(nx, ny, nz, a):

@@ -50,6 +50,12 @@ def kcl32768(rc_max=10.0):
     return ewald_box(16, 16, 16, A_KCL, "KCl", rc_max=rc_max, cells=mdgpu.CELL_LIST)
 
 
+def kcl32768_split():
+    """kcl32768 with the cutoff of the library's rule (mdg_ewald_rc_rule,
+    16.5 A at 32768 ions) instead of 10 A, same 1e-6 Ewald accuracy."""
+    return kcl32768(rc_max=mdgpu.ewald_rc_rule(32768, 16 * A_KCL, 30.0))
+
+
 def cells_for(n_ions):
     """Near-cubic nx x ny x nz conventional cells holding about n_ions."""
     target = max(1.0, n_ions / 8.0)
