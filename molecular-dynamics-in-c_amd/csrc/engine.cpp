@@ -349,10 +349,10 @@ int make_plan(const BoxSpec& sp, int precision, BoxPlan* P, std::string* why) {
 
 // ------------------------------------------------------------------ layout
 struct Layout {
-  size_t ion[12], typ, rpf, fix, cell_of, cnt, cwave, start, sorted, s[3], styp;
+  size_t ion[12], typ, rpf, fix, cell_of, cnt, ctmp, start, sorted, s[3], styp;
   size_t cols, segs, quads, khz, kslot, kce, kcf, kab, kab32, kpot, tx, ty, tw, skp, fp;
   size_t wb, kfa, colw, kft, sb;
-  size_t tab, tab32, rpart, rpairs, scal, err, msd_old, msd_d, obs, total;
+  size_t tab, tab32, rpart, rpairs, spart, scal, err, msd_old, msd_d, obs, total;
 };
 
 static size_t take(size_t& off, size_t bytes) {
@@ -377,7 +377,7 @@ static Layout layout(const BoxPlan& P, int precision) {
   if (!P.all_pairs) {
     L.cell_of = take(off, sizeof(int) * N);
     L.cnt = take(off, sizeof(int) * P.ncell);
-    L.cwave = take(off, sizeof(int) * (size_t)kCellWaves * P.ncell);
+    L.ctmp = take(off, sizeof(int) * N);
     L.start = take(off, sizeof(int) * (P.ncell + 1));
     L.sorted = take(off, sizeof(int) * N);
     for (int d = 0; d < 3; ++d) L.s[d] = take(off, sizeof(double) * N);
@@ -416,6 +416,7 @@ static Layout layout(const BoxPlan& P, int precision) {
   }
   L.rpart = take(off, sizeof(double) * 4 * P.nreal);
   L.rpairs = take(off, sizeof(unsigned long long) * P.nreal);
+  L.spart = take(off, sizeof(double) * 9 * (size_t)ceil_div((int)N, kIonThreads));
   L.scal = take(off, sizeof(double) * kScalCount);
   L.err = take(off, sizeof(int));
   L.msd_old = take(off, sizeof(double) * 3 * N);
@@ -550,7 +551,7 @@ static int bind_box(Box* b, char* base, std::string* why) {
     D.cell_hb = P.cell_hb;
     D.cell_of = (int*)(base + L.cell_of);
     D.cell_count = (int*)(base + L.cnt);
-    D.cell_wave = (int*)(base + L.cwave);
+    D.cell_tmp = (int*)(base + L.ctmp);
     D.cell_start = (int*)(base + L.start);
     D.sorted = (int*)(base + L.sorted);
     D.sx = (double*)(base + L.s[0]);
@@ -625,6 +626,8 @@ static int bind_box(Box* b, char* base, std::string* why) {
   D.rpart = (double*)(base + L.rpart);
   D.rpairs = (unsigned long long*)(base + L.rpairs);
   D.nreal = P.nreal;
+  D.spart = (double*)(base + L.spart);
+  D.nspart = ceil_div(N, kIonThreads);
   D.scal = (double*)(base + L.scal);
   D.err = (int*)(base + L.err);
   D.msd_old = (double*)(base + L.msd_old);
@@ -1091,7 +1094,10 @@ void BoxSet::timed(int kernel, hipStream_t, F&& launch) {
 void BoxSet::launch_cells_real(hipStream_t s, int mode, const Ranges& r) {
   const bool analytic = real_mode_ == MDG_ANALYTIC;
   if (r.in[ICell])
-    timed(KCell, s, [&] { launch_cell_build(d_desc_, d_ids_ + r.ioff[ICell], r.in[ICell], s); });
+    timed(KCell, s, [&] {
+      launch_cell_build(d_desc_, d_ids_ + r.ioff[ICell], r.in[ICell], d_items_ + r.off[LIon],
+                        r.n[LIon], s);
+    });
   timed(KReal, s, [&] {
     if (precision_ == MDG_MIXED)
       launch_real32(analytic, d_desc_, d_items_ + r.off[LReal], r.n[LReal], mode, s);
@@ -1173,7 +1179,8 @@ void BoxSet::enqueue_step(hipStream_t s, bool vv) {
   timed(KVv, s,
         [&] { launch_force_vv(vv, d_desc_, d_items_ + all_.off[LIon], all_.n[LIon], s); });
   timed(KScal, s, [&] {
-    launch_box_scalars(d_desc_, d_ids_ + all_.ioff[IAll], all_.in[IAll], vv ? 1 : 0, s);
+    launch_box_scalars(d_desc_, d_ids_ + all_.ioff[IAll], all_.in[IAll], d_items_ + all_.off[LIon],
+                       all_.n[LIon], vv ? 1 : 0, s);
   });
   // mdc's calc_msd, control_temp and moment_correction of the boxes that
   // have them armed (main.c:44-60)
@@ -1184,7 +1191,8 @@ void BoxSet::enqueue_step(hipStream_t s, bool vv) {
 }
 
 void BoxSet::enqueue_scalars(hipStream_t s, bool count_step) {
-  launch_box_scalars(d_desc_, d_ids_ + all_.ioff[IAll], all_.in[IAll], count_step ? 1 : 0, s);
+  launch_box_scalars(d_desc_, d_ids_ + all_.ioff[IAll], all_.in[IAll], d_items_ + all_.off[LIon],
+                     all_.n[LIon], count_step ? 1 : 0, s);
 }
 void BoxSet::enqueue_clear_foc(hipStream_t s) {
   launch_clear_foc(d_desc_, d_items_ + all_.off[LIon], all_.n[LIon], s);

@@ -18,8 +18,13 @@ int kforce_hz_for(int hmax) {
   return -1;
 }
 
-void launch_cell_build(const BoxDesc* boxes, const int* ids, int n, hipStream_t s) {
-  if (n > 0) MDG_LAUNCH(k_cell_build, MDG_GRID(n, kCellThreads), boxes, ids);
+void launch_cell_build(const BoxDesc* boxes, const int* ids, int nids, const Item* items,
+                       int nitems, hipStream_t s) {
+  if (nids <= 0 || nitems <= 0) return;
+  MDG_LAUNCH(k_cell_bin, MDG_GRID(nitems, kIonThreads), boxes, items);
+  MDG_LAUNCH(k_cell_count, MDG_GRID(nids, kCellThreads), boxes, ids);
+  MDG_LAUNCH(k_cell_scatter, MDG_GRID(nitems, kIonThreads), boxes, items);
+  MDG_LAUNCH(k_cell_place, MDG_GRID(nitems, kIonThreads), boxes, items);
 }
 
 void launch_phase_tables(bool fp32, const BoxDesc* boxes, const Item* items, int n,
@@ -85,8 +90,11 @@ void launch_next_r(const BoxDesc* boxes, const Item* items, int n, hipStream_t s
   if (n > 0) MDG_LAUNCH(k_next_r, MDG_GRID(n, kIonThreads), boxes, items);
 }
 
-void launch_box_scalars(const BoxDesc* boxes, const int* ids, int n, int flags, hipStream_t s) {
-  if (n > 0) MDG_LAUNCH(k_box_scalars, MDG_GRID(n, kScalThreads), boxes, ids, flags);
+void launch_box_scalars(const BoxDesc* boxes, const int* ids, int nids, const Item* items,
+                        int nitems, int flags, hipStream_t s) {
+  if (nids <= 0 || nitems <= 0) return;
+  MDG_LAUNCH(k_scal_part, MDG_GRID(nitems, kIonThreads), boxes, items);
+  MDG_LAUNCH(k_box_scalars, MDG_GRID(nids, kScalThreads), boxes, ids, flags);
 }
 
 void launch_moment_correction(const BoxDesc* boxes, const int* ids, int n, hipStream_t s) {

@@ -34,57 +34,67 @@ __device__ __forceinline__ void block_sum(double (&v)[NV]) {
 }
 
 // ----------------------------------------------------------------------
-// Cell list of one box per workgroup.  Binning is integer and exact:
-// c_a = (int)(r_a * (n_a / L_a)) clamped to n_a - 1 (r_a == L_a is a legal
-// coordinate after mdc's fold, rv.c:59-64).  Counts and starts live in LDS
-// for up to kCellLds cells.  The scatter is stable (ions of a cell keep
-// ascending index): one wave walks the ions in order, 64 at a time, and
-// ranks lanes of equal cell with ballots, so the order never depends on
-// timing.  Also writes cell-ordered copies of positions and species.
+// Cell lists in four launches, so that a large box spreads over the chip:
+//   k_cell_bin     per ion: its cell.  Binning is integer and exact:
+//                  c_a = (int)(r_a * (n_a / L_a)) clamped to n_a - 1
+//                  (r_a == L_a is a legal coordinate after mdc's fold,
+//                  rv.c:59-64).
+//   k_cell_count   per box: counts, cell_start (exclusive scan) and the
+//                  scatter's fill cursors.
+//   k_cell_scatter per ion: a slot of its cell by an atomic cursor; the
+//                  slots within a cell come in whatever order the atomics
+//                  took.
+//   k_cell_place   per slot: the ion goes to cell_start + the number of
+//                  ions of its cell with a lower index, so a cell lists its
+//                  ions in ascending index whatever the scatter's order;
+//                  also writes the cell-ordered copies of positions and
+//                  species.
+// The per-ion launches cover the ion items of every box and skip the
+// boxes without cells.
 // ----------------------------------------------------------------------
-__global__ __launch_bounds__(kCellThreads) void k_cell_build(const BoxDesc* __restrict__ boxes,
+__global__ __launch_bounds__(kIonThreads) void k_cell_bin(const BoxDesc* __restrict__ boxes,
+                                                          const Item* __restrict__ items) {
+  const Item it = items[blockIdx.x];
+  const BoxDesc& B = boxes[it.box];
+  const int i = it.a + threadIdx.x;
+  if (B.all_pairs || i >= B.N) return;
+  const double x = B.rx[i], y = B.ry[i], z = B.rz[i];
+  if (!(x >= 0.0 && x <= B.L[0] && y >= 0.0 && y <= B.L[1] && z >= 0.0 && z <= B.L[2]))
+    atomicOr(B.err, kErrState);
+  int cx = (x >= 0.0) ? (int)(x * B.cellscale[0]) : 0;
+  int cy = (y >= 0.0) ? (int)(y * B.cellscale[1]) : 0;
+  int cz = (z >= 0.0) ? (int)(z * B.cellscale[2]) : 0;
+  cx = min(max(cx, 0), B.nc[0] - 1);
+  cy = min(max(cy, 0), B.nc[1] - 1);
+  cz = min(max(cz, 0), B.nc[2] - 1);
+  B.cell_of[i] = (cz * B.nc[1] + cy) * B.nc[0] + cx;
+}
+
+__global__ __launch_bounds__(kCellThreads) void k_cell_count(const BoxDesc* __restrict__ boxes,
                                                              const int* __restrict__ box_ids) {
   __shared__ int s_cnt[kCellLds];
-  __shared__ int s_start[kCellLds + 1];
   __shared__ int s_scan[kCellThreads];
   const BoxDesc& B = boxes[box_ids[blockIdx.x]];
   const int N = B.N, ncell = B.ncell, tid = threadIdx.x;
-  const int ncx = B.nc[0], ncy = B.nc[1], ncz = B.nc[2];
+  // counters in LDS for up to kCellLds cells, else in cell_count, where
+  // only atomics (which act in L2) touch them until the scan has read them
   const bool lds = ncell <= kCellLds;
   int* cnt = lds ? s_cnt : B.cell_count;
-  int* start = lds ? s_start : B.cell_start;
-  int err = 0;
-  // wave w owns the contiguous ion range [w * per, (w + 1) * per)
-  const int wv = tid / kWave, lane = tid % kWave;
-  const int per = (N + kCellWaves * kWave - 1) / (kCellWaves * kWave) * kWave;
-  int* wcnt = B.cell_wave + (size_t)wv * ncell;
-  for (int c = tid; c < ncell; c += kCellThreads) cnt[c] = 0;
-  for (int c = lane; c < ncell; c += kWave) atomicExch(&wcnt[c], 0);
-  __syncthreads();
-  for (int i = wv * per + lane; i < min(N, (wv + 1) * per); i += kWave) {
-    const double x = B.rx[i], y = B.ry[i], z = B.rz[i];
-    if (!(x >= 0.0 && x <= B.L[0] && y >= 0.0 && y <= B.L[1] && z >= 0.0 && z <= B.L[2]))
-      err |= kErrState;
-    int cx = (x >= 0.0) ? (int)(x * B.cellscale[0]) : 0;
-    int cy = (y >= 0.0) ? (int)(y * B.cellscale[1]) : 0;
-    int cz = (z >= 0.0) ? (int)(z * B.cellscale[2]) : 0;
-    cx = min(max(cx, 0), ncx - 1);
-    cy = min(max(cy, 0), ncy - 1);
-    cz = min(max(cz, 0), ncz - 1);
-    const int c = (cz * ncy + cy) * ncx + cx;
-    B.cell_of[i] = c;
-    atomicAdd(&cnt[c], 1);
-    atomicAdd(&wcnt[c], 1);
+  if (lds) {
+    for (int c = tid; c < ncell; c += kCellThreads) s_cnt[c] = 0;
+  } else {
+    for (int c = tid; c < ncell; c += kCellThreads) atomicExch(&cnt[c], 0);
+    __threadfence();  // every wave's global atomics done before the barrier
   }
   __syncthreads();
-  // exclusive scan of counts -> start
+  for (int i = tid; i < N; i += kCellThreads) atomicAdd(&cnt[B.cell_of[i]], 1);
+  if (!lds) __threadfence();
+  __syncthreads();
+  // exclusive scan: thread t owns cells [t * cper, (t + 1) * cper)
   const int cper = (ncell + kCellThreads - 1) / kCellThreads;
   const int c0 = min(tid * cper, ncell), c1 = min(c0 + cper, ncell);
   int local = 0;
-  for (int c = c0; c < c1; ++c) {
-    if (!lds) cnt[c] = atomicAdd(&cnt[c], 0);  // global counters: read them where the atomics acted
-    local += cnt[c];
-  }
+  for (int c = c0; c < c1; ++c) local += lds ? cnt[c] : atomicAdd(&cnt[c], 0);
   s_scan[tid] = local;
   __syncthreads();
   for (int off = 1; off < kCellThreads; off <<= 1) {
@@ -95,54 +105,40 @@ __global__ __launch_bounds__(kCellThreads) void k_cell_build(const BoxDesc* __re
   }
   int run = s_scan[tid] - local;
   for (int c = c0; c < c1; ++c) {
-    start[c] = run;
-    run += cnt[c];
+    const int n = lds ? cnt[c] : atomicAdd(&cnt[c], 0);  // read before the cursor replaces it
+    B.cell_start[c] = run;
+    B.cell_count[c] = run;
+    run += n;
   }
-  if (tid == kCellThreads - 1) start[ncell] = s_scan[tid];
-  __syncthreads();
-  // per-wave fill cursors: cell start plus the counts of lower waves (the
-  // per-wave counters are only ever touched by atomics, which act in L2)
-  for (int c = tid; c < ncell; c += kCellThreads) {
-    int cur = start[c];
-    for (int w = 0; w < kCellWaves; ++w) cur += atomicExch(B.cell_wave + (size_t)w * ncell + c, cur);
-  }
-  __syncthreads();
-  {
-    const unsigned long long below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-    const int hi = min(N, (wv + 1) * per);
-    for (int base = wv * per; base < hi; base += kWave) {
-      const int i = base + lane;
-      const bool valid = i < hi;
-      const int c = valid ? B.cell_of[i] : -1;
-      unsigned long long todo = __ballot(valid);
-      int rank = 0, first = 0, cur = 0;
-      unsigned long long mine = 0;
-      while (todo) {
-        const int leader = __ffsll((long long)todo) - 1;
-        const int lc = __shfl(c, leader, kWave);
-        const unsigned long long m = __ballot(c == lc);
-        if (c == lc) {
-          mine = m;
-          rank = __popcll(m & below);
-          first = leader;
-        }
-        todo &= ~m;
-      }
-      // the first lane of each cell group claims the group's slots
-      if (valid && lane == first) cur = atomicAdd(&wcnt[c], __popcll(mine));
-      cur = __shfl(cur, first, kWave);
-      if (valid) B.sorted[cur + rank] = i;
-    }
-  }
-  __syncthreads();
-  if (lds)
-    for (int c = tid; c <= ncell; c += kCellThreads) B.cell_start[c] = start[c];
-  for (int p = tid; p < N; p += kCellThreads) {
-    const int i = B.sorted[p];
-    B.sx[p] = B.rx[i]; B.sy[p] = B.ry[i]; B.sz[p] = B.rz[i];
-    B.styp[p] = B.typ[i];
-  }
-  if (err) atomicOr(B.err, err);
+  if (tid == kCellThreads - 1) B.cell_start[ncell] = s_scan[tid];
+}
+
+__global__ __launch_bounds__(kIonThreads) void k_cell_scatter(const BoxDesc* __restrict__ boxes,
+                                                              const Item* __restrict__ items) {
+  const Item it = items[blockIdx.x];
+  const BoxDesc& B = boxes[it.box];
+  const int i = it.a + threadIdx.x;
+  if (B.all_pairs || i >= B.N) return;
+  B.cell_tmp[atomicAdd(&B.cell_count[B.cell_of[i]], 1)] = i;
+}
+
+__global__ __launch_bounds__(kIonThreads) void k_cell_place(const BoxDesc* __restrict__ boxes,
+                                                            const Item* __restrict__ items) {
+  const Item it = items[blockIdx.x];
+  const BoxDesc& B = boxes[it.box];
+  const int p = it.a + threadIdx.x;
+  if (B.all_pairs || p >= B.N) return;
+  const int j = B.cell_tmp[p];
+  const int c = B.cell_of[j];
+  const int s = B.cell_start[c], e = B.cell_start[c + 1];
+  // the slots of a cell are adjacent, so a wave's lanes mostly walk the
+  // same slots together
+  int rank = 0;
+  for (int q = s; q < e; ++q) rank += B.cell_tmp[q] < j ? 1 : 0;
+  const int d = s + rank;
+  B.sorted[d] = j;
+  B.sx[d] = B.rx[j]; B.sy[d] = B.ry[j]; B.sz[d] = B.rz[j];
+  B.styp[d] = B.typ[j];
 }
 
 // ----------------------------------------------------------------------
@@ -323,34 +319,54 @@ __global__ __launch_bounds__(kIonThreads) void k_next_r(const BoxDesc* __restric
 }
 
 // ----------------------------------------------------------------------
-// Per-box scalars (one workgroup per box, fixed summation order):
-// real-space pot and diagonal virial from the item partials, reciprocal
-// energy from the per-k terms, kinetic energy as mdc's calc_kin
-// (rv.c:68-86), pressure as calc_press (pt.c:8-21).
+// Per-box scalars in two launches.  k_scal_part, per ion item q of a box
+// (nspart of them): the kinetic sums m v^2 of its ions and its share of
+// the real-space item partials (pot, diagonal virial, pairs) and of the
+// reciprocal energy blocks, each a fixed-order workgroup sum.  k_box_scalars,
+// one workgroup per box, sums the nspart partials (fixed order) into
+// real-space pot and virial, reciprocal energy, kinetic energy as mdc's
+// calc_kin (rv.c:68-86) and pressure as calc_press (pt.c:8-21).
 // flags bit0: count a completed step.
 // ----------------------------------------------------------------------
-__global__ __launch_bounds__(kScalThreads) void k_box_scalars(const BoxDesc* __restrict__ boxes,
-                                                              const int* __restrict__ box_ids,
-                                                              int flags) {
-  const BoxDesc& B = boxes[box_ids[blockIdx.x]];
-  const int tid = threadIdx.x;
+__global__ __launch_bounds__(kIonThreads) void k_scal_part(const BoxDesc* __restrict__ boxes,
+                                                           const Item* __restrict__ items) {
+  const Item it = items[blockIdx.x];
+  const BoxDesc& B = boxes[it.box];
+  const int tid = threadIdx.x, q = it.a / kIonThreads, nq = B.nspart;
   double v[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-  for (int s = tid; s < B.nreal; s += kScalThreads) {
+  const int s0 = (int)((long)q * B.nreal / nq), s1 = (int)((long)(q + 1) * B.nreal / nq);
+  for (int s = s0 + tid; s < s1; s += kIonThreads) {
     v[0] += B.rpart[4 * s];
     v[1] += B.rpart[4 * s + 1];
     v[2] += B.rpart[4 * s + 2];
     v[3] += B.rpart[4 * s + 3];
     v[8] += (double)B.rpairs[s];
   }
-  if (B.kspace)
-    for (int k = tid; k < (B.nk + kIonThreads - 1) / kIonThreads; k += kScalThreads)
-      v[4] += B.kpot[k];
-  for (int i = tid; i < B.N; i += kScalThreads) {
-    const double m = B.m[B.typ[i]];
-    v[5] += m * B.vx[i] * B.vx[i];
-    v[6] += m * B.vy[i] * B.vy[i];
-    v[7] += m * B.vz[i] * B.vz[i];
+  if (B.kspace) {
+    const int nkb = (B.nk + kIonThreads - 1) / kIonThreads;
+    const int k0 = (int)((long)q * nkb / nq), k1 = (int)((long)(q + 1) * nkb / nq);
+    for (int k = k0 + tid; k < k1; k += kIonThreads) v[4] += B.kpot[k];
   }
+  const int i = it.a + tid;
+  if (i < B.N) {
+    const double m = B.m[B.typ[i]];
+    v[5] = m * B.vx[i] * B.vx[i];
+    v[6] = m * B.vy[i] * B.vy[i];
+    v[7] = m * B.vz[i] * B.vz[i];
+  }
+  block_sum<kIonThreads, 9>(v);
+  if (tid == 0)
+    for (int k = 0; k < 9; ++k) B.spart[9 * (size_t)q + k] = v[k];
+}
+
+__global__ __launch_bounds__(kScalThreads) void k_box_scalars(const BoxDesc* __restrict__ boxes,
+                                                              const int* __restrict__ box_ids,
+                                                              int flags) {
+  const BoxDesc& B = boxes[box_ids[blockIdx.x]];
+  const int tid = threadIdx.x;
+  double v[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  for (int q = tid; q < B.nspart; q += kScalThreads)
+    for (int k = 0; k < 9; ++k) v[k] += B.spart[9 * (size_t)q + k];
   block_sum<kScalThreads, 9>(v);
   if (tid == 0) {
     double* S = B.scal;

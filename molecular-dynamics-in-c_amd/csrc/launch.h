@@ -19,7 +19,10 @@ struct LaunchEvents {
 extern thread_local LaunchEvents g_launch_events;
 
 // k_fp64.hip (compiled with -ffp-contract=off)
-void launch_cell_build(const BoxDesc* boxes, const int* ids, int n, hipStream_t s);
+// ids: the boxes with cell lists; items: the per-ion items of any boxes
+// (those of boxes without cells are skipped)
+void launch_cell_build(const BoxDesc* boxes, const int* ids, int nids, const Item* items,
+                       int nitems, hipStream_t s);
 void launch_phase_tables(bool fp32, const BoxDesc* boxes, const Item* items, int n, hipStream_t s);
 void launch_real64(bool analytic, const BoxDesc* boxes, const Item* items, int n, int mode,
                    hipStream_t s);
@@ -34,7 +37,9 @@ void launch_force_vv(bool vv, const BoxDesc* boxes, const Item* items, int n, hi
 void launch_clear_foc(const BoxDesc* boxes, const Item* items, int n, hipStream_t s);
 void launch_next_v(const BoxDesc* boxes, const Item* items, int n, hipStream_t s);
 void launch_next_r(const BoxDesc* boxes, const Item* items, int n, hipStream_t s);
-void launch_box_scalars(const BoxDesc* boxes, const int* ids, int n, int flags, hipStream_t s);
+// ids: the boxes; items: the per-ion items of exactly those boxes
+void launch_box_scalars(const BoxDesc* boxes, const int* ids, int nids, const Item* items,
+                        int nitems, int flags, hipStream_t s);
 void launch_moment_correction(const BoxDesc* boxes, const int* ids, int n, hipStream_t s);
 // r, v, scalars and observables of every box into one staging buffer
 // (box b at dst + off[b]: [r 3N][v 3N][kScalCount][kObsLen] doubles)
@@ -68,14 +73,16 @@ int kforce_hz_for(int hmax);
 }  // namespace mdg
 
 // MDG_LAUNCH(kernel, grid, block, lds, stream, args...): a kernel launch
-// that carries g_launch_events when they are set.
+// that carries g_launch_events when they are set.  Of several launches
+// under one set of events the first stamps the start and the last the stop,
+// so the events span the group.
 #define MDG_LAUNCH(...) MDG_LAUNCH_(__VA_ARGS__)
 #define MDG_LAUNCH_(kernel, grid, block, lds, strm, ...)                                 \
   do {                                                                                  \
     ::mdg::LaunchEvents& ev_ = ::mdg::g_launch_events;                                  \
     if (ev_.start) {                                                                    \
-      hipExtLaunchKernelGGL(kernel, grid, block, lds, strm, ev_.start, ev_.stop, 0u,    \
-                            __VA_ARGS__);                                               \
+      hipExtLaunchKernelGGL(kernel, grid, block, lds, strm, ev_.used ? nullptr : ev_.start, \
+                            ev_.stop, 0u, __VA_ARGS__);                                 \
       ev_.used = true;                                                                  \
     } else {                                                                            \
       hipLaunchKernelGGL(kernel, grid, block, lds, strm, __VA_ARGS__);                  \

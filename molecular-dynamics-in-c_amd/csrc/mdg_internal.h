@@ -23,9 +23,8 @@ constexpr int kSegLen = 8;          // k vectors per segment (one column, consec
 constexpr int kKfThreads = 64;      // back-projection: one wave, one ion per lane
 constexpr int kKfRows = 64;         // weight rows (4 columns each) staged in LDS at a time
 constexpr int kIonThreads = 256;    // per-ion streaming kernels
-constexpr int kCellThreads = 1024;  // cell-list build, one workgroup per box
+constexpr int kCellThreads = 1024;  // cell counts and starts, one workgroup per box
 constexpr int kCellLds = 4096;      // cells whose counters live in LDS
-constexpr int kCellWaves = kCellThreads / kWave;
 constexpr int kScalThreads = 256;   // per-box scalar reduction
 
 // error bits written by kernels into BoxDesc::err
@@ -116,8 +115,8 @@ struct BoxDesc {
   int cell_hb;        // home blocks per cell item stride (item d: first block)
   double cellscale[3];  // nc / L
   int *cell_of;       // [N]
-  int *cell_count;    // [ncell]
-  int *cell_wave;     // [kCellWaves][ncell] per-wave counts, then per-wave fill cursors
+  int *cell_count;    // [ncell] counts, then the scatter's fill cursors
+  int *cell_tmp;      // [N] ions grouped by cell in the scatter's (unordered) slots
   int *cell_start;    // [ncell + 1]
   int *sorted;        // [N] ion index in cell order (ascending within a cell)
   double *sx, *sy, *sz;
@@ -167,6 +166,8 @@ struct BoxDesc {
   double *rpart;      // [nreal][4]: pot, vir x, y, z per real-space item
   unsigned long long *rpairs;  // [nreal] accepted unordered pairs per item
   int nreal;
+  double *spart;      // [nspart][9] per-block partial sums of the box scalars
+  int nspart;         // ceil(N / kIonThreads)
   double *scal;       // kScal* layout below
   int *err;
   // ---- the per-step observables and thermostat of mdc's step loop
