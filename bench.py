@@ -457,6 +457,8 @@ def leg_kcl32768(steps=10000, fp64_steps=1000, every=500, cpu=True):
     box's k count."""
     p = W.kcl32768_split()
     r, v, sp = W.initial_state(p, 1300.0, seed=33, jitter=0.2)
+    # tilings tuned for the box alone on the chip (context of one copy)
+    tune = {prec: mdgpu.tune([p], prec, context_ions=1) for prec in (mdgpu.MIXED, mdgpu.FP64)}
     f = {}
     for prec in (mdgpu.MIXED, mdgpu.FP64):
         b = mdgpu.Batch([p], prec)
@@ -503,7 +505,8 @@ def leg_kcl32768(steps=10000, fp64_steps=1000, every=500, cpu=True):
                                            "fp64_rel_to_abs_energy": res[mdgpu.FP64]["drift_rel"],
                                            "fp64_steps": fp64_steps,
                                            "fp64_shortened_by": steps / fp64_steps},
-           "fp64_ms_per_step": res[mdgpu.FP64]["ms_per_step"]}
+           "fp64_ms_per_step": res[mdgpu.FP64]["ms_per_step"],
+           "autotune": {"mixed": tune[mdgpu.MIXED], "fp64": tune[mdgpu.FP64]}}
     if cpu:
         sys.path.insert(0, os.path.join(ROOT, "tests"))
         import mdref
@@ -569,6 +572,51 @@ def serving(njobs, steps_scale, budget_gb, tune=True):
                and a.status == b.status == c.status == mdgpu.OK
                for a, b, c in zip(res["batched"], res["one_at_a_time"], res["saturated"]))
     out["saturated"]["note"] = "every job arrives at t=0: latency is queueing plus service"
+    # What bounds the saturated run from below: the jobs' work at the rate
+    # the chip steps all of them together as one static batch, and the
+    # longest job's steps at its own rate alone on the chip.
+    prm = [mdgpu.serve_job_params(j, config("saturated")) for j in jobs]
+    work = float(sum(n_of(j) * j.steps for j in jobs))
+    mix = mdgpu.Batch(prm, mdgpu.MIXED)
+    for k, j in enumerate(jobs):
+        r0, v0, sp0 = W.initial_state(prm[k], j.temp_k, seed=j.seed)
+        mix.set_state(k, r=r0, v=v0, species=sp0)
+    mix.prepare()
+    mix.step(16)
+    mix.sync()
+    t0 = time.perf_counter()
+    mix.step(64)
+    mix.sync()
+    mix_ms = (time.perf_counter() - t0) * 1e3 / 64
+    mix.close()
+    mix_rate = sum(n_of(j) for j in jobs) / (mix_ms * 1e-3)
+    crit = sorted(range(len(jobs)), key=lambda k: -n_of(jobs[k]) * jobs[k].steps)[:3]
+    lone_s = 0.0
+    for k in crit:
+        b = mdgpu.Batch([prm[k]], mdgpu.MIXED)
+        r0, v0, sp0 = W.initial_state(prm[k], jobs[k].temp_k, seed=jobs[k].seed)
+        b.set_state(0, r=r0, v=v0, species=sp0)
+        b.prepare()
+        b.step(16)
+        b.sync()
+        t0 = time.perf_counter()
+        b.step(64)
+        b.sync()
+        lone_s = max(lone_s, (time.perf_counter() - t0) / 64 * jobs[k].steps)
+        b.close()
+    wall_min = max(work / mix_rate, lone_s)
+    out["bounds"] = {
+        "static_mix_ion_steps_per_s": mix_rate,
+        "static_mix_ms_per_step": mix_ms,
+        "longest_job_alone_s": lone_s,
+        "wall_lower_bound_s": wall_min,
+        "saturated_wall_over_bound": out["saturated"]["wall_s"] / wall_min,
+        "saturated_vs_static_mix": out["saturated"]["ion_steps_per_s"] / mix_rate,
+        "note": "static mix: every job of the stream resident at once, stepped as one batch; "
+                "the bound is the larger of the stream's work at that rate and the longest "
+                "job's steps alone"}
+    log(f"  serving bounds: static mix {mix_rate:.4e} ion-steps/s ({mix_ms:.3f} ms/step), "
+        f"longest job alone {lone_s:.2f} s, wall bound {wall_min:.2f} s")
     out.update({"jobs": njobs, "arrival_rate_hz": 50.0, "ions": "216-32768 log-uniform",
                 "ewald_split": "eps 1e-6; rc = 16.5 A (N/32768)^(1/6), at least 10 A, at most L/2 "
                                "(mdg_ewald_rc_rule)",

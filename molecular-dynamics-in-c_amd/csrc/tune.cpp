@@ -10,6 +10,7 @@
 // by 2% to replace it, so tuning never trades a measured loss for noise.
 #include <math.h>
 #include <stdio.h>
+#include <stdlib.h>
 #include <string.h>
 
 #include <algorithm>
@@ -42,7 +43,8 @@ struct Run {
   double real_ms = 0, sk_ms = 0, kf_ms = 0;
 };
 
-// Times one candidate: R copies of the box, 1 warm-up and 3 timed steps.
+// Times one candidate: R copies of the box, untimed warm-up steps, then 3
+// timed steps.
 int time_candidate(const BoxSpec& base, const mdg_box_params& prm, int precision,
                    const Tiling& t, int R, hipStream_t s, Run* out) {
   BoxSpec sp = base;
@@ -78,8 +80,19 @@ int time_candidate(const BoxSpec& base, const mdg_box_params& prm, int precision
     std::vector<std::string> names;
     std::vector<double> ms;
     std::vector<int> launches;
+    // untimed steps for at least 20 ms first: a device left idle while the
+    // host built the boxes runs its first kernels at a lower clock (times
+    // measured straight away came out two to three times too long, and
+    // ranked the candidates wrongly)
+    if (rc == MDG_OK) {
+      const auto w0 = std::chrono::steady_clock::now();
+      do {
+        rc = set.step(4, s, false);
+        if (rc == MDG_OK && hipStreamSynchronize(s) != hipSuccess) rc = MDG_E_HIP;
+      } while (rc == MDG_OK &&
+               std::chrono::steady_clock::now() - w0 < std::chrono::milliseconds(20));
+    }
     // each family once over all copies, back to back (families are timed apart)
-    if (rc == MDG_OK) rc = set.profile(1, s, false, &names, &ms, &launches);
     if (rc == MDG_OK) rc = set.profile(3, s, false, &names, &ms, &launches);
     if (rc == MDG_OK) {
       *out = Run();
@@ -90,6 +103,9 @@ int time_candidate(const BoxSpec& base, const mdg_box_params& prm, int precision
           out->sk_ms += ms[q];
         else if (n == "kforce" || n == "kf_pack") out->kf_ms += ms[q];
       }
+      if (getenv("MDGPU_TUNE_TRACE") && getenv("MDGPU_TUNE_TRACE")[0] == '2')
+        for (size_t q = 0; q < names.size(); ++q)
+          fprintf(stderr, "  %s %.4f ms (%d)\n", names[q].c_str(), ms[q], launches[q]);
       // k_force_vv reads every split's and group's partial forces: charge
       // those bytes (at 5 TB/s) to the families that create them
       const double per = 24.0 * N * R / 5e12 * 1e3;
@@ -153,6 +169,9 @@ int tune_class(const BoxSpec& spec, const mdg_box_params& prm, int precision, lo
   R = std::min(R, 512ll);
   R = std::min(R, std::max(1ll, (long long)((8ull << 30) / bytes)));
   const size_t nrun = std::max(rc_list.size(), std::max(ch_list.size(), gr_list.size()));
+  // MDGPU_TUNE_TRACE=1: every candidate's times on stderr
+  const char* tr = getenv("MDGPU_TUNE_TRACE");
+  const bool trace = tr && (tr[0] == '1' || tr[0] == '2');
   std::vector<double> best_real(rc_list.size(), 1e30), best_ch(ch_list.size(), 1e30),
       best_gr(gr_list.size(), 1e30);
   for (size_t q = 0; q < nrun; ++q) {
@@ -167,6 +186,12 @@ int tune_class(const BoxSpec& spec, const mdg_box_params& prm, int precision, lo
     rc = time_candidate(spec, prm, precision, t, (int)R, s, &run);
     if (rc != MDG_OK) return rc;
     ++*runs;
+    if (trace)
+      fprintf(stderr,
+              "mdg_tune N=%d nk=%d: cells %d w %.2f rsplit %d | nchunk %d | ngroup %d -> "
+              "real %.4f sk %.4f kf %.4f ms\n",
+              N, P.nk, t.cells ? 1 : 0, t.cell_w, t.rsplit, t.nchunk, t.ngroup, run.real_ms,
+              run.sk_ms, run.kf_ms);
     best_real[ir] = std::min(best_real[ir], run.real_ms);
     best_ch[ic] = std::min(best_ch[ic], run.sk_ms);
     best_gr[ig] = std::min(best_gr[ig], run.kf_ms);
