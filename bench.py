@@ -854,10 +854,14 @@ def traffic_summary(fetch_dir, write_dir, out_json):
     kib = COPY_PROBE_BYTES / 1024
     fs = kib / fetch[probe[0]][0]
     ws = kib / write[[k for k in write if "k_copy_probe" in k][0]][0]
+    # kernel -> the family profile() times it under (the cell build and the
+    # box scalars are groups of launches: their bytes add up per family launch)
     names = {"k_real_win": "real_space", "k_mixed_tables": "phase_tables",
              "k_sk_mfma": "sk_partial", "k_sk_finalize": "sk_finalize", "k_kf_pack": "kf_pack",
              "k_kforce_mfma": "kforce", "k_force_vv": "force_vv", "k_box_scalars": "box_scalars",
-             "k_cell_build": "cell_build", "k_observe": "observe", "k_snapshot": "snapshot",
+             "k_scal_part": "box_scalars", "k_cell_bin": "cell_build",
+             "k_cell_count": "cell_build", "k_cell_scatter": "cell_build",
+             "k_cell_place": "cell_build", "k_observe": "observe", "k_snapshot": "snapshot",
              "k_copy_probe": "copy_probe"}
     out = {"source": "profiles/batch256_mixed_hbm_traffic.md", "sources_sha256": sources_sha256(),
            "fetch_scale": fs, "write_scale": ws, "kernels": {}}
@@ -865,11 +869,17 @@ def traffic_summary(fetch_dir, write_dir, out_json):
         base = short(k)
         key = next((v for n, v in names.items() if base.startswith(n)), base)
         wk = write.get(k, (0.0, 0))[0]
-        out["kernels"][key] = {"kernel": base, "fetch_kib_per_dispatch": fk,
-                               "write_kib_per_dispatch": wk, "dispatches": nd,
-                               "read_bytes_per_launch": int(fk * fs * 1024),
-                               "write_bytes_per_launch": int(wk * ws * 1024),
-                               "traffic_bytes_per_launch": int((fk * fs + wk * ws) * 1024)}
+        e = out["kernels"].setdefault(key, {"kernels": [], "fetch_kib_per_dispatch": 0.0,
+                                            "write_kib_per_dispatch": 0.0, "dispatches": nd,
+                                            "read_bytes_per_launch": 0,
+                                            "write_bytes_per_launch": 0,
+                                            "traffic_bytes_per_launch": 0})
+        e["kernels"].append(base)
+        e["fetch_kib_per_dispatch"] += fk
+        e["write_kib_per_dispatch"] += wk
+        e["read_bytes_per_launch"] += int(fk * fs * 1024)
+        e["write_bytes_per_launch"] += int(wk * ws * 1024)
+        e["traffic_bytes_per_launch"] += int((fk * fs + wk * ws) * 1024)
     with open(out_json, "w") as f:
         json.dump(out, f, indent=1)
     print(json.dumps(out, indent=1))
@@ -1075,7 +1085,7 @@ def main():
         "dtype": DTYPE_MIXED, "data": "synthetic",
         "config": {"workload": "256 alkali-halide boxes (NaCl/KCl by seed), 64-4096 ions "
                                "log-uniform in whole cells, 300-2000 K, eps 1e-6 Ewald "
-                               "(rc=min(10 A, L/2)), analytic erfc+BMH, all boxes per launch",
+                               f"(rc=min({W.BATCH_RC_MAX:g} A, L/2)), analytic erfc+BMH, all boxes per launch",
                    "boxes": len(boxes), "ions": total_ions, "precision": "mixed",
                    "parallelism": "single GPU, boxes batched per launch",
                    "graph": "8-step hipGraph captured and uploaded before the warm-up"},

@@ -274,7 +274,7 @@ constexpr int kf_tile_units() { return KB * 2 * kWave + kKfTileCols; }
 // is 3% faster); wider boxes, whose W operands take 16 VGPRs per K block,
 // keep their registers.
 template <int KB>
-constexpr int kf_min_waves() { return KB <= 4 ? 4 : 1; }
+constexpr int kf_min_waves() { return KB <= 4 ? 16 / kKfMfmaWaves : 1; }
 
 template <int KB>
 __global__ __launch_bounds__(kKfMfmaWaves* kWave, kf_min_waves<KB>()) void k_kforce_mfma(
