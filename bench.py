@@ -785,12 +785,14 @@ def traffic_of(kernel):
     return k["traffic_bytes_per_launch"], t.get("source")
 
 
-def mixed_roofline(prof, work):
+def mixed_roofline(prof, work, prof_alone=None):
     """The batched mixed step's kernels against the FP32 peak: algorithmic
     FLOP per launch over the average launch duration of profile() (every
     launch timed by HIP events its own dispatch stamps, in the step's own
-    schedule).  The k-space passes compute FP32-grade products on FP16
-    MFMA (three per product) and are priced against the FP32 peak."""
+    schedule, where the real space and the two reciprocal chains share the
+    chip), and per kernel also over its duration launched alone.  The
+    k-space passes compute FP32-grade products on FP16 MFMA (three per
+    product) and are priced against the FP32 peak."""
     kn = sum(w["nk"] * w["n_ions"] for w in work)
     pairs = sum(w["pairs"] for w in work)
     flops = {"sk_partial": FLOP_SK * kn, "kforce": FLOP_KF * kn,
@@ -804,6 +806,11 @@ def mixed_roofline(prof, work):
         ach = f / launches / (avg * 1e-3) / 1e12
         per[k] = {"achieved": ach, "frac": ach / PEAK_FP32_TFLOPS, "avg_ms": avg,
                   "launches_per_step": launches, "flop_per_launch": f / launches}
+        if prof_alone and k in prof_alone:
+            ms_a, la = prof_alone[k]
+            ach_a = f / (ms_a * 1e-3) / 1e12
+            per[k].update({"alone_ms_per_step": ms_a, "alone_achieved": ach_a,
+                           "alone_frac": ach_a / PEAK_FP32_TFLOPS})
     dom = max(prof, key=lambda k: prof[k][0])
     pipes = {"real_space": "VALU (FP32 pair function, FP64 distances)",
              "kforce": "MFMA f16x3 (hi/lo) + VALU FP32 epilogue",
@@ -818,6 +825,8 @@ def mixed_roofline(prof, work):
             "flop_per_launch": d.get("flop_per_launch"),
             "per_kernel": per,
             "kernels_ms": {k: round(v[0], 5) for k, v in prof.items()},
+            "kernels_alone_ms": ({k: round(v[0], 5) for k, v in prof_alone.items()}
+                                 if prof_alone else None),
             "kernels_launches": {k: v[1] for k, v in prof.items()}}
 
 
@@ -990,6 +999,7 @@ def main():
     drift = (e1 - e0) / n_ions / args.steps * 1e4
     log(f"  {secs * 1e3 / args.steps:.3f} ms/step, {value:.4e} ion-steps/s")
     prof = b.profile(10)
+    prof_alone = b.profile(10, alone=True)
     work = [b.work(k) for k in range(len(boxes))]
     states_now = []
     for k in range(len(boxes)):
@@ -1031,7 +1041,7 @@ def main():
         log(f"  recorded every 10 steps: {rec_ms:.3f} ms/step "
             f"({48 * total_ions / 1e6:.1f} MB per frame)")
     b.close()
-    roof = mixed_roofline(prof, work)
+    roof = mixed_roofline(prof, work, prof_alone)
 
     # ---- accuracy of the mixed path ----
     ferr_max = ferr_rms = drift_fit = None

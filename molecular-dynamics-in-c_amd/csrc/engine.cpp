@@ -378,7 +378,8 @@ static Layout layout(const BoxPlan& P, int precision) {
   size_t off = 0;
   for (int k = 0; k < 12; ++k) L.ion[k] = take(off, sizeof(double) * N);
   L.typ = take(off, sizeof(int) * N);
-  if (P.tile.rsplit > 1) L.rpf = take(off, sizeof(double) * 3 * (size_t)P.tile.rsplit * N);
+  // FP64 split partials (the mixed real space sums into the fixed-point array)
+  if (P.tile.rsplit > 1 && !mixed) L.rpf = take(off, sizeof(double) * 3 * (size_t)P.tile.rsplit * N);
   if (mixed) L.fix = take(off, sizeof(unsigned long long) * 3 * (size_t)N);
   if (!P.all_pairs) {
     L.cell_of = take(off, sizeof(int) * N);
@@ -548,7 +549,7 @@ static int bind_box(Box* b, char* base, std::string* why) {
   D.rho = spec.pot.rho;
   D.table_nb = spec.table_nb;
   D.rsplit = P.tile.rsplit;
-  D.rpf = P.tile.rsplit > 1 ? (double*)(base + L.rpf) : nullptr;
+  D.rpf = L.rpf ? (double*)(base + L.rpf) : nullptr;
   D.fix = mixed ? (unsigned long long*)(base + L.fix) : nullptr;
   if (!P.all_pairs) {
     for (int d = 0; d < 3; ++d) {

@@ -471,11 +471,13 @@ __global__ __launch_bounds__(kRealThreads) void k_real(const BoxDesc* __restrict
 // A home unit meets only the partners that follow it: all pairs, the ions
 // from its own first index on (j > i within it); cell lists, its own cell
 // (j > i) and the 13 cells of the half shell.  The home ion's share of a
-// pair force is summed in its lane in FP64; the partner's share goes into
-// 64-bit fixed-point accumulators (x 2^32) in LDS per window slot, then
-// into the box's fixed-point array with integer atomics.  Integer sums are
-// exact and order-free, so the result has the same bits whatever order the
-// atomics land in; k_force_vv adds the array to the home sums.
+// pair force is summed in its lane in FP64 over the item's partners, then
+// added to the box's 64-bit fixed-point array (x 2^32) with integer
+// atomics; the partner's share goes into fixed-point accumulators in LDS
+// per window slot, then into the same array.  Integer sums are exact and
+// order-free, so the result has the same bits whatever order the atomics
+// land in, and no per-split partial forces are written; k_force_vv
+// converts the array.  (mode is unused: the force assembly always follows.)
 //
 // The pairs, pair function and per-ion FP64 home sums are k_real's mixed
 // kinds, organised for issue rate:
@@ -540,6 +542,15 @@ __device__ __forceinline__ unsigned long long to_fix(float x) {
   const float hi = floorf(x);
   const float lo = (x - hi) * 4294967296.0f;  // in [0, 2^32)
   return ((unsigned long long)(long long)(int)hi << 32) + (unsigned long long)(unsigned)lo;
+}
+
+// x * 2^32 of an FP64 sum as a 64-bit two's complement integer (truncated
+// below 2^-32; |x| clamped to 1e9): both halves are exact in FP64.
+__device__ __forceinline__ unsigned long long to_fix64(double x) {
+  x = fmin(fmax(x, -1.0e9), 1.0e9);
+  const double hi = floor(x);
+  const double lo = (x - hi) * 4294967296.0;  // in [0, 2^32)
+  return ((unsigned long long)(long long)hi << 32) + (unsigned long long)(unsigned)lo;
 }
 
 template <int KIND>
@@ -748,7 +759,13 @@ __global__ __launch_bounds__(kRealThreads) void k_real_win(const BoxDesc* __rest
       }
       __syncthreads();
     }
-    real_store(B, h, split, mode, a);
+    // the home ions' sums over this item's partners join the partner shares
+    // in the fixed-point array (exact, so the items may land in any order)
+    if (h.valid) {
+      atomicAdd(&B.fix[h.i], to_fix64(a.fx));
+      atomicAdd(&B.fix[(size_t)N + h.i], to_fix64(a.fy));
+      atomicAdd(&B.fix[2 * (size_t)N + h.i], to_fix64(a.fz));
+    }
     tot.pot += a.pot; tot.vx += a.vx; tot.vy += a.vy; tot.vz += a.vz;
     tot.np += a.np;
   }

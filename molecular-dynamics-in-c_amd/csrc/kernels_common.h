@@ -221,7 +221,9 @@ __global__ __launch_bounds__(kIonThreads) void k_force_vv(const BoxDesc* __restr
   const int i = it.a + threadIdx.x;
   if (i >= N) return;
   double fx, fy, fz;
-  if (!(terms & 1)) {
+  if (!(terms & 1) || B.fix) {
+    // no real space, or the mixed real space (all of it in the fixed-point
+    // array, added below)
     fx = 0.0; fy = 0.0; fz = 0.0;
     B.fx0[i] = B.fx[i]; B.fy0[i] = B.fy[i]; B.fz0[i] = B.fz[i];
   } else if (B.rsplit > 1) {
@@ -237,8 +239,8 @@ __global__ __launch_bounds__(kIonThreads) void k_force_vv(const BoxDesc* __restr
     fx = B.fx[i]; fy = B.fy[i]; fz = B.fz[i];
   }
   if (B.fix && (terms & 1)) {
-    // the partner shares of the mixed real space, exact in fixed point;
-    // cleared for the next evaluation
+    // the mixed real space (home sums and partner shares), exact in fixed
+    // point; cleared for the next evaluation
     const double s = 1.0 / kFixScale;
     fx += (double)(long long)B.fix[i] * s;
     fy += (double)(long long)B.fix[N + i] * s;

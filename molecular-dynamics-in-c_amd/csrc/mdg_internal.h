@@ -105,10 +105,11 @@ struct BoxDesc {
   int table_nb;       // bins per pair type (table_division + 1)
   // ---- real-space split: partners divided over rsplit items per home unit ----
   int rsplit;         // 1: forces written in place (mdc's order); >1: partials
-  double *rpf;        // [rsplit][3][N] partial real-space forces (rsplit > 1)
-  // mixed path: each pair is evaluated once; the partner's share of its
-  // force is summed here as 64-bit fixed point (x 2^32, two's complement),
-  // [3][N], exactly and in any order; k_force_vv adds it and clears it
+  double *rpf;        // [rsplit][3][N] partial real-space forces (FP64, rsplit > 1)
+  // mixed path: the real-space force as 64-bit fixed point (x 2^32, two's
+  // complement), [3][N]: each pair is evaluated once, the partner's share
+  // and each item's home sums land here exactly and in any order;
+  // k_force_vv reads it and clears it
   unsigned long long *fix;
   // ---- cell list ----
   int nc[3], ncell;

@@ -106,10 +106,10 @@ int time_candidate(const BoxSpec& base, const mdg_box_params& prm, int precision
       if (getenv("MDGPU_TUNE_TRACE") && getenv("MDGPU_TUNE_TRACE")[0] == '2')
         for (size_t q = 0; q < names.size(); ++q)
           fprintf(stderr, "  %s %.4f ms (%d)\n", names[q].c_str(), ms[q], launches[q]);
-      // k_force_vv reads every split's and group's partial forces: charge
-      // those bytes (at 5 TB/s) to the families that create them
+      // k_force_vv reads every group's (and in FP64 every split's) partial
+      // forces: charge those bytes (at 5 TB/s) to the families that create them
       const double per = 24.0 * N * R / 5e12 * 1e3;
-      out->real_ms += (t.rsplit > 1 ? t.rsplit : 0) * per;
+      if (precision == MDG_FP64) out->real_ms += (t.rsplit > 1 ? t.rsplit : 0) * per;
       out->kf_ms += t.ngroup * per;
     }
     set.set_boxes({});
