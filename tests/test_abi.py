@@ -128,6 +128,19 @@ def test_ewald_params_rule():
     assert 380 < hm < 400   # cfg2: alpha ~ 0.372, hm ~ 393
 
 
+def test_ewald_rc_rule():
+    """mdg_ewald_rc_rule: 16.5 A (N/32768)^(1/6), at least 10 A, at most
+    rc_cap and L_min/2; argument errors for empty boxes."""
+    assert mdgpu.ewald_rc_rule(32768, 100.64, 30.0) == pytest.approx(16.5)
+    assert mdgpu.ewald_rc_rule(4096, 45.12, 30.0) == pytest.approx(16.5 / 2 ** 0.5)
+    assert mdgpu.ewald_rc_rule(8 * 32768, 500.0, 30.0) == pytest.approx(16.5 * 2 ** 0.5)
+    assert mdgpu.ewald_rc_rule(512, 22.56, 30.0) == pytest.approx(10.0)      # floor
+    assert mdgpu.ewald_rc_rule(216, 16.92, 30.0) == pytest.approx(8.46)      # L/2 wins
+    assert mdgpu.ewald_rc_rule(32768, 100.64, 12.0) == pytest.approx(12.0)  # cap
+    with pytest.raises(mdgpu.MdgError):
+        mdgpu.ewald_rc_rule(0, 10.0, 12.0)
+
+
 def test_no_device_fails_loudly():
     if mdgpu.device_count() > 0:
         pytest.skip("a device is present")
