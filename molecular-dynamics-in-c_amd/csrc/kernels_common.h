@@ -34,48 +34,59 @@ __device__ __forceinline__ void block_sum(double (&v)[NV]) {
 }
 
 // ----------------------------------------------------------------------
-// Cell lists in four launches, so that a large box spreads over the chip:
-//   k_cell_bin     per ion: its cell.  Binning is integer and exact:
-//                  c_a = (int)(r_a * (n_a / L_a)) clamped to n_a - 1
-//                  (r_a == L_a is a legal coordinate after mdc's fold,
-//                  rv.c:59-64).
+// Cell lists.  Binning is integer and exact: c_a = (int)(r_a * (n_a / L_a))
+// clamped to n_a - 1 (r_a == L_a is a legal coordinate after mdc's fold,
+// rv.c:59-64).  Within a cell the ions are listed in ascending index, so
+// the order never depends on timing.
+//
+// A box of up to kCellFusedN ions (cell_fused) is built entirely by one
+// workgroup of k_cell_count: bin, count, scan, an unordered scatter into
+// LDS slots by LDS cursors, and the placement below.  A larger box is built
+// in four launches so that it spreads over the chip:
+//   k_cell_bin     per ion: its cell;
 //   k_cell_count   per box: counts, cell_start (exclusive scan) and the
-//                  scatter's fill cursors.
+//                  scatter's fill cursors;
 //   k_cell_scatter per ion: a slot of its cell by an atomic cursor; the
 //                  slots within a cell come in whatever order the atomics
-//                  took.
+//                  took;
 //   k_cell_place   per slot: the ion goes to cell_start + the number of
-//                  ions of its cell with a lower index, so a cell lists its
-//                  ions in ascending index whatever the scatter's order;
-//                  also writes the cell-ordered copies of positions and
-//                  species.
-// The per-ion launches cover the ion items of every box and skip the
-// boxes without cells.
+//                  ions of its cell with a lower index, and the
+//                  cell-ordered copies of positions and species are written.
+// The per-ion launches cover the ion items of the boxes that need them and
+// skip any other box.
 // ----------------------------------------------------------------------
-__global__ __launch_bounds__(kIonThreads) void k_cell_bin(const BoxDesc* __restrict__ boxes,
-                                                          const Item* __restrict__ items) {
-  const Item it = items[blockIdx.x];
-  const BoxDesc& B = boxes[it.box];
-  const int i = it.a + threadIdx.x;
-  if (B.all_pairs || i >= B.N) return;
-  const double x = B.rx[i], y = B.ry[i], z = B.rz[i];
+__device__ __forceinline__ int cell_of_pos(const BoxDesc& B, double x, double y, double z,
+                                           int& err) {
   if (!(x >= 0.0 && x <= B.L[0] && y >= 0.0 && y <= B.L[1] && z >= 0.0 && z <= B.L[2]))
-    atomicOr(B.err, kErrState);
+    err |= kErrState;
   int cx = (x >= 0.0) ? (int)(x * B.cellscale[0]) : 0;
   int cy = (y >= 0.0) ? (int)(y * B.cellscale[1]) : 0;
   int cz = (z >= 0.0) ? (int)(z * B.cellscale[2]) : 0;
   cx = min(max(cx, 0), B.nc[0] - 1);
   cy = min(max(cy, 0), B.nc[1] - 1);
   cz = min(max(cz, 0), B.nc[2] - 1);
-  B.cell_of[i] = (cz * B.nc[1] + cy) * B.nc[0] + cx;
+  return (cz * B.nc[1] + cy) * B.nc[0] + cx;
+}
+
+__global__ __launch_bounds__(kIonThreads) void k_cell_bin(const BoxDesc* __restrict__ boxes,
+                                                          const Item* __restrict__ items) {
+  const Item it = items[blockIdx.x];
+  const BoxDesc& B = boxes[it.box];
+  const int i = it.a + threadIdx.x;
+  if (B.all_pairs || cell_fused(B.N, B.ncell) || i >= B.N) return;
+  int err = 0;
+  B.cell_of[i] = cell_of_pos(B, B.rx[i], B.ry[i], B.rz[i], err);
+  if (err) atomicOr(B.err, err);
 }
 
 __global__ __launch_bounds__(kCellThreads) void k_cell_count(const BoxDesc* __restrict__ boxes,
                                                              const int* __restrict__ box_ids) {
   __shared__ int s_cnt[kCellLds];
   __shared__ int s_scan[kCellThreads];
+  __shared__ int s_tmp[kCellFusedN];
   const BoxDesc& B = boxes[box_ids[blockIdx.x]];
   const int N = B.N, ncell = B.ncell, tid = threadIdx.x;
+  const bool fused = cell_fused(N, ncell);
   // counters in LDS for up to kCellLds cells, else in cell_count, where
   // only atomics (which act in L2) touch them until the scan has read them
   const bool lds = ncell <= kCellLds;
@@ -87,8 +98,26 @@ __global__ __launch_bounds__(kCellThreads) void k_cell_count(const BoxDesc* __re
     __threadfence();  // every wave's global atomics done before the barrier
   }
   __syncthreads();
-  for (int i = tid; i < N; i += kCellThreads) atomicAdd(&cnt[B.cell_of[i]], 1);
-  if (!lds) __threadfence();
+  // the one-workgroup box keeps its ions' cells in registers (ion
+  // tid + k kCellThreads in cl[k])
+  constexpr int kPer = kCellFusedN / kCellThreads;
+  int cl[kPer];
+  if (fused) {
+    int err = 0;
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+      const int i = tid + k * kCellThreads;
+      cl[k] = 0;
+      if (i < N) {
+        cl[k] = cell_of_pos(B, B.rx[i], B.ry[i], B.rz[i], err);
+        atomicAdd(&s_cnt[cl[k]], 1);
+      }
+    }
+    if (err) atomicOr(B.err, err);
+  } else {
+    for (int i = tid; i < N; i += kCellThreads) atomicAdd(&cnt[B.cell_of[i]], 1);
+    if (!lds) __threadfence();
+  }
   __syncthreads();
   // exclusive scan: thread t owns cells [t * cper, (t + 1) * cper)
   const int cper = (ncell + kCellThreads - 1) / kCellThreads;
@@ -107,10 +136,34 @@ __global__ __launch_bounds__(kCellThreads) void k_cell_count(const BoxDesc* __re
   for (int c = c0; c < c1; ++c) {
     const int n = lds ? cnt[c] : atomicAdd(&cnt[c], 0);  // read before the cursor replaces it
     B.cell_start[c] = run;
-    B.cell_count[c] = run;
+    if (fused) s_cnt[c] = run;          // LDS cursor
+    else B.cell_count[c] = run;         // the scatter's cursor
     run += n;
   }
   if (tid == kCellThreads - 1) B.cell_start[ncell] = s_scan[tid];
+  if (!fused) return;
+  __syncthreads();
+  // The one-workgroup box: an unordered scatter into LDS slots holding
+  // (cell << 13 | ion) (ions < 2^13, cells < 2^12), after which each LDS
+  // cursor is its cell's end and the previous cell's cursor its start; the
+  // slots of one cell share the high bits, so comparing slots compares ions.
+  static_assert(kCellFusedN <= (1 << 13) && kCellLds <= (1 << 12), "slot packing");
+#pragma unroll
+  for (int k = 0; k < kPer; ++k) {
+    const int i = tid + k * kCellThreads;
+    if (i < N) s_tmp[atomicAdd(&s_cnt[cl[k]], 1)] = (cl[k] << 13) | i;
+  }
+  __syncthreads();
+  for (int p = tid; p < N; p += kCellThreads) {
+    const int v = s_tmp[p], c = v >> 13, j = v & ((1 << 13) - 1);
+    const int st = c == 0 ? 0 : s_cnt[c - 1], e = s_cnt[c];
+    int rank = 0;
+    for (int q = st; q < e; ++q) rank += s_tmp[q] < v ? 1 : 0;
+    const int d = st + rank;
+    B.sorted[d] = j;
+    B.sx[d] = B.rx[j]; B.sy[d] = B.ry[j]; B.sz[d] = B.rz[j];
+    B.styp[d] = B.typ[j];
+  }
 }
 
 __global__ __launch_bounds__(kIonThreads) void k_cell_scatter(const BoxDesc* __restrict__ boxes,
@@ -118,7 +171,7 @@ __global__ __launch_bounds__(kIonThreads) void k_cell_scatter(const BoxDesc* __r
   const Item it = items[blockIdx.x];
   const BoxDesc& B = boxes[it.box];
   const int i = it.a + threadIdx.x;
-  if (B.all_pairs || i >= B.N) return;
+  if (B.all_pairs || cell_fused(B.N, B.ncell) || i >= B.N) return;
   B.cell_tmp[atomicAdd(&B.cell_count[B.cell_of[i]], 1)] = i;
 }
 
@@ -127,15 +180,15 @@ __global__ __launch_bounds__(kIonThreads) void k_cell_place(const BoxDesc* __res
   const Item it = items[blockIdx.x];
   const BoxDesc& B = boxes[it.box];
   const int p = it.a + threadIdx.x;
-  if (B.all_pairs || p >= B.N) return;
-  const int j = B.cell_tmp[p];
-  const int c = B.cell_of[j];
-  const int s = B.cell_start[c], e = B.cell_start[c + 1];
+  if (B.all_pairs || cell_fused(B.N, B.ncell) || p >= B.N) return;
   // the slots of a cell are adjacent, so a wave's lanes mostly walk the
   // same slots together
+  const int j = B.cell_tmp[p];
+  const int c = B.cell_of[j];
+  const int st = B.cell_start[c], e = B.cell_start[c + 1];
   int rank = 0;
-  for (int q = s; q < e; ++q) rank += B.cell_tmp[q] < j ? 1 : 0;
-  const int d = s + rank;
+  for (int q = st; q < e; ++q) rank += B.cell_tmp[q] < j ? 1 : 0;
+  const int d = st + rank;
   B.sorted[d] = j;
   B.sx[d] = B.rx[j]; B.sy[d] = B.ry[j]; B.sz[d] = B.rz[j];
   B.styp[d] = B.typ[j];

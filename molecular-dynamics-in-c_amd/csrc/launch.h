@@ -19,8 +19,8 @@ struct LaunchEvents {
 extern thread_local LaunchEvents g_launch_events;
 
 // k_fp64.hip (compiled with -ffp-contract=off)
-// ids: the boxes with cell lists; items: the per-ion items of any boxes
-// (those of boxes without cells are skipped)
+// ids: the boxes with cell lists; items: the per-ion items of those built
+// in four launches (not cell_fused; none: one launch does every box)
 void launch_cell_build(const BoxDesc* boxes, const int* ids, int nids, const Item* items,
                        int nitems, hipStream_t s);
 void launch_phase_tables(bool fp32, const BoxDesc* boxes, const Item* items, int n, hipStream_t s);

@@ -25,6 +25,14 @@ constexpr int kKfRows = 64;         // weight rows (4 columns each) staged in LD
 constexpr int kIonThreads = 256;    // per-ion streaming kernels
 constexpr int kCellThreads = 1024;  // cell counts and starts, one workgroup per box
 constexpr int kCellLds = 4096;      // cells whose counters live in LDS
+constexpr int kCellFusedN = 8192;   // boxes up to this size build their cell list in one workgroup
+
+// A box whose whole cell list is built by k_cell_count's one workgroup (bin,
+// count, scan, scatter and placement in LDS): few enough ions for an LDS
+// slot array and few enough cells for LDS counters.
+__host__ __device__ constexpr bool cell_fused(int n, int ncell) {
+  return n <= kCellFusedN && ncell <= kCellLds;
+}
 constexpr int kScalThreads = 256;   // per-box scalar reduction
 
 // error bits written by kernels into BoxDesc::err
