@@ -534,23 +534,21 @@ __device__ __forceinline__ unsigned screen32(uint2 hq, const uint2* pq, int u0, 
   return m;
 }
 
-// x * 2^32 as a 64-bit two's complement integer (truncated below 2^-32):
-// floor(x) in the upper word, the fraction (exact in FP32) scaled into the
-// lower one.  |x| is clamped to 1e9 (a force far beyond any pair's).
-__device__ __forceinline__ unsigned long long to_fix(float x) {
-  x = fminf(fmaxf(x, -1.0e9f), 1.0e9f);
-  const float hi = floorf(x);
-  const float lo = (x - hi) * 4294967296.0f;  // in [0, 2^32)
+// x * 2^32 as a 64-bit two's complement integer, truncated below 2^-32,
+// for |x| <= 1e9: floor(x) in the upper word, the fraction scaled into the
+// lower one.  Done in FP64: for a negative x the fraction is 1 - frac|x|,
+// which FP32 rounds to its spacing near 1 (6e-8, a large error against the
+// 2^-32 resolution); in FP64 it is exact for any widened FP32 value down to
+// 2^-21 in magnitude and within one unit of 2^-32 below that.
+__device__ __forceinline__ unsigned long long to_fix(double x) {
+  const double hi = floor(x);
+  const double lo = (x - hi) * 4294967296.0;  // in [0, 2^32]; 2^32 saturates to 2^32 - 1
   return ((unsigned long long)(long long)(int)hi << 32) + (unsigned long long)(unsigned)lo;
 }
 
-// x * 2^32 of an FP64 sum as a 64-bit two's complement integer (truncated
-// below 2^-32; |x| clamped to 1e9): both halves are exact in FP64.
+// to_fix of an FP64 sum, |x| clamped to 1e9 first
 __device__ __forceinline__ unsigned long long to_fix64(double x) {
-  x = fmin(fmax(x, -1.0e9), 1.0e9);
-  const double hi = floor(x);
-  const double lo = (x - hi) * 4294967296.0;  // in [0, 2^32)
-  return ((unsigned long long)(long long)hi << 32) + (unsigned long long)(unsigned)lo;
+  return to_fix(fmin(fmax(x, -1.0e9), 1.0e9));
 }
 
 template <int KIND>
@@ -691,6 +689,7 @@ __global__ __launch_bounds__(kRealThreads) void k_real_win(const BoxDesc* __rest
       // The body is branch-free but for the partner's atomics; a lane out of
       // candidates evaluates a discarded pair.  Energy and virial count each
       // evaluated (unordered) pair once.
+      unsigned npw = 0u;  // accepted pairs of this window (32 bits suffice)
       int w = nz ? __builtin_ctz(nz) : 0;
       unsigned cur = nz ? pm[w][lane] : 0u;
       if (__ballot(cur != 0u)) do {
@@ -734,19 +733,22 @@ __global__ __launch_bounds__(kRealThreads) void k_real_win(const BoxDesc* __rest
         }
         // the pair force f d once, in FP32; +f d on the home ion, -f d on
         // the partner (exact fixed point)
+        // (clamped to 1e9, a force far beyond any pair's, for the fixed point)
         const float fs = ok ? f : 0.f;
-        const float gx = fs * (float)dx, gy = fs * (float)dy, gz = fs * (float)dz;
-        a.fx += (double)gx; a.fy += (double)gy; a.fz += (double)gz;
+        const double gx = (double)__builtin_amdgcn_fmed3f(fs * (float)dx, -1.0e9f, 1.0e9f);
+        const double gy = (double)__builtin_amdgcn_fmed3f(fs * (float)dy, -1.0e9f, 1.0e9f);
+        const double gz = (double)__builtin_amdgcn_fmed3f(fs * (float)dz, -1.0e9f, 1.0e9f);
+        a.fx += gx; a.fy += gy; a.fz += gz;
         a.pot += ok ? (double)p : 0.0;
-        a.vx = fma((double)gx, dx, a.vx); a.vy = fma((double)gy, dy, a.vy);
-        a.vz = fma((double)gz, dz, a.vz);
-        a.np += ok ? 1u : 0u;
-        if (ok) {
-          atomicAdd(&pacc[0][slot], to_fix(-gx));
-          atomicAdd(&pacc[1][slot], to_fix(-gy));
-          atomicAdd(&pacc[2][slot], to_fix(-gz));
+        a.vx = fma(gx, dx, a.vx); a.vy = fma(gy, dy, a.vy); a.vz = fma(gz, dz, a.vz);
+        npw += ok ? 1u : 0u;
+        if (ok) {  // +f d here; the flush adds its negation to the partner
+          atomicAdd(&pacc[0][slot], to_fix(gx));
+          atomicAdd(&pacc[1][slot], to_fix(gy));
+          atomicAdd(&pacc[2][slot], to_fix(gz));
         }
       } while (__ballot(cur != 0u));
+      a.np += npw;
       __syncthreads();
       // the window's partner shares into the box's fixed-point sums
       for (int s = lane; s < cnt; s += kRealThreads) {
@@ -754,7 +756,7 @@ __global__ __launch_bounds__(kRealThreads) void k_real_win(const BoxDesc* __rest
 #pragma unroll
         for (int q = 0; q < 3; ++q) {
           const unsigned long long v = pacc[q][s];
-          if (v) atomicAdd(&B.fix[q * (size_t)N + id], v);
+          if (v) atomicAdd(&B.fix[q * (size_t)N + id], 0ull - v);  // -f d, exact
         }
       }
       __syncthreads();
