@@ -207,6 +207,10 @@ int make_plan(const BoxSpec& sp, int precision, BoxPlan* P, std::string* why) {
     if (why) *why = "box needs N > 0, L > 0 and rc > 0";
     return MDG_E_ARG;
   }
+  if (sp.N > kMaxIons) {
+    if (why) *why = "at most 2^21 ions per box";
+    return MDG_E_RANGE;
+  }
   if (sp.kinds < 1 || sp.kinds > 4) {
     if (why) *why = "1..4 species supported";
     return MDG_E_ARG;

@@ -355,8 +355,10 @@ __global__ __launch_bounds__(kKfMfmaWaves* kWave, kf_min_waves<KB>()) void k_kfo
       for (int j = 0; j < 4; ++j) {
         cw[j] = __builtin_bit_cast(f4v_t, sa[KB * 2 * kWave + 4 * g + j]);
         const unsigned bits = __float_as_uint(cw[j].w);
-        xv[j] = gload<f2v_t>(X, (bits & 0xffu) * n8 + i8);
-        yv[j] = gload<f2v_t>(Y, ((bits >> 8) & 0xffu) * n8 + i8);
+        // 24-bit products (8 N < 2^24): full-rate v_mad_u32_u24, not a
+        // quarter-rate 64-bit multiply-add
+        xv[j] = gload<f2v_t>(X, __umul24(bits & 0xffu, n8) + i8);
+        yv[j] = gload<f2v_t>(Y, __umul24((bits >> 8) & 0xffu, n8) + i8);
       }
     }
     // the next tile's copy goes out after this tile's gathers, and always
@@ -507,29 +509,59 @@ __global__ __launch_bounds__(kSkmThreads) void k_sk_mfma(const BoxDesc* __restri
   // (large hmax) are loaded at the tile itself, kSkmXYBatch at a time.
   u4_t vss[es / kSkmThreads];
   f2v_t vpre[kSkmPre];
+  // Each thread's staging addresses are the same for every tile but for a
+  // uniform tile term, so they are formed once here: the record offsets
+  // (and the last record group they may read), the prefetched X/Y
+  // elements' offsets from the X table (the Y table lies a fixed distance
+  // after it) and their LDS slots.
+  constexpr int kRecs = es / kSkmThreads;
+  unsigned rec_off[kRecs];
+  int rec_grp[kRecs];
+#pragma unroll
+  for (int m = 0; m < kRecs; ++m) {
+    const int q = tid + m * kSkmThreads;
+    const int grp = q >> 7, rem = q & 127, hz = 16 * t0 + (rem >> 2);
+    rec_off[m] = 16u * (((unsigned)grp * (unsigned)HP + 16u * (unsigned)t0) * 4u + (unsigned)rem);
+    rec_grp[m] = hz < HP ? grp : 0x3fffffff;  // past the box's hz: never read
+  }
+  const unsigned xy_dist = (unsigned)((const char*)B.taby - (const char*)B.tabx);
+  const int col = tid % kSkmTile;  // every element of this thread is ion column col
+  unsigned xy_off[kSkmPre];
+  int xy_lds[kSkmPre];  // index into the LDS X rows (the Y rows follow them); -1: none
+#pragma unroll
+  for (int m = 0; m < kSkmPre; ++m) {
+    const int e = tid + m * kSkmThreads, tb = e >= exf, q = e - tb * exf;
+    const int row = q / kSkmTile;
+    xy_off[m] = (tb ? xy_dist : 0u) + 8u * ((unsigned)row * (unsigned)N + (unsigned)col);
+    xy_lds[m] = e < 2 * exf ? (tb ? (hl + 1) * kSkmRow : 0) + row * kSkmRow + col : -1;
+  }
   auto xy_elem = [&](int j0, int e) {
     const int tb = e >= exf, q = e - tb * exf, i = j0 + q % kSkmTile;
     f2v_t v = f2v_t{0.f, 0.f};
-    if (e < 2 * exf && i < N)
-      v = gload<f2v_t>(tb ? gyb : gxb, 8u * (unsigned)((size_t)(q / kSkmTile) * N + i));
+    if (e < 2 * exf && i < N)  // 24-bit row product (rows < 2^8, N < 2^24): full rate
+      v = gload<f2v_t>(tb ? gyb : gxb,
+                       8u * (__umul24((unsigned)(q / kSkmTile), (unsigned)N) + (unsigned)i));
     return v;
   };
   auto xy_store = [&](int e, f2v_t v) {
     const int tb = e >= exf, q = e - tb * exf;
-    if (e < 2 * exf) (tb ? ysf : xsf)[(q / kSkmTile) * kSkmRow + q % kSkmTile] = v;
+    if (e < 2 * exf)
+      (tb ? ysf : xsf)[__umul24((unsigned)(q / kSkmTile), (unsigned)kSkmRow) + q % kSkmTile] = v;
   };
   auto fetch = [&](int j0) {
+    const int g0 = j0 >> 2;                                // the tile's first record group
+    const unsigned rec_t = 64u * (unsigned)HP * (unsigned)g0;  // uniform
 #pragma unroll
-    for (int m = 0; m < es / kSkmThreads; ++m) {
-      const int q = tid + m * kSkmThreads;
-      const int grp = q >> 7, rem = q & 127, hz = 16 * t0 + (rem >> 2);
-      const int gg = (j0 >> 2) + grp;
+    for (int m = 0; m < kRecs; ++m) {
       vss[m] = u4_t{0u, 0u, 0u, 0u};
-      if (gg < ngrp && hz < HP)
-        vss[m] = gload<u4_t>(gsb, 16u * (unsigned)(((size_t)gg * HP + 16 * t0) * 4 + rem));
+      if (g0 + rec_grp[m] < ngrp) vss[m] = gload<u4_t>(gsb, rec_off[m] + rec_t);
     }
+    const bool ion_ok = j0 + col < N;
 #pragma unroll
-    for (int m = 0; m < kSkmPre; ++m) vpre[m] = xy_elem(j0, tid + m * kSkmThreads);
+    for (int m = 0; m < kSkmPre; ++m) {
+      vpre[m] = f2v_t{0.f, 0.f};
+      if (ion_ok && xy_lds[m] >= 0) vpre[m] = gload<f2v_t>(gxb, xy_off[m] + 8u * (unsigned)j0);
+    }
   };
   // the tile loop for NT hz tiles per item (a box with hmax < 16 has one:
   // no MFMAs on its all-zero second tile)
@@ -541,7 +573,8 @@ __global__ __launch_bounds__(kSkmThreads) void k_sk_mfma(const BoxDesc* __restri
 #pragma unroll
       for (int m = 0; m < es / kSkmThreads; ++m) ss[tid + m * kSkmThreads] = vss[m];
 #pragma unroll
-      for (int m = 0; m < kSkmPre; ++m) xy_store(tid + m * kSkmThreads, vpre[m]);
+      for (int m = 0; m < kSkmPre; ++m)
+        if (xy_lds[m] >= 0) xsf[xy_lds[m]] = vpre[m];
       for (int e0 = kSkmPre * kSkmThreads; e0 < 2 * exf; e0 += kSkmXYBatch * kSkmThreads) {
         f2v_t v[kSkmXYBatch];
 #pragma unroll

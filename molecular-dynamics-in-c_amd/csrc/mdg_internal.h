@@ -26,6 +26,11 @@ constexpr int kIonThreads = 256;    // per-ion streaming kernels
 constexpr int kCellThreads = 1024;  // cell counts and starts, one workgroup per box
 constexpr int kCellLds = 4096;      // cells whose counters live in LDS
 constexpr int kCellFusedN = 8192;   // boxes up to this size build their cell list in one workgroup
+// Ions per box at most.  With hmax <= 63 the kernels' 32-bit byte offsets
+// hold: the mixed k-space's Z.W records (16 B x up to 64 hz per ion,
+// < 2^31 in all) and its X and Y tables addressed from X (16 B x up to 65
+// rows per ion, < 2^32); and index products fit 24-bit multiplies (8 N < 2^24).
+constexpr int kMaxIons = 1 << 21;
 
 // A box whose whole cell list is built by k_cell_count's one workgroup (bin,
 // count, scan, scatter and placement in LDS): few enough ions for an LDS
