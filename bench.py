@@ -986,6 +986,11 @@ def main():
             log(f"  tuned {tune_rep['classes']} classes ({tune_rep['runs']} runs) "
                 f"in {tune_rep['seconds']:.1f} s")
     b = make_batch(boxes, states, mdgpu.MIXED)
+    # the split of the boxes between the step's two concurrent groups, timed
+    # on this batch (every box's state is restored afterwards)
+    sched_share = None if args.no_tune else b.tune_schedule(32)
+    if sched_share is not None:
+        log(f"  step schedule: group 0 share {sched_share:.2f}")
     b.prepare()            # the step graph is captured here, outside every timed step
     b.step(args.warmup)
     b.sync()
@@ -1107,6 +1112,9 @@ def main():
                                     float(np.median(np.abs(drift))),
                                     "max_abs": float(np.abs(drift).max()), "unit": "1e-18 J"},
         "autotune": tune_rep,
+        "step_schedule": {"group0_share": sched_share,
+                          "picked_by": "mdg_batch_tune_schedule (32 timed steps per split, "
+                                       "state restored)"},
         "mdc_step_loop_armed": step_loop,
         "hbm_copy_probe": probe,
         "configs": configs,

@@ -165,6 +165,16 @@ int  mdg_batch_step(mdg_batch *b, int nsteps);
  * or latency-bound stretch of steps that follows contains no capture.
  * Membership or state changes that rebuild the work lists discard it. */
 int  mdg_batch_prepare(mdg_batch *b);
+/* No counterpart in mdc: times nsteps steps of the batch under each of a
+ * few splits of its boxes between the step's two concurrent groups and
+ * keeps the fastest (the choice moves a 256-box step by several percent
+ * and depends on which boxes share a group).  Every box's device state is
+ * saved first and restored afterwards, so the trajectory, and every bit
+ * of it, is as if nothing had run; the step graph is then prepared.
+ * Needs device memory for one copy of the batch's boxes.  *share_out (may
+ * be NULL) receives the chosen share of the group that starts its
+ * k-space work at once. */
+int  mdg_batch_tune_schedule(mdg_batch *b, int nsteps, double *share_out);
 int  mdg_batch_scalars(mdg_batch *b, int box, mdg_scalars *out);
 int  mdg_batch_sync(mdg_batch *b);
 

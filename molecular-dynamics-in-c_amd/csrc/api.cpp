@@ -2,6 +2,8 @@
 #include <math.h>
 #include <string.h>
 
+#include <algorithm>
+#include <chrono>
 #include <string>
 #include <vector>
 
@@ -325,6 +327,61 @@ int mdg_batch_step(mdg_batch* b, int nsteps) {
 int mdg_batch_prepare(mdg_batch* b) {
   if (!b) return MDG_E_ARG;
   return b->set->prepare(b->stream);
+}
+
+int mdg_batch_tune_schedule(mdg_batch* b, int nsteps, double* share_out) {
+  if (!b || nsteps <= 0) return MDG_E_ARG;
+  for (Box* x : b->boxes)
+    if (!x->dmem) return MDG_E_ARG;
+  // every box's device memory is saved and put back, so the batch's state
+  // (and so its trajectory) is the same as if nothing had run
+  size_t total = 0;
+  for (Box* x : b->boxes) total += x->bytes;
+  void* save = nullptr;
+  if (hipMalloc(&save, std::max<size_t>(total, 1)) != hipSuccess) {
+    (void)hipGetLastError();
+    return MDG_E_NOMEM;
+  }
+  auto copy_all = [&](bool back) {
+    size_t off = 0;
+    for (Box* x : b->boxes) {
+      char* keep = (char*)save + off;
+      if (hipMemcpyAsync(back ? x->dmem : keep, back ? keep : x->dmem, x->bytes,
+                         hipMemcpyDeviceToDevice, b->stream) != hipSuccess)
+        return MDG_E_HIP;
+      off += x->bytes;
+    }
+    return hipStreamSynchronize(b->stream) == hipSuccess ? MDG_OK : MDG_E_HIP;
+  };
+  int rc = copy_all(false);
+  const double cand[] = {0.5, 0.4, 0.45, 0.55, 0.6, 0.65, 0.7};
+  double best = b->set->group_share(), best_ms = 1e30;
+  for (double c : cand) {
+    if (rc != MDG_OK) break;
+    b->set->set_group_share(c);
+    rc = b->set->set_boxes(b->boxes);
+    if (rc == MDG_OK) rc = b->set->prepare(b->stream);
+    if (rc == MDG_OK) rc = b->set->step(8, b->stream);  // warm
+    if (rc == MDG_OK && hipStreamSynchronize(b->stream) != hipSuccess) rc = MDG_E_HIP;
+    if (rc != MDG_OK) break;
+    const auto t0 = std::chrono::steady_clock::now();
+    rc = b->set->step(nsteps, b->stream);
+    if (rc == MDG_OK && hipStreamSynchronize(b->stream) != hipSuccess) rc = MDG_E_HIP;
+    const double ms =
+        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    if (rc == MDG_OK && ms < best_ms) {
+      best_ms = ms;
+      best = c;
+    }
+  }
+  if (rc == MDG_OK) rc = copy_all(true);
+  (void)hipFree(save);
+  b->set->set_group_share(best);
+  const int rc2 = b->set->set_boxes(b->boxes);
+  if (rc == MDG_OK) rc = rc2;
+  if (rc == MDG_OK) rc = b->set->prepare(b->stream);
+  if (rc == MDG_OK && share_out) *share_out = best;
+  return rc;
 }
 
 int mdg_batch_set_observe(mdg_batch* b, int box, const mdg_observe* o) {

@@ -895,10 +895,9 @@ int BoxSet::set_boxes(const std::vector<Box*>& boxes) {
   std::vector<int> group_of(nb, 0);
   {
     // group 0 starts its reciprocal chain at once, the others after their
-    // real space; kGroup0Share is group 0's share of the work (an equal
-    // split measured best on the 256-box batch)
-    std::vector<double> load(G, 0.0), share(G, G > 1 ? (1.0 - kGroup0Share) / (G - 1) : 1.0);
-    share[0] = G > 1 ? kGroup0Share : 1.0;
+    // real space; group0_share_ is group 0's share of the work
+    std::vector<double> load(G, 0.0), share(G, G > 1 ? (1.0 - group0_share_) / (G - 1) : 1.0);
+    share[0] = G > 1 ? group0_share_ : 1.0;
     for (int bi : order) {
       const BoxPlan& P = boxes_[bi]->plan;
       const double N = (double)boxes_[bi]->desc.N;

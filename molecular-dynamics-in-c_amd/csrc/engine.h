@@ -136,7 +136,9 @@ int box_write_observe(Box* b, const double* obs, const double* old, const double
 
 // Box groups of a batch whose step chains run concurrently (16+ boxes).
 constexpr int kStepGroups = 2;
-constexpr double kGroup0Share = 0.5;  // share of the work in the group that starts k-space first
+// default share of the work in the group that starts k-space first
+// (mdg_batch_tune_schedule picks one per batch by timing)
+constexpr double kGroup0Share = 0.5;
 
 // Steps per captured step graph (BoxSet::step replays it).
 constexpr int kGraphSteps = 8;
@@ -147,6 +149,10 @@ class BoxSet {
   explicit BoxSet(int precision, int real_mode);
   ~BoxSet();
   int set_boxes(const std::vector<Box*>& boxes);  // rebuild work lists
+  // Group 0's share of the work in the two-group step schedule (takes
+  // effect at the next set_boxes).
+  void set_group_share(double share) { group0_share_ = share; }
+  double group_share() const { return group0_share_; }
   // Device memory for the set's descriptors and work lists, owned by the
   // caller: set_boxes then allocates nothing (MDG_E_BUDGET when they do
   // not fit), and steps run on the caller's stream alone (no side streams
@@ -191,6 +197,7 @@ class BoxSet {
   Item* d_items_ = nullptr;
   int* d_ids_ = nullptr;
   size_t items_cap_ = 0, ids_cap_ = 0, desc_cap_ = 0;
+  double group0_share_ = kGroup0Share;
   char* arena_ = nullptr;   // caller-owned device memory (set_arena)
   size_t arena_bytes_ = 0;
   // offsets into d_items_ / d_ids_ and counts

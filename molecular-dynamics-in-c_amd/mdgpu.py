@@ -243,6 +243,7 @@ def lib():
             "mdg_batch_force_terms": ([vp, C.c_int], C.c_int),
             "mdg_batch_step": ([vp, C.c_int], C.c_int),
             "mdg_batch_prepare": ([vp], C.c_int),
+            "mdg_batch_tune_schedule": ([vp, C.c_int, _dp], C.c_int),
             "mdg_batch_scalars": ([vp, C.c_int, C.POINTER(Scalars)], C.c_int),
             "mdg_batch_sync": ([vp], C.c_int),
             "mdg_batch_profile": ([vp, C.c_int, C.c_int, C.c_void_p, _dp,
@@ -442,6 +443,13 @@ class Batch:
     def prepare(self):
         """Capture and upload the step graph without stepping."""
         check(lib().mdg_batch_prepare(self.h), "prepare")
+
+    def tune_schedule(self, nsteps=32):
+        """Pick the step schedule's group split by timing (state restored);
+        returns the chosen share."""
+        share = C.c_double()
+        check(lib().mdg_batch_tune_schedule(self.h, nsteps, C.byref(share)), "tune_schedule")
+        return share.value
 
     def sync(self):
         check(lib().mdg_batch_sync(self.h), "sync")

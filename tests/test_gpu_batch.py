@@ -102,6 +102,41 @@ def test_concurrent_box_groups_match_lone_boxes(gpu):
     mix.close()
 
 
+def test_tune_schedule_leaves_the_trajectory_unchanged(gpu):
+    """mdg_batch_tune_schedule steps the batch under several group splits
+    but restores every box's device state: right after it the states are
+    the same bits, and 24 steps later (graph replays under the chosen
+    split) they equal a batch that never tuned."""
+    ps, states = [], []
+    for k in range(20):
+        n = 2 + k % 4
+        p = _params(n, H.A_KCL if k % 3 == 0 else H.A_NACL,
+                    salt="KCl" if k % 3 == 0 else "NaCl")
+        ps.append(p)
+        states.append(_state(p, 300 + k, amp=0.2))
+    a, b = (mdgpu.Batch(ps, mdgpu.MIXED) for _ in range(2))
+    for x in (a, b):
+        for k, (r, v, sp) in enumerate(states):
+            x.set_state(k, r=r, v=v, species=sp)
+        x.step(16)
+    before = [a.state(k) for k in range(len(ps))]
+    share = a.tune_schedule(16)
+    assert 0.3 < share < 0.8
+    for k in range(len(ps)):
+        st = a.state(k)
+        for key in ("r", "v", "f", "f0"):
+            assert np.array_equal(st[key], before[k][key]), (k, key)
+    for x in (a, b):
+        x.step(24)
+    for k in range(len(ps)):
+        sa, sb = a.state(k), b.state(k)
+        for key in ("r", "v", "f", "f0"):
+            assert np.array_equal(sa[key], sb[key]), (k, key)
+        assert a.scalars(k)["step"] == b.scalars(k)["step"]
+    a.close()
+    b.close()
+
+
 def test_profile_times_the_step_schedule(gpu):
     """profile() times every launch of the step as the step schedules it:
     with 16 or more boxes the reciprocal chain runs once per box group, the
