@@ -493,8 +493,8 @@ __global__ __launch_bounds__(kRealThreads) void k_real(const BoxDesc* __restrict
 //  * A lane's candidates are bits of 32-bit words kept in LDS; the lane
 //    pops them in increasing partner order and applies the FP64
 //    minimum-image fold and the FP64 cutoff test, then the FP32 pair
-//    function.  The pair force f d is formed once in FP32; the home sums,
-//    energy and virial accumulate it in FP64.
+//    function.  The pair force f d is formed once, in FP64 from the FP32
+//    f/r; the home sums, energy and virial accumulate it in FP64.
 // The fold is x - L rint(x / L), which equals mdc's strict-comparison
 // fold (mdc/program/real.c:43-49) except for a separation of exactly L/2
 // on an axis, where both images are at the same distance.
@@ -586,6 +586,7 @@ __global__ __launch_bounds__(kRealThreads) void k_real_win(const BoxDesc* __rest
     screen_all = s * s >= 2.0e9;
     thr1 = screen_all ? 0 : (int)(s * s) + 1;
   }
+  const float fsmax = (float)(1.0e9 / sqrt(B.rc2));
   RealAcc tot;
   // partner source: all pairs -> index range; cells -> run of neighbour cells.
   // Partners before self_end (a run index) pair with the home unit's own
@@ -731,13 +732,11 @@ __global__ __launch_bounds__(kRealThreads) void k_real_win(const BoxDesc* __rest
           const float2 pf = row[bin];
           p = pf.x; f = pf.y;
         }
-        // the pair force f d once, in FP32; +f d on the home ion, -f d on
-        // the partner (exact fixed point)
-        // (clamped to 1e9, a force far beyond any pair's, for the fixed point)
-        const float fs = ok ? f : 0.f;
-        const double gx = (double)__builtin_amdgcn_fmed3f(fs * (float)dx, -1.0e9f, 1.0e9f);
-        const double gy = (double)__builtin_amdgcn_fmed3f(fs * (float)dy, -1.0e9f, 1.0e9f);
-        const double gz = (double)__builtin_amdgcn_fmed3f(fs * (float)dz, -1.0e9f, 1.0e9f);
+        // the pair force f d once: +f d on the home ion, -f d on the partner
+        // (exact fixed point); f/r clamped so that |f d| <= 1e9 (a force far
+        // beyond any pair's) for the fixed point
+        const double fs = (double)(ok ? __builtin_amdgcn_fmed3f(f, -fsmax, fsmax) : 0.f);
+        const double gx = fs * dx, gy = fs * dy, gz = fs * dz;
         a.fx += gx; a.fy += gy; a.fz += gz;
         a.pot += ok ? (double)p : 0.0;
         a.vx = fma(gx, dx, a.vx); a.vy = fma(gy, dy, a.vy); a.vz = fma(gz, dz, a.vz);
