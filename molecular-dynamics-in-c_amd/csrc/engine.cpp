@@ -207,6 +207,10 @@ int make_plan(const BoxSpec& sp, int precision, BoxPlan* P, std::string* why) {
     if (why) *why = "box needs N > 0, L > 0 and rc > 0";
     return MDG_E_ARG;
   }
+  if (precision == MDG_MIXED && sp.alpha > 0.0 && sp.alpha <= 10.0 && sp.alpha * sp.rc > 8.0) {
+    if (why) *why = "mixed real space: erfc fitted for alpha * rc <= 8 (Ewald accuracy >= 1e-27)";
+    return MDG_E_RANGE;
+  }
   if (sp.N > kMaxIons) {
     if (why) *why = "at most 2^21 ions per box";
     return MDG_E_RANGE;
