@@ -950,8 +950,10 @@ int BoxSet::set_boxes(const std::vector<Box*>& boxes) {
       }
     } else {
       I[ICell][g].push_back(bi);
-      if (!cell_fused(N, P.ncell))  // built in four launches: per-ion items
+      if (!cell_fused(N, P.ncell)) {  // built in four launches: per-ion and per-cell items
         for (int i0 = 0; i0 < N; i0 += kIonThreads) L[LCellIon][g].push_back({bi, i0, 0, 0, 0});
+        for (int c = 0; c < P.ncell; ++c) L[LCellPlace][g].push_back({bi, c, 0, 0, 0});
+      }
       int slot = 0;
       for (int c = 0; c < P.ncell; ++c)
         for (int hb = 0; hb < P.cell_hb; ++hb)
@@ -1114,7 +1116,8 @@ void BoxSet::launch_cells_real(hipStream_t s, int mode, const Ranges& r) {
   if (r.in[ICell])
     timed(KCell, s, [&] {
       launch_cell_build(d_desc_, d_ids_ + r.ioff[ICell], r.in[ICell],
-                        d_items_ + r.off[LCellIon], r.n[LCellIon], s);
+                        d_items_ + r.off[LCellIon], r.n[LCellIon], d_items_ + r.off[LCellPlace],
+                        r.n[LCellPlace], s);
     });
   timed(KReal, s, [&] {
     if (precision_ == MDG_MIXED)

@@ -202,7 +202,7 @@ class BoxSet {
   size_t arena_bytes_ = 0;
   // offsets into d_items_ / d_ids_ and counts
   // Work lists: item ranges in d_items_ per kernel, box-id ranges in d_ids_.
-  enum { LReal, LTab, LSk, LFin, LKf, LIon, LPack, LCellIon, kLists };
+  enum { LReal, LTab, LSk, LFin, LKf, LIon, LPack, LCellIon, LCellPlace, kLists };
   enum { ICell, IAll, IObs, kIdLists };
   struct Ranges {
     int off[kLists] = {}, n[kLists] = {};

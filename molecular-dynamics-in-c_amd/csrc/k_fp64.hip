@@ -19,14 +19,12 @@ int kforce_hz_for(int hmax) {
 }
 
 void launch_cell_build(const BoxDesc* boxes, const int* ids, int nids, const Item* items,
-                       int nitems, hipStream_t s) {
+                       int nitems, const Item* cells, int ncells, hipStream_t s) {
   if (nids <= 0) return;
   if (nitems > 0) MDG_LAUNCH(k_cell_bin, MDG_GRID(nitems, kIonThreads), boxes, items);
   MDG_LAUNCH(k_cell_count, MDG_GRID(nids, kCellThreads), boxes, ids);
-  if (nitems > 0) {
-    MDG_LAUNCH(k_cell_scatter, MDG_GRID(nitems, kIonThreads), boxes, items);
-    MDG_LAUNCH(k_cell_place, MDG_GRID(nitems, kIonThreads), boxes, items);
-  }
+  if (nitems > 0) MDG_LAUNCH(k_cell_scatter, MDG_GRID(nitems, kIonThreads), boxes, items);
+  if (ncells > 0) MDG_LAUNCH(k_cell_place, MDG_GRID(ncells, kWave), boxes, cells);
 }
 
 void launch_phase_tables(bool fp32, const BoxDesc* boxes, const Item* items, int n,

@@ -46,14 +46,14 @@ __device__ __forceinline__ void block_sum(double (&v)[NV]) {
 //   k_cell_bin     per ion: its cell;
 //   k_cell_count   per box: counts, cell_start (exclusive scan) and the
 //                  scatter's fill cursors;
-//   k_cell_scatter per ion: a slot of its cell by an atomic cursor; the
-//                  slots within a cell come in whatever order the atomics
-//                  took;
-//   k_cell_place   per slot: the ion goes to cell_start + the number of
-//                  ions of its cell with a lower index, and the
+//   k_cell_scatter per ion: a slot of its cell by an atomic cursor (one
+//                  atomic per cell and wave); the slots within a cell come
+//                  in whatever order the atomics took;
+//   k_cell_place   per cell: each of its ions goes to cell_start + the
+//                  number of ions of its cell with a lower index, and the
 //                  cell-ordered copies of positions and species are written.
-// The per-ion launches cover the ion items of the boxes that need them and
-// skip any other box.
+// The per-ion and per-cell launches cover the items of the boxes that need
+// them and skip any other box.
 // ----------------------------------------------------------------------
 __device__ __forceinline__ int cell_of_pos(const BoxDesc& B, double x, double y, double z,
                                            int& err) {
@@ -170,28 +170,56 @@ __global__ __launch_bounds__(kIonThreads) void k_cell_scatter(const BoxDesc* __r
                                                               const Item* __restrict__ items) {
   const Item it = items[blockIdx.x];
   const BoxDesc& B = boxes[it.box];
-  const int i = it.a + threadIdx.x;
-  if (B.all_pairs || cell_fused(B.N, B.ncell) || i >= B.N) return;
-  B.cell_tmp[atomicAdd(&B.cell_count[B.cell_of[i]], 1)] = i;
+  if (B.all_pairs || cell_fused(B.N, B.ncell)) return;  // uniform over the block
+  const int i = it.a + threadIdx.x, lane = threadIdx.x % kWave;
+  const bool act = i < B.N;
+  const int c = act ? B.cell_of[i] : -1;
+  // one atomic per distinct cell of the wave (ions of consecutive index
+  // often share a cell): the lowest lane of each cell group claims the
+  // group's slots, the others take theirs by rank within the group
+  const unsigned long long below = (1ull << lane) - 1ull;
+  unsigned long long todo = __ballot(act);
+  while (todo) {
+    const int leader = __ffsll((long long)todo) - 1;
+    const int cl = __shfl(c, leader, kWave);
+    const unsigned long long m = __ballot(c == cl);
+    int base = 0;
+    if (lane == leader) base = atomicAdd(&B.cell_count[cl], __popcll(m));
+    base = __shfl(base, leader, kWave);
+    if (c == cl) B.cell_tmp[base + __popcll(m & below)] = i;
+    todo &= ~m;
+  }
 }
 
-__global__ __launch_bounds__(kIonThreads) void k_cell_place(const BoxDesc* __restrict__ boxes,
-                                                            const Item* __restrict__ items) {
+// one wave per cell: the cell's slots staged in LDS (a cell of more than
+// kCellPlaceLds ions reads them from memory), each slot's ion placed at the
+// cell's start plus its rank by index among the cell's ions
+constexpr int kCellPlaceLds = 4096;
+__global__ __launch_bounds__(kWave) void k_cell_place(const BoxDesc* __restrict__ boxes,
+                                                      const Item* __restrict__ items) {
+  __shared__ int s_slot[kCellPlaceLds];
   const Item it = items[blockIdx.x];
   const BoxDesc& B = boxes[it.box];
-  const int p = it.a + threadIdx.x;
-  if (B.all_pairs || cell_fused(B.N, B.ncell) || p >= B.N) return;
-  // the slots of a cell are adjacent, so a wave's lanes mostly walk the
-  // same slots together
-  const int j = B.cell_tmp[p];
-  const int c = B.cell_of[j];
-  const int st = B.cell_start[c], e = B.cell_start[c + 1];
-  int rank = 0;
-  for (int q = st; q < e; ++q) rank += B.cell_tmp[q] < j ? 1 : 0;
-  const int d = st + rank;
-  B.sorted[d] = j;
-  B.sx[d] = B.rx[j]; B.sy[d] = B.ry[j]; B.sz[d] = B.rz[j];
-  B.styp[d] = B.typ[j];
+  if (B.all_pairs || cell_fused(B.N, B.ncell)) return;
+  const int lane = threadIdx.x;
+  const int st = B.cell_start[it.a], n = B.cell_start[it.a + 1] - st;
+  const bool lds = n <= kCellPlaceLds;
+  if (lds)
+    for (int k = lane; k < n; k += kWave) s_slot[k] = B.cell_tmp[st + k];
+  __syncthreads();
+  for (int k = lane; k < n; k += kWave) {
+    const int j = lds ? s_slot[k] : B.cell_tmp[st + k];
+    int rank = 0;
+    if (lds) {
+      for (int q = 0; q < n; ++q) rank += s_slot[q] < j ? 1 : 0;
+    } else {
+      for (int q = 0; q < n; ++q) rank += B.cell_tmp[st + q] < j ? 1 : 0;
+    }
+    const int d = st + rank;
+    B.sorted[d] = j;
+    B.sx[d] = B.rx[j]; B.sy[d] = B.ry[j]; B.sz[d] = B.rz[j];
+    B.styp[d] = B.typ[j];
+  }
 }
 
 // ----------------------------------------------------------------------

@@ -19,10 +19,11 @@ struct LaunchEvents {
 extern thread_local LaunchEvents g_launch_events;
 
 // k_fp64.hip (compiled with -ffp-contract=off)
-// ids: the boxes with cell lists; items: the per-ion items of those built
-// in four launches (not cell_fused; none: one launch does every box)
+// ids: the boxes with cell lists; items / cells: the per-ion and per-cell
+// items of those built in four launches (not cell_fused; none: one launch
+// does every box)
 void launch_cell_build(const BoxDesc* boxes, const int* ids, int nids, const Item* items,
-                       int nitems, hipStream_t s);
+                       int nitems, const Item* cells, int ncells, hipStream_t s);
 void launch_phase_tables(bool fp32, const BoxDesc* boxes, const Item* items, int n, hipStream_t s);
 void launch_real64(bool analytic, const BoxDesc* boxes, const Item* items, int n, int mode,
                    hipStream_t s);
