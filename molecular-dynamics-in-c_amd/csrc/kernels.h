@@ -96,12 +96,6 @@ struct RealAcc {
   unsigned long long np = 0;
 };
 
-// Pair-function constants of a home ion against partner species 0 and 1
-// (analytic FP32); selected per pair by the partner's species.
-struct Row32 {
-  float zz, cb, sig, c6, d8;
-};
-
 struct HomeCtx {
   int i, ti;
   bool valid;
@@ -111,13 +105,13 @@ struct HomeCtx {
   const float2* t32[4];
   double Zi;
   int ta[4];              // analytic64 pair type against partner species
-  Row32 row[2];
 };
 
 struct BoxCtx {
   double hx, hy, hz, Lx, Ly, Lz, rc2;
   int nb;
   float rc2f, alpha, two_a3, inv_rho, inv_rho_l2e, inv_rho_l2e_lo;
+  float gauss_l2;  // -alpha^2 log2(e): e^{-alpha^2 r^2} = 2^(gauss_l2 r^2)
   float hxf, hyf, hzf, Lxf, Lyf, Lzf;
   float rc2m;  // screening cutoff: rc^2 plus a margin above FP32 rounding
 };
@@ -132,16 +126,6 @@ __device__ __forceinline__ void make_home(const BoxDesc& B, int i, int ti, bool 
     if constexpr (KIND == kRealTable64) h.t64[s] = B.table + (size_t)t * B.table_nb;
     if constexpr (KIND == kRealTable32) h.t32[s] = B.table32 + (size_t)t * B.table_nb;
     h.ta[s] = t;
-  }
-  if constexpr (KIND == kRealAnalytic32) {
-    for (int s = 0; s < 2; ++s) {
-      const int t = B.lookup[ti * B.kinds + s];
-      h.row[s].zz = (float)(B.Z[ti] * B.Z[s]);
-      h.row[s].cb = (float)B.cb[t];
-      h.row[s].sig = (float)B.sig[t];
-      h.row[s].c6 = (float)B.Cd[t];
-      h.row[s].d8 = (float)B.Dd[t];
-    }
   }
 }
 
@@ -168,9 +152,15 @@ __device__ __forceinline__ void fold(const BoxCtx& c, double& dx, double& dy, do
 // denormal range; here the exponents are about -20 (Gaussian, (alpha rc)^2
 // log2 e at an Ewald accuracy of 1e-6) and -32 (Born-Mayer at 10 A), and a
 // result below 2^-126 would be flushed to a zero that is negligible anyway.
-__device__ __forceinline__ void pair32_fast(const BoxCtx& c, const Row32& w, double dr, float& p,
-                                            float& f) {
-  const float kLog2e = 1.4426950408889634f;
+// Pair constants of one (home species, partner species) combination, two
+// 16-byte LDS rows: zz = Z_i Z_j, cb, sigma, C, D of the BMH form, and the
+// force's 6 C and 8 D.
+struct alignas(16) PairRow {
+  float zz, cb, sig, c6, d8, c6x6, d8x8, pad;
+};
+
+__device__ __forceinline__ void pair32_fast(const BoxCtx& c, const PairRow& w, double dr,
+                                            float& p, float& f) {
   const float kLn2 = 0.6931471805599453f;
   const float r2 = (float)dr;
   const float r2lo = (float)(dr - (double)r2);
@@ -185,7 +175,8 @@ __device__ __forceinline__ void pair32_fast(const BoxCtx& c, const Row32& w, dou
   // relative error below 1e-7 for alpha r <= 8, i.e. every pair within the
   // cutoff down to an Ewald accuracy of 1e-27 (the plan refuses larger
   // alpha rc).  The Gaussian e^{-z^2} is the force's own, so one
-  // exponential serves both terms.
+  // exponential serves both terms; its argument -alpha^2 log2(e) r^2 comes
+  // from r^2 directly (one rounding fewer than squaring alpha r).
   const float t = __builtin_amdgcn_rcpf(fmaf(0.5f, z, 1.0f));
   float q = fmaf(t, -0.0588165745f, 0.283194035f);
   q = fmaf(t, q, -0.483994156f);
@@ -195,8 +186,7 @@ __device__ __forceinline__ void pair32_fast(const BoxCtx& c, const Row32& w, dou
   q = fmaf(t, q, 0.243530557f);
   q = fmaf(t, q, 0.282309562f);
   q = fmaf(t, q, 0.282090604f);
-  const float mz2 = -z * z;
-  const float g = __builtin_amdgcn_exp2f(mz2 * kLog2e);
+  const float g = __builtin_amdgcn_exp2f(c.gauss_l2 * r2);
   const float erfc_r = t * g * q * rinv;
   const float a = (w.sig - r) - rlo;
   const float x = a * c.inv_rho_l2e;
@@ -204,9 +194,10 @@ __device__ __forceinline__ void pair32_fast(const BoxCtx& c, const Row32& w, dou
   const float eb = w.cb * __builtin_amdgcn_exp2f(x) * fmaf(kLn2, xlo, 1.0f);
   const float r6 = rinv2 * rinv2 * rinv2;
   const float r8 = r6 * rinv2;
-  p = w.zz * erfc_r + eb - w.c6 * r6 - w.d8 * r8;
+  // C r^-6 + D r^-8 and its force 6 C r^-8 + 8 D r^-10
+  p = w.zz * erfc_r + eb - r6 * fmaf(w.d8, rinv2, w.c6);
   f = w.zz * (c.two_a3 * g + erfc_r) * rinv2 +
-      (eb * c.inv_rho * rinv - 6.0f * w.c6 * r8 - 8.0f * w.d8 * r8 * rinv2);
+      (eb * c.inv_rho * rinv - r8 * fmaf(w.d8x8, rinv2, w.c6x6));
 }
 
 // FP64 analytic pair function in the operation order of mdc's mk_table
@@ -539,6 +530,26 @@ __device__ __forceinline__ unsigned long long to_fix64(double x) {
   return to_fix(fmin(fmax(x, -1.0e9), 1.0e9));
 }
 
+// x * 2^32 rounded to the nearest integer, as a 64-bit two's complement
+// integer, for |x| < 2^19: fma(x, 2^32, 1.5 * 2^52) lies in (2^52, 2^53),
+// where FP64 numbers are the integers, so it is 1.5 * 2^52 + round(x 2^32)
+// exactly and its bit pattern is 0x4338000000000000 + round(x 2^32).  The
+// constant's low word is zero, so the subtraction touches the high word
+// only: one FMA and one integer subtraction per value.
+// (two32 = 2^32 and magic = 1.5 * 2^52, passed in registers: a VOP3 FMA
+// takes neither as an inline constant)
+__device__ __forceinline__ unsigned long long to_fix_near(double x, double two32, double magic) {
+  const double t = fma(x, two32, magic);
+  const unsigned long long b = (unsigned long long)__double_as_longlong(t);
+  const unsigned hi = (unsigned)(b >> 32) - 0x43380000u;
+  return ((unsigned long long)hi << 32) | (unsigned)b;
+}
+
+// Largest pair force component the partner shares carry: |f d| <= 2^18
+// (force units of 1e-8 N; a pair of ions at contact exerts about 1), so
+// to_fix_near's range holds with room to spare.
+constexpr double kPairForceMax = 262144.0;
+
 template <int KIND>
 __global__ __launch_bounds__(kRealThreads) void k_real_win(const BoxDesc* __restrict__ boxes,
                                                            const Item* __restrict__ items,
@@ -549,16 +560,34 @@ __global__ __launch_bounds__(kRealThreads) void k_real_win(const BoxDesc* __rest
   __shared__ unsigned pm[kRealWinWords][kRealThreads];
   __shared__ unsigned long long pacc[3][kRealWin];  // partner shares, fixed point
   __shared__ int ncum[28], nst[27];
+  __shared__ PairRow prow[4];  // by home species * 2 + partner species
   const Item it = items[blockIdx.x];
   const BoxDesc& B = boxes[it.box];
   const int lane = threadIdx.x;
   const int N = B.N, split = it.c, nsplit = B.rsplit;
   int err = 0;
+  if constexpr (KIND == kRealAnalytic32) {
+    if (lane < 4) {
+      const int ti = lane >> 1, tj = lane & 1;
+      const int t = B.lookup[min(ti, B.kinds - 1) * B.kinds + min(tj, B.kinds - 1)];
+      PairRow w;
+      w.zz = (float)(B.Z[ti] * B.Z[tj]);
+      w.cb = (float)B.cb[t];
+      w.sig = (float)B.sig[t];
+      w.c6 = (float)B.Cd[t];
+      w.d8 = (float)B.Dd[t];
+      w.c6x6 = (float)(6.0 * B.Cd[t]);
+      w.d8x8 = (float)(8.0 * B.Dd[t]);
+      w.pad = 0.f;
+      prow[lane] = w;
+    }
+  }
   BoxCtx c;
   c.Lx = B.L[0]; c.Ly = B.L[1]; c.Lz = B.L[2];
   c.rc2 = B.rc2;
   c.nb = B.table_nb;
   c.alpha = (float)B.alpha; c.two_a3 = (float)(2.0 * B.a3);
+  c.gauss_l2 = (float)(-B.alpha * B.alpha * 1.4426950408889634);
   c.inv_rho = (float)(1.0 / B.rho);
   const double irl = 1.4426950408889634 / B.rho;  // log2(e) / rho, split in two floats
   c.inv_rho_l2e = (float)irl;
@@ -574,7 +603,7 @@ __global__ __launch_bounds__(kRealThreads) void k_real_win(const BoxDesc* __rest
     screen_all = s * s >= 2.0e9;
     thr1 = screen_all ? 0 : (int)(s * s) + 1;
   }
-  const float fsmax = (float)(1.0e9 / sqrt(B.rc2));
+  const float fsmax = (float)(kPairForceMax / sqrt(B.rc2));
   RealAcc tot;
   // partner source: all pairs -> index range; cells -> run of neighbour cells.
   // Partners before self_end (a run index) pair with the home unit's own
@@ -627,13 +656,13 @@ __global__ __launch_bounds__(kRealThreads) void k_real_win(const BoxDesc* __rest
     else
       make_home<KIND>(B, B.sorted[hq], B.styp[hq], valid, B.sx[hq], B.sy[hq], B.sz[hq], h);
     const uint2 hqz = quantize(h.xi, h.yi, h.zi, sx, sy, sz);
-    if constexpr (KIND == kRealAnalytic32) {
-      // pin the FP32 pair constants in registers; otherwise the compiler
-      // re-derives them from the FP64 box constants inside the pair loop
-      for (int s = 0; s < 2; ++s)
-        __asm__ volatile("" : "+v"(h.row[s].zz), "+v"(h.row[s].cb), "+v"(h.row[s].sig),
-                         "+v"(h.row[s].c6), "+v"(h.row[s].d8));
-    }
+    // partner positions p (index or run slot) below self_lim pair with this
+    // lane's ion in the other order; self_max bounds them for the wave
+    const int self_lim = B.all_pairs ? hq + 1 : min(hq - hs + 1, self_end);
+    const int self_max = B.all_pairs ? he : self_end;
+    const PairRow* hrow = prow + 2 * (h.ti & 1);  // this lane's two rows
+    double fx_two32 = 4294967296.0, fx_magic = 6755399441055744.0;
+    __asm__ volatile("" : "+v"(fx_two32), "+v"(fx_magic));
     RealAcc a;
     for (int w0 = pbeg; w0 < pend; w0 += kRealWin) {
       const int cnt = min(kRealWin, pend - w0);
@@ -671,6 +700,13 @@ __global__ __launch_bounds__(kRealThreads) void k_real_win(const BoxDesc* __rest
         const int nv = min(32, cnt - w * 32);
         if (nv < 32) m &= ~(0xffffffffu >> nv);
         if (!valid) m = 0u;
+        // each unordered pair once: partners p < self_lim are the home
+        // unit's own ions at or below this lane's (by index; in a cell, by
+        // slot, the ascending index order within the cell)
+        if (w0 + 32 * w < self_max) {
+          const int cut = min(max(self_lim - (w0 + 32 * w), 0), 32);
+          m &= cut >= 32 ? 0u : (0xffffffffu >> cut);
+        }
         pm[w][lane] = m;
         nz |= (m != 0u ? 1u : 0u) << w;
       }
@@ -699,15 +735,14 @@ __global__ __launch_bounds__(kRealThreads) void k_real_win(const BoxDesc* __rest
         dx = fma(-c.Lx, rint(dx * ilx), dx);
         dy = fma(-c.Ly, rint(dy * ily), dy);
         dz = fma(-c.Lz, rint(dz * ilz), dz);
-        // each unordered pair once: by index within the home unit, all of
-        // the half shell's
-        bool ok = live && (P.id > h.i || w0 + slot >= self_end);
+        // (pairs with the home unit's lower ions are off the candidate words)
+        bool ok = live;
         float p, f;
         if constexpr (KIND == kRealAnalytic32) {
           const double dr = fma(dx, dx, fma(dy, dy, dz * dz));
           ok = ok && dr <= c.rc2;
           // a discarded pair may give inf or NaN here; it is selected away
-          pair32_fast(c, P.t == 0 ? h.row[0] : h.row[1], dr, p, f);
+          pair32_fast(c, hrow[P.t & 1], dr, p, f);
         } else {
           const double dr = dx * dx + dy * dy + dz * dz;
           ok = ok && dr <= c.rc2;
@@ -721,21 +756,26 @@ __global__ __launch_bounds__(kRealThreads) void k_real_win(const BoxDesc* __rest
           p = pf.x; f = pf.y;
         }
         // the pair force f d once: +f d on the home ion, -f d on the partner
-        // (exact fixed point); f/r clamped so that |f d| <= 1e9 (a force far
-        // beyond any pair's) for the fixed point
-        const double fs = (double)(ok ? __builtin_amdgcn_fmed3f(f, -fsmax, fsmax) : 0.f);
+        // (fixed point); f/r clamped so that |f d| <= kPairForceMax (a
+        // force far beyond any pair's) for the fixed point
+        // (the selects stay in FP32, ahead of the widening: the asm keeps
+        // the compiler from widening first and selecting both halves)
+        float fsel = ok ? __builtin_amdgcn_fmed3f(f, -fsmax, fsmax) : 0.f;
+        float psel = ok ? p : 0.f;
+        __asm__ volatile("" : "+v"(fsel), "+v"(psel));
+        const double fs = (double)fsel;
         const double gx = fs * dx, gy = fs * dy, gz = fs * dz;
         a.fx += gx; a.fy += gy; a.fz += gz;
-        a.pot += ok ? (double)p : 0.0;
+        a.pot += (double)psel;
         a.vx = fma(gx, dx, a.vx); a.vy = fma(gy, dy, a.vy); a.vz = fma(gz, dz, a.vz);
-        npw += ok ? 1u : 0u;
+        npw += (unsigned)__popcll(__ballot(ok));  // the wave's accepted pairs
         if (ok) {  // +f d here; the flush adds its negation to the partner
-          atomicAdd(&pacc[0][slot], to_fix(gx));
-          atomicAdd(&pacc[1][slot], to_fix(gy));
-          atomicAdd(&pacc[2][slot], to_fix(gz));
+          atomicAdd(&pacc[0][slot], to_fix_near(gx, fx_two32, fx_magic));
+          atomicAdd(&pacc[1][slot], to_fix_near(gy, fx_two32, fx_magic));
+          atomicAdd(&pacc[2][slot], to_fix_near(gz, fx_two32, fx_magic));
         }
       } while (__ballot(cur != 0u));
-      a.np += npw;
+      if (lane == 0) a.np += npw;
       __syncthreads();
       // the window's partner shares into the box's fixed-point sums
       for (int s = lane; s < cnt; s += kRealThreads) {

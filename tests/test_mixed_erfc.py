@@ -15,7 +15,7 @@ def _coefficients():
     """R(t)'s coefficients, highest degree first, as written in pair32_fast."""
     body = open(SRC).read()
     body = body[body.index("void pair32_fast"):]
-    body = body[:body.index("const float mz2")]
+    body = body[:body.index("const float g =")]
     first = re.search(r"float q = fmaf\(t, ([-0-9.e]+)f, ([-0-9.e]+)f\);", body)
     rest = re.findall(r"q = fmaf\(t, q, ([-0-9.e]+)f\);", body)
     return [float(first.group(1)), float(first.group(2))] + [float(x) for x in rest]
