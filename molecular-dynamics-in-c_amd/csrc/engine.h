@@ -208,7 +208,8 @@ class BoxSet {
     int off[kLists] = {}, n[kLists] = {};
     int ioff[kIdLists] = {}, in[kIdLists] = {};
   };
-  Ranges all_;              // every box (each list is its groups' lists back to back)
+  Ranges all_;              // every box (each list is its groups' lists back to back;
+                            // the real-space list is all in group 0's range, cost-ordered)
   std::vector<Ranges> grp_; // the box groups whose chains a step runs concurrently
   int skn_ = 1;  // hmax + 1 of the widest box (MFMA structure factor LDS tiles)
   int hs_ = 1, hsw_ = 8, hz_ = 8;  // LDS strides / register row of the k-space kernels
