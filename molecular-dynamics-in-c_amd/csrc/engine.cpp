@@ -1309,11 +1309,13 @@ void BoxSet::enqueue_step(hipStream_t s, bool vv) {
     launch_cells_real(s, 0, all_);
     if (all_.n[LSk] > 0) launch_recip(s, all_);
   }
-  timed(KVv, s,
-        [&] { launch_force_vv(vv, d_desc_, d_items_ + all_.off[LIon], all_.n[LIon], s); });
+  // the force assembly also forms the scalars' per-item partials
+  timed(KVv, s, [&] {
+    launch_force_vv(vv, d_desc_, d_items_ + all_.off[LIon], all_.n[LIon], s, 3, true);
+  });
   timed(KScal, s, [&] {
     launch_box_scalars(d_desc_, d_ids_ + all_.ioff[IAll], all_.in[IAll], d_items_ + all_.off[LIon],
-                       all_.n[LIon], vv ? 1 : 0, s);
+                       all_.n[LIon], vv ? 1 : 0, s, false);
   });
   // mdc's calc_msd, control_temp and moment_correction of the boxes that
   // have them armed (main.c:44-60)

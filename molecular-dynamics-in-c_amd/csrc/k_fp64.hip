@@ -72,10 +72,15 @@ void launch_sk_finalize(const BoxDesc* boxes, const Item* items, int n, hipStrea
 }
 
 void launch_force_vv(bool vv, const BoxDesc* boxes, const Item* items, int n, hipStream_t s,
-                     int terms) {
+                     int terms, bool scal) {
   if (n <= 0) return;
-  if (vv) MDG_LAUNCH(k_force_vv<1>, MDG_GRID(n, kIonThreads), boxes, items, terms);
-  else MDG_LAUNCH(k_force_vv<0>, MDG_GRID(n, kIonThreads), boxes, items, terms);
+  if (scal) {
+    if (vv) MDG_LAUNCH((k_force_vv<1, 1>), MDG_GRID(n, kIonThreads), boxes, items, terms);
+    else MDG_LAUNCH((k_force_vv<0, 1>), MDG_GRID(n, kIonThreads), boxes, items, terms);
+  } else {
+    if (vv) MDG_LAUNCH(k_force_vv<1>, MDG_GRID(n, kIonThreads), boxes, items, terms);
+    else MDG_LAUNCH(k_force_vv<0>, MDG_GRID(n, kIonThreads), boxes, items, terms);
+  }
 }
 
 void launch_clear_foc(const BoxDesc* boxes, const Item* items, int n, hipStream_t s) {
@@ -91,9 +96,9 @@ void launch_next_r(const BoxDesc* boxes, const Item* items, int n, hipStream_t s
 }
 
 void launch_box_scalars(const BoxDesc* boxes, const int* ids, int nids, const Item* items,
-                        int nitems, int flags, hipStream_t s) {
+                        int nitems, int flags, hipStream_t s, bool parts) {
   if (nids <= 0 || nitems <= 0) return;
-  MDG_LAUNCH(k_scal_part, MDG_GRID(nitems, kIonThreads), boxes, items);
+  if (parts) MDG_LAUNCH(k_scal_part, MDG_GRID(nitems, kIonThreads), boxes, items);
   MDG_LAUNCH(k_box_scalars, MDG_GRID(nids, kScalThreads), boxes, ids, flags);
 }
 

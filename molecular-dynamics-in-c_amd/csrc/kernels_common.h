@@ -284,6 +284,37 @@ __global__ __launch_bounds__(kIonThreads) void k_sk_finalize(const BoxDesc* __re
   if (threadIdx.x == 0) B.kpot[it.a / kIonThreads] = e[0];
 }
 
+// k_scal_part's work for one ion item (a whole workgroup): vx, vy, vz are
+// the velocities of this thread's ion (ignored past the box's end).
+__device__ __forceinline__ void scal_part_block(const BoxDesc& B, const Item& it, double vx,
+                                                double vy, double vz) {
+  const int tid = threadIdx.x, q = it.a / kIonThreads, nq = B.nspart;
+  double v[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  const int s0 = (int)((long)q * B.nreal / nq), s1 = (int)((long)(q + 1) * B.nreal / nq);
+  for (int s = s0 + tid; s < s1; s += kIonThreads) {
+    v[0] += B.rpart[4 * s];
+    v[1] += B.rpart[4 * s + 1];
+    v[2] += B.rpart[4 * s + 2];
+    v[3] += B.rpart[4 * s + 3];
+    v[8] += (double)B.rpairs[s];
+  }
+  if (B.kspace) {
+    const int nkb = (B.nk + kIonThreads - 1) / kIonThreads;
+    const int k0 = (int)((long)q * nkb / nq), k1 = (int)((long)(q + 1) * nkb / nq);
+    for (int k = k0 + tid; k < k1; k += kIonThreads) v[4] += B.kpot[k];
+  }
+  const int i = it.a + tid;
+  if (i < B.N) {
+    const double m = B.m[B.typ[i]];
+    v[5] = m * vx * vx;
+    v[6] = m * vy * vy;
+    v[7] = m * vz * vz;
+  }
+  block_sum<kIonThreads, 9>(v);
+  if (tid == 0)
+    for (int k = 0; k < 9; ++k) B.spart[9 * (size_t)q + k] = v[k];
+}
+
 // ----------------------------------------------------------------------
 // Force assembly: with split real space, f0 <- f and f <- sum of the
 // partials (split order); then the reciprocal partials are added (group
@@ -292,7 +323,10 @@ __global__ __launch_bounds__(kIonThreads) void k_sk_finalize(const BoxDesc* __re
 // ----------------------------------------------------------------------
 // terms: bit 0 = the real-space kernel ran (its split partials or f hold
 // the real-space force), bit 1 = add the reciprocal-space partials.
-template <int VV>
+// SCAL: the block then does k_scal_part's work for its ions (the kinetic
+// sums of the velocities just written, its share of the real-space and
+// reciprocal partials), one launch less per step; k_box_scalars follows.
+template <int VV, int SCAL = 0>
 __global__ __launch_bounds__(kIonThreads) void k_force_vv(const BoxDesc* __restrict__ boxes,
                                                           const Item* __restrict__ items,
                                                           int terms) {
@@ -300,62 +334,69 @@ __global__ __launch_bounds__(kIonThreads) void k_force_vv(const BoxDesc* __restr
   const BoxDesc& B = boxes[it.box];
   const int N = B.N;
   const int i = it.a + threadIdx.x;
-  if (i >= N) return;
-  double fx, fy, fz;
-  if (!(terms & 1) || B.fix) {
-    // no real space, or the mixed real space (all of it in the fixed-point
-    // array, added below)
-    fx = 0.0; fy = 0.0; fz = 0.0;
-    B.fx0[i] = B.fx[i]; B.fy0[i] = B.fy[i]; B.fz0[i] = B.fz[i];
-  } else if (B.rsplit > 1) {
-    fx = 0.0; fy = 0.0; fz = 0.0;
-    for (int s = 0; s < B.rsplit; ++s) {
-      const double* p = B.rpf + (size_t)s * 3 * N;
-      fx += p[i];
-      fy += p[N + i];
-      fz += p[2 * N + i];
+  if (!SCAL && i >= N) return;
+  double vx = 0.0, vy = 0.0, vz = 0.0;
+  if (i < N) {
+    double fx, fy, fz;
+    if (!(terms & 1) || B.fix) {
+      // no real space, or the mixed real space (all of it in the fixed-point
+      // array, added below)
+      fx = 0.0; fy = 0.0; fz = 0.0;
+      B.fx0[i] = B.fx[i]; B.fy0[i] = B.fy[i]; B.fz0[i] = B.fz[i];
+    } else if (B.rsplit > 1) {
+      fx = 0.0; fy = 0.0; fz = 0.0;
+      for (int s = 0; s < B.rsplit; ++s) {
+        const double* p = B.rpf + (size_t)s * 3 * N;
+        fx += p[i];
+        fy += p[N + i];
+        fz += p[2 * N + i];
+      }
+      B.fx0[i] = B.fx[i]; B.fy0[i] = B.fy[i]; B.fz0[i] = B.fz[i];
+    } else {
+      fx = B.fx[i]; fy = B.fy[i]; fz = B.fz[i];
     }
-    B.fx0[i] = B.fx[i]; B.fy0[i] = B.fy[i]; B.fz0[i] = B.fz[i];
-  } else {
-    fx = B.fx[i]; fy = B.fy[i]; fz = B.fz[i];
-  }
-  if (B.fix && (terms & 1)) {
-    // the mixed real space (home sums and partner shares), exact in fixed
-    // point; cleared for the next evaluation
-    const double s = 1.0 / kFixScale;
-    fx += (double)(long long)B.fix[i] * s;
-    fy += (double)(long long)B.fix[N + i] * s;
-    fz += (double)(long long)B.fix[2 * N + i] * s;
-    B.fix[i] = 0ull; B.fix[N + i] = 0ull; B.fix[2 * N + i] = 0ull;
-  }
-  if (B.kspace && (terms & 2)) {
-    for (int g = 0; g < B.ngroup; ++g) {
-      const double* fp = B.fpart + (size_t)g * 3 * N;
-      fx += fp[i];
-      fy += fp[N + i];
-      fz += fp[2 * N + i];
+    if (B.fix && (terms & 1)) {
+      // the mixed real space (home sums and partner shares), exact in fixed
+      // point; cleared for the next evaluation
+      const double s = 1.0 / kFixScale;
+      fx += (double)(long long)B.fix[i] * s;
+      fy += (double)(long long)B.fix[N + i] * s;
+      fz += (double)(long long)B.fix[2 * N + i] * s;
+      B.fix[i] = 0ull; B.fix[N + i] = 0ull; B.fix[2 * N + i] = 0ull;
+    }
+    if (B.kspace && (terms & 2)) {
+      for (int g = 0; g < B.ngroup; ++g) {
+        const double* fp = B.fpart + (size_t)g * 3 * N;
+        fx += fp[i];
+        fy += fp[N + i];
+        fz += fp[2 * N + i];
+      }
+    }
+    B.fx[i] = fx; B.fy[i] = fy; B.fz[i] = fz;
+    if (VV) {
+      const double m2 = 2.0 * B.m[B.typ[i]];
+      vx = B.vx[i] + B.dt * (fx + B.fx0[i]) / m2;
+      vy = B.vy[i] + B.dt * (fy + B.fy0[i]) / m2;
+      vz = B.vz[i] + B.dt * (fz + B.fz0[i]) / m2;
+      double rx = B.rx[i] + (B.dt * vx + B.dt2 * fx / m2);
+      double ry = B.ry[i] + (B.dt * vy + B.dt2 * fy / m2);
+      double rz = B.rz[i] + (B.dt * vz + B.dt2 * fz / m2);
+      if (rx > B.L[0]) rx = rx - B.L[0];
+      if (ry > B.L[1]) ry = ry - B.L[1];
+      if (rz > B.L[2]) rz = rz - B.L[2];
+      if (rx < 0.0) rx = B.L[0] + rx;
+      if (ry < 0.0) ry = B.L[1] + ry;
+      if (rz < 0.0) rz = B.L[2] + rz;
+      if (!(rx >= 0.0 && rx <= B.L[0] && ry >= 0.0 && ry <= B.L[1] && rz >= 0.0 &&
+            rz <= B.L[2]))
+        atomicOr(B.err, kErrState);
+      B.vx[i] = vx; B.vy[i] = vy; B.vz[i] = vz;
+      B.rx[i] = rx; B.ry[i] = ry; B.rz[i] = rz;
+    } else if (SCAL) {
+      vx = B.vx[i]; vy = B.vy[i]; vz = B.vz[i];
     }
   }
-  B.fx[i] = fx; B.fy[i] = fy; B.fz[i] = fz;
-  if (VV) {
-    const double m2 = 2.0 * B.m[B.typ[i]];
-    const double vx = B.vx[i] + B.dt * (fx + B.fx0[i]) / m2;
-    const double vy = B.vy[i] + B.dt * (fy + B.fy0[i]) / m2;
-    const double vz = B.vz[i] + B.dt * (fz + B.fz0[i]) / m2;
-    double rx = B.rx[i] + (B.dt * vx + B.dt2 * fx / m2);
-    double ry = B.ry[i] + (B.dt * vy + B.dt2 * fy / m2);
-    double rz = B.rz[i] + (B.dt * vz + B.dt2 * fz / m2);
-    if (rx > B.L[0]) rx = rx - B.L[0];
-    if (ry > B.L[1]) ry = ry - B.L[1];
-    if (rz > B.L[2]) rz = rz - B.L[2];
-    if (rx < 0.0) rx = B.L[0] + rx;
-    if (ry < 0.0) ry = B.L[1] + ry;
-    if (rz < 0.0) rz = B.L[2] + rz;
-    if (!(rx >= 0.0 && rx <= B.L[0] && ry >= 0.0 && ry <= B.L[1] && rz >= 0.0 && rz <= B.L[2]))
-      atomicOr(B.err, kErrState);
-    B.vx[i] = vx; B.vy[i] = vy; B.vz[i] = vz;
-    B.rx[i] = rx; B.ry[i] = ry; B.rz[i] = rz;
-  }
+  if (SCAL) scal_part_block(B, it, vx, vy, vz);
 }
 
 // mdc's clear_foc (init.c:179-190), next_v (rv.c:33-44), next_r (rv.c:46-66)
@@ -408,38 +449,18 @@ __global__ __launch_bounds__(kIonThreads) void k_next_r(const BoxDesc* __restric
 // reciprocal energy blocks, each a fixed-order workgroup sum.  k_box_scalars,
 // one workgroup per box, sums the nspart partials (fixed order) into
 // real-space pot and virial, reciprocal energy, kinetic energy as mdc's
-// calc_kin (rv.c:68-86) and pressure as calc_press (pt.c:8-21).
+// calc_kin (rv.c:68-86) and pressure as calc_press (pt.c:8-21).  A step
+// runs k_scal_part's work inside the force assembly (k_force_vv<VV, 1>).
 // flags bit0: count a completed step.
 // ----------------------------------------------------------------------
 __global__ __launch_bounds__(kIonThreads) void k_scal_part(const BoxDesc* __restrict__ boxes,
                                                            const Item* __restrict__ items) {
   const Item it = items[blockIdx.x];
   const BoxDesc& B = boxes[it.box];
-  const int tid = threadIdx.x, q = it.a / kIonThreads, nq = B.nspart;
-  double v[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-  const int s0 = (int)((long)q * B.nreal / nq), s1 = (int)((long)(q + 1) * B.nreal / nq);
-  for (int s = s0 + tid; s < s1; s += kIonThreads) {
-    v[0] += B.rpart[4 * s];
-    v[1] += B.rpart[4 * s + 1];
-    v[2] += B.rpart[4 * s + 2];
-    v[3] += B.rpart[4 * s + 3];
-    v[8] += (double)B.rpairs[s];
-  }
-  if (B.kspace) {
-    const int nkb = (B.nk + kIonThreads - 1) / kIonThreads;
-    const int k0 = (int)((long)q * nkb / nq), k1 = (int)((long)(q + 1) * nkb / nq);
-    for (int k = k0 + tid; k < k1; k += kIonThreads) v[4] += B.kpot[k];
-  }
-  const int i = it.a + tid;
-  if (i < B.N) {
-    const double m = B.m[B.typ[i]];
-    v[5] = m * B.vx[i] * B.vx[i];
-    v[6] = m * B.vy[i] * B.vy[i];
-    v[7] = m * B.vz[i] * B.vz[i];
-  }
-  block_sum<kIonThreads, 9>(v);
-  if (tid == 0)
-    for (int k = 0; k < 9; ++k) B.spart[9 * (size_t)q + k] = v[k];
+  const int i = it.a + threadIdx.x;
+  double vx = 0.0, vy = 0.0, vz = 0.0;
+  if (i < B.N) { vx = B.vx[i]; vy = B.vy[i]; vz = B.vz[i]; }
+  scal_part_block(B, it, vx, vy, vz);
 }
 
 __global__ __launch_bounds__(kScalThreads) void k_box_scalars(const BoxDesc* __restrict__ boxes,

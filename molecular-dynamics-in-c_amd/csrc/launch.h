@@ -33,14 +33,17 @@ void launch_sk64(const BoxDesc* boxes, const Item* items, int n, int hs, int hsw
 void launch_kforce64(int hz, const BoxDesc* boxes, const Item* items, int n, hipStream_t s);
 void launch_sk_finalize(const BoxDesc* boxes, const Item* items, int n, hipStream_t s);
 void launch_copy_probe(const void* src, void* dst, size_t n16, hipStream_t s);
+// scal: also k_scal_part's per-item partials (then launch_box_scalars
+// with parts = false finishes the box scalars)
 void launch_force_vv(bool vv, const BoxDesc* boxes, const Item* items, int n, hipStream_t s,
-                     int terms = 3);
+                     int terms = 3, bool scal = false);
 void launch_clear_foc(const BoxDesc* boxes, const Item* items, int n, hipStream_t s);
 void launch_next_v(const BoxDesc* boxes, const Item* items, int n, hipStream_t s);
 void launch_next_r(const BoxDesc* boxes, const Item* items, int n, hipStream_t s);
-// ids: the boxes; items: the per-ion items of exactly those boxes
+// ids: the boxes; items: the per-ion items of exactly those boxes; parts:
+// compute the per-item partials first (false when the force assembly did)
 void launch_box_scalars(const BoxDesc* boxes, const int* ids, int nids, const Item* items,
-                        int nitems, int flags, hipStream_t s);
+                        int nitems, int flags, hipStream_t s, bool parts = true);
 void launch_moment_correction(const BoxDesc* boxes, const int* ids, int n, hipStream_t s);
 // r, v, scalars and observables of every box into one staging buffer
 // (box b at dst + off[b]: [r 3N][v 3N][kScalCount][kObsLen] doubles)
