@@ -913,16 +913,34 @@ static double real_item_cost(const Box* B, const Item& it) {
   return blocks * (800.0 + std::ceil(partners / 128.0) * 880.0 + 148.0 * dens * partners);
 }
 
-// The real-space launch covers every box.  Its items are uneven (box sizes
-// and cutoffs differ) and few per wave slot, so the order sets the kernel's
+// Relative cost of a mixed structure-factor item (k_sk_mfma): its chunk's
+// ions times its hz tiles (one or two), plus a fixed part.
+static double sk_item_cost(const Box* B, const Item& it) {
+  const BoxPlan& P = B->plan;
+  const int N = B->desc.N, chunk = (N + P.tile.nchunk - 1) / std::max(1, P.tile.nchunk);
+  const int ions = std::max(0, std::min(chunk, N - it.a * chunk));
+  const int nt = (P.hmax + 16) / 16;
+  return (ions + 64.0) * std::min(2, nt - it.c);
+}
+
+// Relative cost of a mixed back-projection item (k_kforce_mfma): its tiles,
+// each three MFMAs per K block of the box and an epilogue, plus a fixed
+// part.
+static double kf_item_cost(const Box* B, const Item& it) {
+  return it.c * (3.0 * B->desc.kbk + 8.0) + 20.0;
+}
+
+// Launches over many boxes have uneven items (box sizes, cutoffs and k
+// counts differ) and few per wave slot, so the order sets the kernel's
 // tail: boxes are dealt to the eight XCDs by cost, heaviest first to the
 // least loaded (a box heavier than 1/16 of the launch is spread over all of
 // them), each XCD's items run heaviest first, and the launch order takes
 // the XCD queues in turn, the order in which workgroups are assigned to
-// XCDs.  A dispatch model of the 256-box batch put the slowest XCD's end
-// 1.16x above the ideal with the item-count balance of xcd_order and 1.03x
-// with this one.  No result depends on the order.
-static void xcd_cost_order(std::vector<Item>& v, const std::vector<Box*>& boxes) {
+// XCDs.  A dispatch model of the 256-box batch's real space put the slowest
+// XCD's end 1.16x above the ideal with the item-count balance of xcd_order
+// and 1.03x with this one.  No result depends on the order.
+template <typename Cost>
+static void xcd_cost_order(std::vector<Item>& v, const std::vector<Box*>& boxes, Cost cost) {
   constexpr int kXcds = 8;
   if (v.size() < 4 * kXcds) return;
   std::vector<int> first;
@@ -931,7 +949,7 @@ static void xcd_cost_order(std::vector<Item>& v, const std::vector<Box*>& boxes)
   for (const Item& it : v) {
     auto& q = by[it.box];
     if (q.empty()) first.push_back(it.box);
-    const double c = real_item_cost(boxes[it.box], it);
+    const double c = cost(boxes[it.box], it);
     q.push_back({c, it});
     total += c;
   }
@@ -1108,12 +1126,17 @@ int BoxSet::set_boxes(const std::vector<Box*>& boxes) {
       r0.insert(r0.end(), L[LReal][g].begin(), L[LReal][g].end());
       L[LReal][g].clear();
     }
-    xcd_cost_order(r0, boxes_);
+    xcd_cost_order(r0, boxes_, real_item_cost);
   }
   for (int l = 0; l < kLists; ++l) {
     all_.off[l] = (int)items.size();
     for (int g = 0; g < G; ++g) {
-      if (l != LReal) xcd_order(L[l][g]);
+      if (mixed && l == LSk)
+        xcd_cost_order(L[l][g], boxes_, sk_item_cost);
+      else if (mixed && l == LKf)
+        xcd_cost_order(L[l][g], boxes_, kf_item_cost);
+      else if (l != LReal)
+        xcd_order(L[l][g]);
       grp_[g].off[l] = (int)items.size();
       grp_[g].n[l] = (int)L[l][g].size();
       items.insert(items.end(), L[l][g].begin(), L[l][g].end());
