@@ -988,7 +988,7 @@ def main():
     b = make_batch(boxes, states, mdgpu.MIXED)
     # the split of the boxes between the step's two concurrent groups, timed
     # on this batch (every box's state is restored afterwards)
-    sched_share = None if args.no_tune else b.tune_schedule(32)
+    sched_share = None if args.no_tune else b.tune_schedule(64)
     if sched_share is not None:
         log(f"  step schedule: group 0 share {sched_share:.2f}")
     b.prepare()            # the step graph is captured here, outside every timed step
@@ -1113,8 +1113,8 @@ def main():
                                     "max_abs": float(np.abs(drift).max()), "unit": "1e-18 J"},
         "autotune": tune_rep,
         "step_schedule": {"group0_share": sched_share,
-                          "picked_by": "mdg_batch_tune_schedule (32 timed steps per split, "
-                                       "state restored)"},
+                          "picked_by": "mdg_batch_tune_schedule (each split timed twice "
+                                       "over 64 steps, state restored)"},
         "mdc_step_loop_armed": step_loop,
         "hbm_copy_probe": probe,
         "configs": configs,

@@ -167,8 +167,9 @@ int  mdg_batch_step(mdg_batch *b, int nsteps);
 int  mdg_batch_prepare(mdg_batch *b);
 /* No counterpart in mdc: times nsteps steps of the batch under each of a
  * few splits of its boxes between the step's two concurrent groups and
- * keeps the fastest (the choice moves a 256-box step by several percent
- * and depends on which boxes share a group).  Every box's device state is
+ * keeps the fastest, each timed twice, its faster time counting (the
+ * choice moves a 256-box step by several percent and depends on which
+ * boxes share a group).  Every box's device state is
  * saved first and restored afterwards, so the trajectory, and every bit
  * of it, is as if nothing had run; the step graph is then prepared.
  * Needs device memory for one copy of the batch's boxes.  *share_out (may
