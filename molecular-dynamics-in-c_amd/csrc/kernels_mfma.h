@@ -450,7 +450,10 @@ __host__ __device__ inline int sk_mfma_lds_bytes(int H) {
   return 2 * (H + 1) * kSkmRow * 8 + 8 * 32 * 4 * 16;
 }
 
-__global__ __launch_bounds__(kSkmThreads) void k_sk_mfma(const BoxDesc* __restrict__ boxes,
+// (four workgroups per CU: the compiler keeps the kernel in 128 registers, 116
+// without spills, for four waves per SIMD instead of three: pass A alone
+// 0.120 -> 0.109 ms on the 256-box batch)
+__global__ __launch_bounds__(kSkmThreads, 4) void k_sk_mfma(const BoxDesc* __restrict__ boxes,
                                                          const Item* __restrict__ items,
                                                          int hl) {
   extern __shared__ u4_t smem4[];
