@@ -900,7 +900,7 @@ def main():
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--plain-steps", type=int, default=20)
-    ap.add_argument("--drift-steps", type=int, default=2000)
+    ap.add_argument("--drift-steps", type=int, default=10000)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--cpu-cores", type=int, default=16,
                     help="processes of the all-core CPU baseline (capped by the usable CPUs)")
