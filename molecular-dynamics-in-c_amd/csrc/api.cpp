@@ -356,7 +356,7 @@ int mdg_batch_tune_schedule(mdg_batch* b, int nsteps, double* share_out) {
   int rc = copy_all(false);
   // two passes over the candidates, each candidate's faster time counting:
   // a single window of a few tens of steps can be disturbed
-  const double cand[] = {0.5, 0.4, 0.45, 0.55, 0.6, 0.65, 0.7};
+  const double cand[] = {0.5, 0.3, 0.35, 0.4, 0.45, 0.55, 0.6, 0.65, 0.7};
   constexpr int kCand = sizeof(cand) / sizeof(cand[0]);
   double cand_ms[kCand];
   for (double& m : cand_ms) m = 1e30;

@@ -361,6 +361,41 @@ int make_plan(const BoxSpec& sp, int precision, BoxPlan* P, std::string* why) {
   return MDG_OK;
 }
 
+// The mixed real space over all pairs visits a box's ions in the order
+// p -> (p s) mod N, s the integer nearest 0.618 N that is coprime with N:
+// consecutive positions are then spread over the box evenly (a rock-salt
+// lattice in mdc's order has neighbouring indices side by side), so every
+// 64-ion home block and every 128-partner window samples the whole box, and
+// the lanes' candidate counts in a window are close to one another.  On
+// the 256-box batch a count over the windows gives 84% of the pop loop's
+// lane slots doing accepted pairs in this order against 73% in index
+// order.  Which ion pairs are evaluated, and their forces, do not depend on
+// the order; the fixed-point sums are exact.  The real-space force of the
+// ion at position p accumulates at fix[p], so a window's partner shares and
+// a home block's sums land on consecutive words; the force assembly reads
+// ion i's at position i rstride^-1 mod N.
+static int real_stride(int n) {
+  if (n <= 2) return 1;
+  auto gcd = [](long a, long b) {
+    while (b) { const long t = a % b; a = b; b = t; }
+    return a;
+  };
+  int s = (int)std::lround(0.6180339887498949 * n);
+  while (gcd(s, n) != 1) ++s;
+  return s % n;
+}
+
+// s^-1 mod n for s coprime with n (extended Euclid); 1 for n <= 1
+static int inverse_mod(int s, int n) {
+  if (n <= 1) return 1;
+  long r0 = n, r1 = s % n, t0 = 0, t1 = 1;
+  while (r1) {
+    const long q = r0 / r1, r2 = r0 - q * r1, t2 = t0 - q * t1;
+    r0 = r1; r1 = r2; t0 = t1; t1 = t2;
+  }
+  return (int)(((t0 % n) + n) % n);
+}
+
 // ------------------------------------------------------------------ layout
 struct Layout {
   size_t ion[12], typ, rpf, fix, cell_of, cnt, ctmp, start, sorted, s[3], styp;
@@ -531,6 +566,8 @@ static int bind_box(Box* b, char* base, std::string* why) {
   D.kspace = P.kspace ? 1 : 0;
   D.real_mode = spec.real_mode;
   D.all_pairs = P.all_pairs ? 1 : 0;
+  D.rstride = P.mixed && P.all_pairs ? real_stride(N) : 1;
+  D.rstride_inv = inverse_mod(D.rstride, N);
   D.kinds = spec.kinds;
   for (int k = 0; k < 16; ++k) D.lookup[k] = spec.lookup[k];
   for (int d = 0; d < 3; ++d) {

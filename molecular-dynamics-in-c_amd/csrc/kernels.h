@@ -448,7 +448,9 @@ __global__ __launch_bounds__(kRealThreads) void k_real(const BoxDesc* __restrict
 // mdc/program/real.c:70-75).
 //
 // A home unit meets only the partners that follow it: all pairs, the ions
-// from its own first index on (j > i within it); cell lists, its own cell
+// from its own first position on in the box's visiting order (position p
+// is ion p rstride mod N, which spreads home blocks and windows over the
+// box; a later position within the unit); cell lists, its own cell
 // (j > i) and the 13 cells of the half shell.  The home ion's share of a
 // pair force is summed in its lane in FP64 over the item's partners, then
 // added to the box's 64-bit fixed-point array (x 2^32) with integer
@@ -604,6 +606,7 @@ __global__ __launch_bounds__(kRealThreads) void k_real_win(const BoxDesc* __rest
     thr1 = screen_all ? 0 : (int)(s * s) + 1;
   }
   const float fsmax = (float)(kPairForceMax / sqrt(B.rc2));
+  const double inv_n = 1.0 / (double)N;
   RealAcc tot;
   // partner source: all pairs -> index range; cells -> run of neighbour cells.
   // Partners before self_end (a run index) pair with the home unit's own
@@ -651,9 +654,12 @@ __global__ __launch_bounds__(kRealThreads) void k_real_win(const BoxDesc* __rest
     const bool valid = hp < he;
     const int hq = valid ? hp : hs;
     HomeCtx h;
-    if (B.all_pairs)
-      make_home<KIND>(B, hq, B.typ[hq], valid, B.rx[hq], B.ry[hq], B.rz[hq], h);
-    else
+    if (B.all_pairs) {
+      // the ion at position hq of the box's visiting order (p -> p rstride
+      // mod N); its force accumulates at fix[hq]
+      const int jh = mul_mod(hq, B.rstride, N, inv_n);
+      make_home<KIND>(B, hq, B.typ[jh], valid, B.rx[jh], B.ry[jh], B.rz[jh], h);
+    } else
       make_home<KIND>(B, B.sorted[hq], B.styp[hq], valid, B.sx[hq], B.sy[hq], B.sz[hq], h);
     const uint2 hqz = quantize(h.xi, h.yi, h.zi, sx, sy, sz);
     // partner positions p (index or run slot) below self_lim pair with this
@@ -672,8 +678,9 @@ __global__ __launch_bounds__(kRealThreads) void k_real_win(const BoxDesc* __rest
         double x, y, z;
         int t;
         if (B.all_pairs) {
-          j = w0 + s;
-          x = B.rx[j]; y = B.ry[j]; z = B.rz[j]; t = B.typ[j];
+          const int jp = mul_mod(w0 + s, B.rstride, N, inv_n);  // at position w0 + s
+          x = B.rx[jp]; y = B.ry[jp]; z = B.rz[jp]; t = B.typ[jp];
+          j = w0 + s;  // its force accumulates at fix[w0 + s]
         } else {
           const int g = w0 + s;
           int lo = 0, hi = 26;  // last run entry with ncum[n] <= g

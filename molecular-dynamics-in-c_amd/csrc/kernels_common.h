@@ -357,12 +357,14 @@ __global__ __launch_bounds__(kIonThreads) void k_force_vv(const BoxDesc* __restr
     }
     if (B.fix && (terms & 1)) {
       // the mixed real space (home sums and partner shares), exact in fixed
-      // point; cleared for the next evaluation
+      // point, at the ion's position in the real space's visiting order;
+      // cleared for the next evaluation
+      const int p = B.rstride == 1 ? i : mul_mod(i, B.rstride_inv, N, 1.0 / (double)N);
       const double s = 1.0 / kFixScale;
-      fx += (double)(long long)B.fix[i] * s;
-      fy += (double)(long long)B.fix[N + i] * s;
-      fz += (double)(long long)B.fix[2 * N + i] * s;
-      B.fix[i] = 0ull; B.fix[N + i] = 0ull; B.fix[2 * N + i] = 0ull;
+      fx += (double)(long long)B.fix[p] * s;
+      fy += (double)(long long)B.fix[N + p] * s;
+      fz += (double)(long long)B.fix[2 * N + p] * s;
+      B.fix[p] = 0ull; B.fix[N + p] = 0ull; B.fix[2 * N + p] = 0ull;
     }
     if (B.kspace && (terms & 2)) {
       for (int g = 0; g < B.ngroup; ++g) {

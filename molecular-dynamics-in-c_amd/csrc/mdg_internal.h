@@ -91,6 +91,16 @@ struct Item {
   int d;     // back-projection partial slot
 };
 
+// (a s) mod n for 0 <= a, s < 2^21: the product is exact in FP64 and so is
+// the remainder, an integer, formed by one FMA
+__device__ __forceinline__ int mul_mod(int a, int st, int n, double inv_n) {
+  const double x = (double)a * (double)st;
+  const double q = floor(x * inv_n);
+  double r = fma(-q, (double)n, x);
+  r = r < 0.0 ? r + n : (r >= (double)n ? r - n : r);
+  return (int)r;
+}
+
 struct BoxDesc {
   // ---- ion state (N each) ----
   double *rx, *ry, *rz, *vx, *vy, *vz, *fx, *fy, *fz, *fx0, *fy0, *fz0;
@@ -100,6 +110,10 @@ struct BoxDesc {
   int kspace;         // 1: ewald and at least one k vector (reciprocal work)
   int real_mode;      // 0 table, 1 analytic
   int all_pairs;      // 1: O(N^2) pair loop in mdc's order; 0: cell list
+  int rstride;        // mixed all pairs: the real space visits ions in the order
+                      // p -> (p * rstride) mod N (coprime with N; 1: index order)
+  int rstride_inv;    // its inverse mod N: ion i sits at position i * rstride_inv
+                      // mod N, the index of its real-space force in `fix`
   int kinds;          // species count (table lookup stride)
   int lookup[16];     // pair type of (ti, tj): lookup[ti*kinds + tj]
   // ---- geometry and Ewald constants ----
