@@ -153,20 +153,21 @@ __device__ __forceinline__ void fold(const BoxCtx& c, double& dx, double& dy, do
 // log2 e at an Ewald accuracy of 1e-6) and -32 (Born-Mayer at 10 A), and a
 // result below 2^-126 would be flushed to a zero that is negligible anyway.
 // Pair constants of one (home species, partner species) combination, two
-// 16-byte LDS rows: zz = Z_i Z_j, cb, sigma, C, D of the BMH form, and the
-// force's 6 C and 8 D.
+// 16-byte LDS rows: zz = Z_i Z_j, cb, sigma, C, D of the BMH form, the
+// force's 6 C and 8 D, and zz times the Gaussian's prefactor 2 alpha / sqrt(pi).
 struct alignas(16) PairRow {
-  float zz, cb, sig, c6, d8, c6x6, d8x8, pad;
+  float zz, cb, sig, c6, d8, c6x6, d8x8, zz2a3;
 };
 
 __device__ __forceinline__ void pair32_fast(const BoxCtx& c, const PairRow& w, double dr,
                                             float& p, float& f) {
   const float kLn2 = 0.6931471805599453f;
   const float r2 = (float)dr;
-  const float r2lo = (float)(dr - (double)r2);
   const float rinv = __builtin_amdgcn_rsqf(r2);
   const float r = r2 * rinv;
-  const float rlo = 0.5f * (fmaf(-r, r, r2) + r2lo) * rinv;  // r + rlo = sqrt(dr)
+  // r + rlo = sqrt(dr): dr - r^2 is exact in FP64 (r^2 has 48 bits)
+  const double rd = (double)r;
+  const float rlo = (float)fma(-rd, rd, dr) * (0.5f * rinv);
   const float rinv2 = rinv * rinv;
   const float z = c.alpha * r;
   // erfc(z) = t e^{-z^2} R(t), t = 1/(1 + z/2), with R(t) = erfc(z) e^{z^2} / t
@@ -195,9 +196,9 @@ __device__ __forceinline__ void pair32_fast(const BoxCtx& c, const PairRow& w, d
   const float r6 = rinv2 * rinv2 * rinv2;
   const float r8 = r6 * rinv2;
   // C r^-6 + D r^-8 and its force 6 C r^-8 + 8 D r^-10
-  p = w.zz * erfc_r + eb - r6 * fmaf(w.d8, rinv2, w.c6);
-  f = w.zz * (c.two_a3 * g + erfc_r) * rinv2 +
-      (eb * c.inv_rho * rinv - r8 * fmaf(w.d8x8, rinv2, w.c6x6));
+  const float ze = w.zz * erfc_r;
+  p = ze + eb - r6 * fmaf(w.d8, rinv2, w.c6);
+  f = fmaf(w.zz2a3, g, ze) * rinv2 + (eb * c.inv_rho * rinv - r8 * fmaf(w.d8x8, rinv2, w.c6x6));
 }
 
 // FP64 analytic pair function in the operation order of mdc's mk_table
@@ -580,7 +581,7 @@ __global__ __launch_bounds__(kRealThreads) void k_real_win(const BoxDesc* __rest
       w.d8 = (float)B.Dd[t];
       w.c6x6 = (float)(6.0 * B.Cd[t]);
       w.d8x8 = (float)(8.0 * B.Dd[t]);
-      w.pad = 0.f;
+      w.zz2a3 = (float)(B.Z[ti] * B.Z[tj] * 2.0 * B.a3);
       prow[lane] = w;
     }
   }
