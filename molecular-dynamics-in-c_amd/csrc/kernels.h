@@ -560,7 +560,9 @@ __global__ __launch_bounds__(kRealThreads) void k_real_win(const BoxDesc* __rest
   static_assert(KIND == kRealAnalytic32 || KIND == kRealTable32, "mixed kinds only");
   __shared__ WinRec pr[kRealWin];
   __shared__ uint2 pq[kRealWin];
-  __shared__ unsigned pm[kRealWinWords][kRealThreads];
+  // candidate words; word kRealWinWords is a zero sentinel that a lane out
+  // of words lands on
+  __shared__ unsigned pm[kRealWinWords + 1][kRealThreads];
   __shared__ unsigned long long pacc[3][kRealWin];  // partner shares, fixed point
   __shared__ int ncum[28], nst[27];
   __shared__ PairRow prow[4];  // by home species * 2 + partner species
@@ -569,6 +571,7 @@ __global__ __launch_bounds__(kRealThreads) void k_real_win(const BoxDesc* __rest
   const int lane = threadIdx.x;
   const int N = B.N, split = it.c, nsplit = B.rsplit;
   int err = 0;
+  pm[kRealWinWords][lane] = 0u;  // read back by this lane only
   if constexpr (KIND == kRealAnalytic32) {
     if (lane < 4) {
       const int ti = lane >> 1, tj = lane & 1;
@@ -597,6 +600,7 @@ __global__ __launch_bounds__(kRealThreads) void k_real_win(const BoxDesc* __rest
   c.inv_rho_l2e_lo = (float)(irl - (double)c.inv_rho_l2e);
   const double ilx = B.invL[0], ily = B.invL[1], ilz = B.invL[2];
   const double sx = 65536.0 * ilx, sy = 65536.0 * ily, sz = 65536.0 * ilz;
+  const double hlx = 0.5 * c.Lx, hly = 0.5 * c.Ly, hlz = 0.5 * c.Lz;
   int thr1;
   bool screen_all;
   {
@@ -663,6 +667,8 @@ __global__ __launch_bounds__(kRealThreads) void k_real_win(const BoxDesc* __rest
     } else
       make_home<KIND>(B, B.sorted[hq], B.styp[hq], valid, B.sx[hq], B.sy[hq], B.sz[hq], h);
     const uint2 hqz = quantize(h.xi, h.yi, h.zi, sx, sy, sz);
+    // the home ion in box units (x/L per axis)
+    const double hux = h.xi * ilx, huy = h.yi * ily, huz = h.zi * ilz;
     // partner positions p (index or run slot) below self_lim pair with this
     // lane's ion in the other order; self_max bounds them for the wave
     const int self_lim = B.all_pairs ? hq + 1 : min(hq - hs + 1, self_end);
@@ -692,8 +698,12 @@ __global__ __launch_bounds__(kRealThreads) void k_real_win(const BoxDesc* __rest
           const int jp = nst[lo] + (g - ncum[lo]);
           x = B.sx[jp]; y = B.sy[jp]; z = B.sz[jp]; t = B.styp[jp]; j = B.sorted[jp];
         }
+        // staged as 1/2 - x/L per axis: the pop loop folds home - partner
+        // in box units with one add and one fract
         WinRec r;
-        r.x = x; r.y = y; r.z = z; r.id = j; r.t = t;
+        r.x = 0.5 - x * ilx; r.y = 0.5 - y * ily; r.z = 0.5 - z * ilz; r.id = j;
+        // analytic: the byte offset of the partner species' pair row
+        r.t = KIND == kRealAnalytic32 ? (t & 1) * (int)sizeof(PairRow) : t;
         pr[s] = r;
         pq[s] = quantize(x, y, z, sx, sy, sz);
         pacc[0][s] = 0ull; pacc[1][s] = 0ull; pacc[2][s] = 0ull;
@@ -723,8 +733,8 @@ __global__ __launch_bounds__(kRealThreads) void k_real_win(const BoxDesc* __rest
       // candidates evaluates a discarded pair.  Energy and virial count each
       // evaluated (unordered) pair once.
       unsigned npw = 0u;  // accepted pairs of this window (32 bits suffice)
-      int w = nz ? __builtin_ctz(nz) : 0;
-      unsigned cur = nz ? pm[w][lane] : 0u;
+      int w = __builtin_ctz(nz | (1u << kRealWinWords));
+      unsigned cur = pm[w][lane];
       if (__ballot(cur != 0u)) do {
         const bool live = cur != 0u;
         const int b = __builtin_clz(cur | 1u);
@@ -736,13 +746,17 @@ __global__ __launch_bounds__(kRealThreads) void k_real_win(const BoxDesc* __rest
         const int slot = live ? w * 32 + b : 0;
         const WinRec P = pr[slot];
         if (live && cur == 0u) {
-          nz &= ~(1u << w);
-          if (nz) { w = __builtin_ctz(nz); cur = pm[w][lane]; }
+          // w is nz's lowest bit: drop it and move to the next non-empty
+          // word, or to the zero sentinel
+          nz &= nz - 1u;
+          w = __builtin_ctz(nz | (1u << kRealWinWords));
+          cur = pm[w][lane];
         }
-        double dx = h.xi - P.x, dy = h.yi - P.y, dz = h.zi - P.z;
-        dx = fma(-c.Lx, rint(dx * ilx), dx);
-        dy = fma(-c.Ly, rint(dy * ily), dy);
-        dz = fma(-c.Lz, rint(dz * ilz), dz);
+        // minimum image: u = x_i/L - x_j/L + 1/2 folded into [0, 1) by fract
+        // (x - floor x), then d = (u - 1/2) L in [-L/2, L/2)
+        const double dx = fma(__builtin_amdgcn_fract(hux + P.x), c.Lx, -hlx);
+        const double dy = fma(__builtin_amdgcn_fract(huy + P.y), c.Ly, -hly);
+        const double dz = fma(__builtin_amdgcn_fract(huz + P.z), c.Lz, -hlz);
         // (pairs with the home unit's lower ions are off the candidate words)
         bool ok = live;
         float p, f;
@@ -750,7 +764,8 @@ __global__ __launch_bounds__(kRealThreads) void k_real_win(const BoxDesc* __rest
           const double dr = fma(dx, dx, fma(dy, dy, dz * dz));
           ok = ok && dr <= c.rc2;
           // a discarded pair may give inf or NaN here; it is selected away
-          pair32_fast(c, hrow[P.t & 1], dr, p, f);
+          pair32_fast(c, *reinterpret_cast<const PairRow*>(
+                             reinterpret_cast<const char*>(hrow) + P.t), dr, p, f);
         } else {
           const double dr = dx * dx + dy * dy + dz * dz;
           ok = ok && dr <= c.rc2;
