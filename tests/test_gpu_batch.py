@@ -121,7 +121,7 @@ def test_tune_schedule_leaves_the_trajectory_unchanged(gpu):
         x.step(16)
     before = [a.state(k) for k in range(len(ps))]
     share = a.tune_schedule(16)
-    assert 0.3 < share < 0.8
+    assert 0.3 <= share <= 0.7  # the tuner's candidate splits
     for k in range(len(ps)):
         st = a.state(k)
         for key in ("r", "v", "f", "f0"):
