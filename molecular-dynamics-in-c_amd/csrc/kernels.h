@@ -110,7 +110,7 @@ struct HomeCtx {
 struct BoxCtx {
   double hx, hy, hz, Lx, Ly, Lz, rc2;
   int nb;
-  float rc2f, alpha, two_a3, inv_rho, inv_rho_l2e, inv_rho_l2e_lo;
+  float rc2f, alpha, two_a3, inv_rho, inv_rho_l2e;
   float gauss_l2;  // -alpha^2 log2(e): e^{-alpha^2 r^2} = 2^(gauss_l2 r^2)
   float hxf, hyf, hzf, Lxf, Lyf, Lzf;
   float rc2m;  // screening cutoff: rc^2 plus a margin above FP32 rounding
@@ -161,7 +161,6 @@ struct alignas(16) PairRow {
 
 __device__ __forceinline__ void pair32_fast(const BoxCtx& c, const PairRow& w, double dr,
                                             float& p, float& f) {
-  const float kLn2 = 0.6931471805599453f;
   const float r2 = (float)dr;
   const float rinv = __builtin_amdgcn_rsqf(r2);
   const float r = r2 * rinv;
@@ -189,10 +188,11 @@ __device__ __forceinline__ void pair32_fast(const BoxCtx& c, const PairRow& w, d
   q = fmaf(t, q, 0.282090604f);
   const float g = __builtin_amdgcn_exp2f(c.gauss_l2 * r2);
   const float erfc_r = t * g * q * rinv;
+  // Born-Mayer: 2^((sigma - r) log2(e) / rho); the FP32 exponent's rounding
+  // (relative 6e-8 of it) moves the term by 4e-8 |exponent|, which is large
+  // only where the term itself is negligible (exponent -32 at 10 A)
   const float a = (w.sig - r) - rlo;
-  const float x = a * c.inv_rho_l2e;
-  const float xlo = fmaf(a, c.inv_rho_l2e, -x) + a * c.inv_rho_l2e_lo;
-  const float eb = w.cb * __builtin_amdgcn_exp2f(x) * fmaf(kLn2, xlo, 1.0f);
+  const float eb = w.cb * __builtin_amdgcn_exp2f(a * c.inv_rho_l2e);
   const float r6 = rinv2 * rinv2 * rinv2;
   const float r8 = r6 * rinv2;
   // C r^-6 + D r^-8 and its force 6 C r^-8 + 8 D r^-10
@@ -355,9 +355,7 @@ __global__ __launch_bounds__(kRealThreads) void k_real(const BoxDesc* __restrict
   c.nb = B.table_nb;
   c.rc2f = (float)B.rc2; c.alpha = (float)B.alpha; c.two_a3 = (float)(2.0 * B.a3);
   c.inv_rho = (float)(1.0 / B.rho);
-  const double irl = 1.4426950408889634 / B.rho;  // log2(e) / rho, split in two floats
-  c.inv_rho_l2e = (float)irl;
-  c.inv_rho_l2e_lo = (float)(irl - (double)c.inv_rho_l2e);
+  c.inv_rho_l2e = (float)(1.4426950408889634 / B.rho);  // log2(e) / rho
   c.hxf = (float)c.hx; c.hyf = (float)c.hy; c.hzf = (float)c.hz;
   c.Lxf = (float)c.Lx; c.Lyf = (float)c.Ly; c.Lzf = (float)c.Lz;
   // FP32 screening coordinates are within ~1e-5 A: a relative margin of
@@ -595,9 +593,7 @@ __global__ __launch_bounds__(kRealThreads) void k_real_win(const BoxDesc* __rest
   c.alpha = (float)B.alpha; c.two_a3 = (float)(2.0 * B.a3);
   c.gauss_l2 = (float)(-B.alpha * B.alpha * 1.4426950408889634);
   c.inv_rho = (float)(1.0 / B.rho);
-  const double irl = 1.4426950408889634 / B.rho;  // log2(e) / rho, split in two floats
-  c.inv_rho_l2e = (float)irl;
-  c.inv_rho_l2e_lo = (float)(irl - (double)c.inv_rho_l2e);
+  c.inv_rho_l2e = (float)(1.4426950408889634 / B.rho);  // log2(e) / rho
   const double ilx = B.invL[0], ily = B.invL[1], ilz = B.invL[2];
   const double sx = 65536.0 * ilx, sy = 65536.0 * ily, sz = 65536.0 * ilz;
   const double hlx = 0.5 * c.Lx, hly = 0.5 * c.Ly, hlz = 0.5 * c.Lz;
