@@ -141,11 +141,11 @@ __device__ __forceinline__ void fold(const BoxCtx& c, double& dx, double& dy, do
 }
 
 // FP32 analytic pair function: -U'(r)/r and U(r) of erfc Coulomb + BMH.
-// dr is the FP64 squared distance.  r is rounded to FP32 but its residual
-// rlo is kept, and the Born-Mayer exponent (sigma - r) / rho is formed as
-// an unevaluated FP32 pair (x, xlo): the exponential multiplies r's
-// rounding error by r / rho (about 9 for alkali halides), which would
-// otherwise dominate the pair force error near contact.
+// dr is the FP64 squared distance, rounded to FP32.  The Born-Mayer
+// exponential multiplies r's rounding error by r / rho (about 9 for alkali
+// halides), so near contact that term errs by about 1e-6 of itself, ten
+// times the Coulomb term; the per-ion force error it leaves stays below
+// the reciprocal part's (measured with and without an FP64 residual of r).
 // The hardware reciprocal square root, reciprocal and base-2 exponential
 // (v_rsq_f32, v_rcp_f32, v_exp_f32, about 1 ulp) stand in for the
 // correctly rounded forms, which only add a rescaling for results in the
@@ -161,12 +161,10 @@ struct alignas(16) PairRow {
 
 __device__ __forceinline__ void pair32_fast(const BoxCtx& c, const PairRow& w, double dr,
                                             float& p, float& f) {
+  // r and 1/r in FP32 from v_rsq (about 1 ulp)
   const float r2 = (float)dr;
   const float rinv = __builtin_amdgcn_rsqf(r2);
   const float r = r2 * rinv;
-  // r + rlo = sqrt(dr): dr - r^2 is exact in FP64 (r^2 has 48 bits)
-  const double rd = (double)r;
-  const float rlo = (float)fma(-rd, rd, dr) * (0.5f * rinv);
   const float rinv2 = rinv * rinv;
   const float z = c.alpha * r;
   // erfc(z) = t e^{-z^2} R(t), t = 1/(1 + z/2), with R(t) = erfc(z) e^{z^2} / t
@@ -191,7 +189,7 @@ __device__ __forceinline__ void pair32_fast(const BoxCtx& c, const PairRow& w, d
   // Born-Mayer: 2^((sigma - r) log2(e) / rho); the FP32 exponent's rounding
   // (relative 6e-8 of it) moves the term by 4e-8 |exponent|, which is large
   // only where the term itself is negligible (exponent -32 at 10 A)
-  const float a = (w.sig - r) - rlo;
+  const float a = w.sig - r;
   const float eb = w.cb * __builtin_amdgcn_exp2f(a * c.inv_rho_l2e);
   const float r6 = rinv2 * rinv2 * rinv2;
   const float r8 = r6 * rinv2;
