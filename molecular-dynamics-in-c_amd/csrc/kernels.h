@@ -713,8 +713,9 @@ __global__ __launch_bounds__(kRealThreads) void k_real_win(const BoxDesc* __rest
         if (nv < 32) m &= ~(0xffffffffu >> nv);
         if (!valid) m = 0u;
         // each unordered pair once: partners p < self_lim are the home
-        // unit's own ions at or below this lane's (by index; in a cell, by
-        // slot, the ascending index order within the cell)
+        // unit's own ions at or below this lane's (all pairs: by position in
+        // the visiting order; in a cell, by slot, the ascending index order
+        // within the cell)
         if (w0 + 32 * w < self_max) {
           const int cut = min(max(self_lim - (w0 + 32 * w), 0), 32);
           m &= cut >= 32 ? 0u : (0xffffffffu >> cut);
