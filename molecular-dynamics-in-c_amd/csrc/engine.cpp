@@ -400,7 +400,7 @@ static int inverse_mod(int s, int n) {
 struct Layout {
   size_t ion[12], typ, rpf, fix, cell_of, cnt, ctmp, start, sorted, s[3], styp;
   size_t cols, segs, quads, khz, kslot, kce, kcf, kab, kab32, kpot, tx, ty, tw, skp, fp;
-  size_t wb, kfa, colw, kft, sb;
+  size_t wb, kfa, kft, sb;
   size_t tab, tab32, rpart, rpairs, spart, scal, err, msd_old, msd_d, obs, total;
 };
 
@@ -446,8 +446,7 @@ static Layout layout(const BoxPlan& P, int precision) {
     if (mixed) {
       const size_t nt = P.kftiles.size();
       L.wb = take(off, 32 * (size_t)(2 * P.kbk) * N);
-      L.kfa = take(off, 16 * (size_t)kWave * 2 * P.kbk * nt);
-      L.colw = take(off, sizeof(float4) * kKfTileCols * nt);
+      L.kfa = take(off, 16 * (size_t)kf_tile_stride(P.kbk) * nt);
       L.kft = take(off, sizeof(KfTile) * nt);
       const size_t npad = (size_t)(N + 7) / 8 * 8, hp = 16 * (size_t)((P.hmax + 16) / 16);
       L.sb = take(off, 16 * npad * hp);
@@ -639,7 +638,6 @@ static int bind_box(Box* b, char* base, std::string* why) {
     if (mixed) {
       D.wb = base + L.wb;
       D.kfa = base + L.kfa;
-      D.colw = (float4*)(base + L.colw);
       D.kft = (const KfTile*)(base + L.kft);
       D.kbk = P.kbk;
       D.ntile = (int)P.kftiles.size();

@@ -222,7 +222,8 @@ __global__ __launch_bounds__(kWave) void k_kf_pack(const BoxDesc* __restrict__ b
   const double sv = mv > 0.0 ? ldexp(1.0, 11 - ev) : 1.0;
   const bool isv = q >= 2;
   const double sc = isv ? sv : su;
-  u4_t* kfa = reinterpret_cast<u4_t*>(B.kfa);
+  // the tile's 16-byte units: its 8 column records, then its K blocks
+  u4_t* kfa = reinterpret_cast<u4_t*>(B.kfa) + (size_t)t * kf_tile_stride(B.kbk);
   for (int kb = 0; kb < B.kbk; ++kb) {
     _Float16 hv[8], lv[8];
 #pragma unroll
@@ -237,13 +238,13 @@ __global__ __launch_bounds__(kWave) void k_kf_pack(const BoxDesc* __restrict__ b
       }
       split_f16(v, hv[e], lv[e]);
     }
-    u4_t* dst = kfa + ((size_t)t * B.kbk + kb) * 2 * kWave + l;
+    u4_t* dst = kfa + kKfTileCols + (size_t)kb * 2 * kWave + l;
     dst[0] = pack_h8(hv);
     dst[kWave] = pack_h8(lv);
   }
   if (q == 0 && g == 0) {  // one lane per tile column ct, padding included
     const int bits = abs(col.hx) | (abs(col.hy) << 8);
-    B.colw[(size_t)t * kKfTileCols + (ct & 1) * 4 + (ct >> 1)] =
+    reinterpret_cast<float4*>(kfa)[(ct & 1) * 4 + (ct >> 1)] =
         cv ? make_float4((float)(col.hx / su), (float)(col.hy / su), (float)(1.0 / sv),
                          __int_as_float(bits))
            : make_float4(0.f, 0.f, 0.f, 0.f);
@@ -262,10 +263,10 @@ __global__ __launch_bounds__(kWave) void k_kf_pack(const BoxDesc* __restrict__ b
 // lane's 4 columns with Q = X[hx] Y[hy] gathered from the phase tables.
 // Tile sums in FP32, sums over tiles in FP64.
 // ----------------------------------------------------------------------
-// 16-byte units of one LDS tile: KB blocks of (hi, lo) x 64 lane records,
-// then the tile's column records
+// 16-byte units of one LDS tile: the tile's column records, then KB blocks
+// of (hi, lo) x 64 lane records
 template <int KB>
-constexpr int kf_tile_units() { return KB * 2 * kWave + kKfTileCols; }
+constexpr int kf_tile_units() { return kKfTileCols + KB * 2 * kWave; }
 
 // Workgroups per CU the back-projection is compiled for (its waves per
 // SIMD, one wave of each workgroup per SIMD): up to four K blocks the
@@ -308,32 +309,25 @@ __global__ __launch_bounds__(kKfMfmaWaves* kWave, kf_min_waves<KB>()) void k_kfo
   const gbytes kfa = (gbytes)B.kfa;
   const gbytes X = (gbytes)B.tabx;
   const gbytes Y = (gbytes)B.taby;
-  const gbytes colw = (gbytes)B.colw;
   const cptr<KfTile> tiles = as_const(B.kft);
   const unsigned n8 = 8u * (unsigned)N, i8 = 8u * (unsigned)ii;
-  // Staging of tile t: unit u < na is unit kb_lo * 128 + u of its packed
-  // blocks (contiguous in kfa), the next 8 are its column records.  Every
-  // thread issues exactly P loads (units past the tile reload a column
-  // record and are not stored), so the wait counts stay static.
+  const unsigned ut = 16u * (unsigned)kf_tile_stride(kbk);  // bytes per tile
+  // Staging of tile t: its first na units (the column records, then K
+  // blocks 0 .. kb_hi - 1) in one contiguous run.  Every thread issues
+  // exactly P loads (units past the run reload its last unit and are not
+  // stored), so the wait counts stay static.
   auto fetch = [&](int t, u4_t(&pf)[P]) {
-    const int klo = tiles[t].kb_lo, na = (tiles[t].kb_hi - klo) * 2 * kWave;
+    const int na = kKfTileCols + tiles[t].kb_hi * 2 * kWave;
 #pragma unroll
-    for (int m = 0; m < P; ++m) {
-      const int u = tid + NT * m;
-      pf[m] = u < na ? gload<u4_t>(kfa, 16u * (unsigned)((t * kbk + klo) * 2 * kWave + u))
-                     : gload<u4_t>(colw, 16u * (unsigned)(t * kKfTileCols +
-                                                          min(u - na, kKfTileCols - 1)));
-    }
+    for (int m = 0; m < P; ++m)
+      pf[m] = gload<u4_t>(kfa, (unsigned)t * ut + 16u * (unsigned)min(tid + NT * m, na - 1));
   };
   auto store = [&](int t, int buf, const u4_t(&pf)[P]) {
-    const int klo = tiles[t].kb_lo, na = (tiles[t].kb_hi - klo) * 2 * kWave;
+    const int na = kKfTileCols + tiles[t].kb_hi * 2 * kWave;
 #pragma unroll
     for (int m = 0; m < P; ++m) {
       const int u = tid + NT * m;
-      if (u < na)
-        lds[buf][klo * 2 * kWave + u] = pf[m];
-      else if (u < na + kKfTileCols)
-        lds[buf][KB * 2 * kWave + (u - na)] = pf[m];
+      if (u < na) lds[buf][u] = pf[m];
     }
   };
   double F0 = 0.0, F1 = 0.0, F2 = 0.0;
@@ -353,7 +347,7 @@ __global__ __launch_bounds__(kKfMfmaWaves* kWave, kf_min_waves<KB>()) void k_kfo
     if (act) {
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
-        cw[j] = __builtin_bit_cast(f4v_t, sa[KB * 2 * kWave + 4 * g + j]);
+        cw[j] = __builtin_bit_cast(f4v_t, sa[4 * g + j]);
         const unsigned bits = __float_as_uint(cw[j].w);
         // 24-bit products (8 N < 2^24): full-rate v_mad_u32_u24, not a
         // quarter-rate 64-bit multiply-add
@@ -374,8 +368,8 @@ __global__ __launch_bounds__(kKfMfmaWaves* kWave, kf_min_waves<KB>()) void k_kfo
 #pragma unroll
       for (int kb = 0; kb < KB; ++kb) {
         if (kb >= klo && kb < khi) {
-          const h8_t ah = __builtin_bit_cast(h8_t, sa[kb * 2 * kWave + l]);
-          const h8_t al = __builtin_bit_cast(h8_t, sa[kb * 2 * kWave + kWave + l]);
+          const h8_t ah = __builtin_bit_cast(h8_t, sa[kKfTileCols + kb * 2 * kWave + l]);
+          const h8_t al = __builtin_bit_cast(h8_t, sa[kKfTileCols + kb * 2 * kWave + kWave + l]);
           acc = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, bh[kb], acc, 0, 0, 0);
           acc = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, bl[kb], acc, 0, 0, 0);
           acc = __builtin_amdgcn_mfma_f32_32x32x16_f16(al, bh[kb], acc, 0, 0, 0);

@@ -78,6 +78,11 @@ struct KfTile {
   int c0, nc, kb_lo, kb_hi;
 };
 constexpr int kKfTileCols = 8;
+// 16-byte units of one tile in the packed operand buffer: its column
+// records, then kbk K blocks of (hi, lo) x 64 lane records.  A tile's K
+// blocks start at block 0 (every column's hz range starts at 0 or 1), so the
+// blocks it uses follow its records in one run.
+__host__ __device__ inline int kf_tile_stride(int kbk) { return kKfTileCols + kbk * 2 * 64; }
 constexpr int kKfMfmaWaves = 4;     // waves per back-projection workgroup
 constexpr int kSkmWaves = 4;        // waves per MFMA structure-factor workgroup (32 columns each)
 constexpr int kSkmChunk = 512;      // mixed structure factor: ions per FP32 partial sum
@@ -172,14 +177,13 @@ struct BoxDesc {
   // v_mfma_f32_32x32x16_f16: [hz/4][N] records of 8 hi then 8 lo halves
   // (re, im of 4 consecutive hz), hz < 8 * kbk, zero beyond hmax.
   void *wb;
-  // Packed A operands per tile and K block: [ntile][kbk][hi, lo][64 lanes]
-  // 16-byte lane records; rows are (column, {Re U, Im U, Re V, Im V}).
+  // Per tile (kf_tile_stride(kbk) 16-byte units): its epilogue records in
+  // lane order [half g][j], column 2 j + g as (hx/sU, hy/sU, 1/sV,
+  // |hx| | |hy| << 8), all zero for the padding columns of a short tile (the
+  // signs of hx and hy are those of the first two fields); then its packed
+  // A operands per K block, [kbk][hi, lo][64 lanes] 16-byte lane records,
+  // rows (column, {Re U, Im U, Re V, Im V}).
   void *kfa;
-  // Epilogue records per tile, in lane order [ntile][half g][j]: column
-  // 2 j + g as (hx/sU, hy/sU, 1/sV, |hx| | |hy| << 8); all zero for the
-  // padding columns of a short tile.  The signs of hx and hy are those of
-  // the first two fields.
-  float4 *colw;
   const KfTile *kft;  // [ntile]
   int kbk, ntile;
   // ---- structure factor on MFMA (mixed path) ----
