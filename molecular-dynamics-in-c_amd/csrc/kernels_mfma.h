@@ -224,7 +224,8 @@ __global__ __launch_bounds__(kWave) void k_kf_pack(const BoxDesc* __restrict__ b
   const double sc = isv ? sv : su;
   // the tile's 16-byte units: its 8 column records, then its K blocks
   u4_t* kfa = reinterpret_cast<u4_t*>(B.kfa) + (size_t)t * kf_tile_stride(B.kbk);
-  for (int kb = 0; kb < B.kbk; ++kb) {
+  // blocks past the tile's hz span are never staged: not written
+  for (int kb = 0; kb < tl.kb_hi; ++kb) {
     _Float16 hv[8], lv[8];
 #pragma unroll
     for (int e = 0; e < 8; ++e) {
