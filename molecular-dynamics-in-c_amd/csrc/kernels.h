@@ -110,7 +110,7 @@ struct HomeCtx {
 struct BoxCtx {
   double hx, hy, hz, Lx, Ly, Lz, rc2;
   int nb;
-  float rc2f, alpha, two_a3, inv_rho, inv_rho_l2e;
+  float rc2f, alpha, half_alpha, two_a3, inv_rho, inv_rho_l2e;
   float gauss_l2;  // -alpha^2 log2(e): e^{-alpha^2 r^2} = 2^(gauss_l2 r^2)
   float hxf, hyf, hzf, Lxf, Lyf, Lzf;
   float rc2m;  // screening cutoff: rc^2 plus a margin above FP32 rounding
@@ -153,10 +153,10 @@ __device__ __forceinline__ void fold(const BoxCtx& c, double& dx, double& dy, do
 // log2 e at an Ewald accuracy of 1e-6) and -32 (Born-Mayer at 10 A), and a
 // result below 2^-126 would be flushed to a zero that is negligible anyway.
 // Pair constants of one (home species, partner species) combination, two
-// 16-byte LDS rows: zz = Z_i Z_j, cb, sigma, C, D of the BMH form, the
+// 16-byte LDS rows: zz = Z_i Z_j, cb, sigma log2(e) / rho, C, D of the BMH form, the
 // force's 6 C and 8 D, and zz times the Gaussian's prefactor 2 alpha / sqrt(pi).
 struct alignas(16) PairRow {
-  float zz, cb, sig, c6, d8, c6x6, d8x8, zz2a3;
+  float zz, cb, sig_l2e, c6, d8, c6x6, d8x8, zz2a3;
 };
 
 __device__ __forceinline__ void pair32_fast(const BoxCtx& c, const PairRow& w, double dr,
@@ -166,7 +166,6 @@ __device__ __forceinline__ void pair32_fast(const BoxCtx& c, const PairRow& w, d
   const float rinv = __builtin_amdgcn_rsqf(r2);
   const float r = r2 * rinv;
   const float rinv2 = rinv * rinv;
-  const float z = c.alpha * r;
   // erfc(z) = t e^{-z^2} R(t), t = 1/(1 + z/2), with R(t) = erfc(z) e^{z^2} / t
   // as a degree-8 polynomial fitted for this library (relative least squares
   // on 4000 Chebyshev nodes of z in [0, 8], coefficients rounded to FP32):
@@ -175,7 +174,7 @@ __device__ __forceinline__ void pair32_fast(const BoxCtx& c, const PairRow& w, d
   // alpha rc).  The Gaussian e^{-z^2} is the force's own, so one
   // exponential serves both terms; its argument -alpha^2 log2(e) r^2 comes
   // from r^2 directly (one rounding fewer than squaring alpha r).
-  const float t = __builtin_amdgcn_rcpf(fmaf(0.5f, z, 1.0f));
+  const float t = __builtin_amdgcn_rcpf(fmaf(c.half_alpha, r, 1.0f));  // z = alpha r
   float q = fmaf(t, -0.0588165745f, 0.283194035f);
   q = fmaf(t, q, -0.483994156f);
   q = fmaf(t, q, 0.273955554f);
@@ -186,11 +185,11 @@ __device__ __forceinline__ void pair32_fast(const BoxCtx& c, const PairRow& w, d
   q = fmaf(t, q, 0.282090604f);
   const float g = __builtin_amdgcn_exp2f(c.gauss_l2 * r2);
   const float erfc_r = t * g * q * rinv;
-  // Born-Mayer: 2^((sigma - r) log2(e) / rho); the FP32 exponent's rounding
-  // (relative 6e-8 of it) moves the term by 4e-8 |exponent|, which is large
-  // only where the term itself is negligible (exponent -32 at 10 A)
-  const float a = w.sig - r;
-  const float eb = w.cb * __builtin_amdgcn_exp2f(a * c.inv_rho_l2e);
+  // Born-Mayer: 2^((sigma - r) log2(e) / rho) by one FMA from the row's
+  // sigma log2(e) / rho; the FP32 exponent's rounding (about 5e-7 near
+  // contact, below r's own) moves the term by 4e-8 |exponent| elsewhere,
+  // large only where the term itself is negligible (exponent -32 at 10 A)
+  const float eb = w.cb * __builtin_amdgcn_exp2f(fmaf(-r, c.inv_rho_l2e, w.sig_l2e));
   const float r6 = rinv2 * rinv2 * rinv2;
   const float r8 = r6 * rinv2;
   // C r^-6 + D r^-8 and its force 6 C r^-8 + 8 D r^-10
@@ -575,7 +574,7 @@ __global__ __launch_bounds__(kRealThreads) void k_real_win(const BoxDesc* __rest
       PairRow w;
       w.zz = (float)(B.Z[ti] * B.Z[tj]);
       w.cb = (float)B.cb[t];
-      w.sig = (float)B.sig[t];
+      w.sig_l2e = (float)(B.sig[t] * 1.4426950408889634 / B.rho);
       w.c6 = (float)B.Cd[t];
       w.d8 = (float)B.Dd[t];
       w.c6x6 = (float)(6.0 * B.Cd[t]);
@@ -589,6 +588,7 @@ __global__ __launch_bounds__(kRealThreads) void k_real_win(const BoxDesc* __rest
   c.rc2 = B.rc2;
   c.nb = B.table_nb;
   c.alpha = (float)B.alpha; c.two_a3 = (float)(2.0 * B.a3);
+  c.half_alpha = (float)(0.5 * B.alpha);
   c.gauss_l2 = (float)(-B.alpha * B.alpha * 1.4426950408889634);
   c.inv_rho = (float)(1.0 / B.rho);
   c.inv_rho_l2e = (float)(1.4426950408889634 / B.rho);  // log2(e) / rho
